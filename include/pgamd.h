@@ -1,0 +1,94 @@
+/* Entry points of the MI355X libraries that go beyond the reference API:
+ * device-pointer variants of the force calls (inputs already in HBM, results
+ * left in HBM, asynchronous on a caller-chosen HIP stream), launch-shape
+ * control for the autotuner, and error reporting.  None of these has a
+ * counterpart in the reference (a CPU library whose data always lives in
+ * host memory); each names the reference call whose work it performs.
+ * Stream arguments are hipStream_t passed as void*; nullptr = the board's
+ * own stream.  All pointers named d_* or documented "device" are device
+ * pointers.  Returns: 0 on success, -1 after an API error.
+ */
+#ifndef PGAMD_H
+#define PGAMD_H
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ every library
+ * pgamd-library: pg5 pg5_sym pg5_table pg6 */
+/* 1 (default): API misuse prints a message and aborts, like the reference's
+ * assert()s; 0: the call returns without effect and records the error. */
+void pgamd_set_abort_on_error(int flag);
+/* Returns and clears the last recorded error code (0 = none); copies its message. */
+int pgamd_last_error(char *buf, int buflen);
+
+/* ------------------------------------------------------------------ libpg5.so
+ * pgamd-library: pg5 pg5_sym */
+/* Work of g5_calculate_force_on_xMC (PG5/newton/libpg5/phantom_g5.c:276):
+ * xi[ni] float4 {x,y,z,eps_i^2}, xj[nj] float4 {x,y,z,m}, epsj2[nj] (symmetric
+ * build only), acc[ni][3] and pot[ni] FP64 with pot = + sum m/r. */
+int pg5_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
+                  const float *epsj2, double *acc, double *pot, void *stream);
+/* Work of g5c_calculate_force_on_xMC (PG5/newton_quad/quad/quad_g5c.c:285):
+ * cells[nc] = 12 floats {x,y,z,M, Qxx,Qxy,Qxz,Qyy, Qyz,Qzz,0,0};
+ * pot = + sum (M/r + d.Q.d/(2 r^5)). */
+int pg5_dev_quad(int devid, int ni, const void *xi, int nc, const void *cells,
+                 double *acc, double *pot, void *stream);
+void *pg5_dev_stream(int devid);
+/* bi in {0 = automatic, 64, 128, 256} i-particles per block; blocks_per_cu and
+ * min_jper (> 0) set the j-range split. */
+int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
+
+/* ------------------------------------------------------------------ libpg5_table.so
+ * pgamd-library: pg5_table */
+/* Table resolution 2^frc_bit entries per octave of s (reference FRC_BIT = 5,
+ * compile-time, PG5/table/libpg5/pg5_table.h:7-15); rebuilds the table. */
+int pg5_set_table_frc_bit(int frc_bit);
+/* Copies the current table (2*ntab floats) if out != NULL; returns ntab. */
+int pg5_get_table(float *out, double *xscale, float *r2scale);
+/* Work of g5_calculate_force_on_xMC (PG5/table/libpg5/phantom_g5.c:160) on
+ * scaled device positions: xi[ni] float4 {x*xs,y*xs,z*xs,0}, xj[nj] float4
+ * {x*xs,y*xs,z*xs,m}; acc[ni][3] FP64 (unscaled). */
+int pg5t_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
+                   double *acc, void *stream);
+/* Table value f(s) at n device values s (gravity_kernel_avx2.c:69-107). */
+int pg5t_dev_eval(int n, const float *s, float *out);
+int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
+void *pg5t_dev_stream(int devid);
+
+/* ------------------------------------------------------------------ libpg6.so
+ * pgamd-library: pg6 */
+/* Bulk form of nj g6_set_j_particle calls (PG6/libpg6/phantom_g6.c:118) at
+ * addresses 0..nj-1: pos, vel, acc2 (= a/2), jrk6 (= j/6) as [nj][3]. */
+int pg6_dev_set_j(int nj, const double *pos, const double *vel, const double *acc2,
+                  const double *jrk6, const double *mass, const double *tj, const int *idx);
+/* Work of g6_set_ti + g6calc_firsthalf (phantom_g6.c:138,153): predict
+ * j-particles 0..nj-1 to ti. */
+int pg6_dev_predict(int nj, double ti, void *stream);
+/* Work of g6calc_lasthalf2p (phantom_g6.c:270) on device arrays: pos, vel
+ * [ni][3] FP64, id[ni]; acc, jrk [ni][3], pot[ni] FP64; nnb/rnnb may be NULL
+ * (no nearest-neighbour search, the work of g6calc_lasthalf). */
+int pg6_dev_force(int nj, int ni, const double *pos, const double *vel, const int *id,
+                  double eps2, double *acc, double *jrk, double *pot, int *nnb, float *rnnb,
+                  void *stream);
+int pg6_dev_sync(void);
+void *pg6_dev_stream(void);
+int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
+/* Launch shape for (ni, nj): out3 = {bi, nsplit, jper}. */
+int pg6_dev_plan(int ni, int nj, int *out3);
+/* Read-back of the predicted j-particles (what the force kernel reads):
+ * pos[nj][3] FP64, vel[nj][3] FP32, mass[nj] FP32 (may be NULL).  The
+ * reference keeps them in its static Pred_Mem (PG6/libpg6/gravity.c:42-47). */
+int pg6_dev_read_predicted(int nj, double *pos, float *vel, float *mass);
+/* Force-kernel timing: events recorded on the launch stream around each of
+ * the next max_launches pg6_force kernel launches; read returns the summed
+ * kernel time (ms) and the number of launches timed, and stops timing. */
+int pg6_dev_timing_start(int max_launches);
+int pg6_dev_timing_read(double *ms, int *count);
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif

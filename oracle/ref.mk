@@ -1,0 +1,72 @@
+# Builds the reference Phantom-GRAPE libraries from their own C sources, where
+# they lie under the reference tree, into oracle/_ref/ (git-ignored).
+# These shared objects are test infrastructure only: the CPU baseline in
+# bench.py and the parity tests load them as the checker, never as product.
+#
+#   make -f oracle/ref.mk REF=/root/reference
+#
+# Flags follow the reference's own setting.mk / Makefiles:
+#   PG5/newton/setting.mk:3  CFLAGS = -O3 -ffast-math -funroll-loops, avx2 -> -mavx2 -mfma
+#   PG5/table/libpg5/Makefile:31-32  compiled with $(ARCH) only (no CFLAGS)
+#   PG6/setting.mk:3,15,33  -O3 -ffast-math -funroll-loops -fopenmp -DNORMAL
+# Objects are compiled with the reference's flags but linked without
+# -ffast-math: linking with it would add crtfastmath.o, whose constructor sets
+# flush-to-zero for the whole host process that loads the library.
+# Each library is built twice for PG5 (serial and ENABLE_OPENMP) because the
+# reference selects g5_calculate_force_on_x's threading at compile time
+# (PG5/newton/libpg5/phantom_g5.c:302-383).
+
+REF ?= /root/reference
+OUT ?= $(dir $(lastword $(MAKEFILE_LIST)))_ref
+CC  ?= gcc
+
+PG5N  := $(REF)/PG5/newton/libpg5
+PG5T  := $(REF)/PG5/table/libpg5
+PG5Q  := $(REF)/PG5/newton_quad/quad
+PG6   := $(REF)/PG6/libpg6
+
+NEWTON_FLAGS := -O3 -ffast-math -funroll-loops -mavx2 -mfma -fPIC
+TABLE_FLAGS  := -mavx2 -mfma -fPIC
+PG6_FLAGS    := -O3 -ffast-math -funroll-loops -fopenmp -DNORMAL -mavx2 -mfma -fPIC
+
+# compile each source to its own object in a scratch dir, then link a .so
+# $(call so,output,flags,includes,sources,linkflags)
+define so
+	rm -rf $(1).objs && mkdir -p $(1).objs
+	cd $(1).objs && for f in $(4); do $(CC) $(2) $(3) -c $$f -o $$(basename $$f .c).o || exit 1; done
+	$(CC) -shared $(5) $(1).objs/*.o -o $(1) -lm
+	rm -rf $(1).objs
+endef
+
+ALL := $(OUT)/libref_pg5.so $(OUT)/libref_pg5_omp.so $(OUT)/libref_pg5sym.so $(OUT)/libref_pg5quad.so \
+       $(OUT)/libref_pg5table.so $(OUT)/libref_pg5table_omp.so $(OUT)/libref_pg6.so
+
+all: $(ALL)
+
+$(OUT):
+	mkdir -p $(OUT)
+
+$(OUT)/libref_pg5.so: $(PG5N)/gravity_avx2.c $(PG5N)/phantom_g5.c $(PG5N)/rsqrt.c $(PG5N)/pg5_fortran.c | $(OUT)
+	$(call so,$(abspath $@),$(NEWTON_FLAGS),-I$(PG5N),$^,)
+
+$(OUT)/libref_pg5_omp.so: $(PG5N)/gravity_avx2.c $(PG5N)/phantom_g5.c $(PG5N)/rsqrt.c $(PG5N)/pg5_fortran.c | $(OUT)
+	$(call so,$(abspath $@),$(NEWTON_FLAGS) -fopenmp -DENABLE_OPENMP,-I$(PG5N),$^,-fopenmp)
+
+# enable_symmetric=yes (PG5/newton/setting.mk:6,17-19)
+$(OUT)/libref_pg5sym.so: $(PG5N)/gravity_avx2.c $(PG5N)/phantom_g5.c $(PG5N)/rsqrt.c $(PG5N)/pg5_fortran.c | $(OUT)
+	$(call so,$(abspath $@),$(NEWTON_FLAGS) -DSYMMETRIC,-I$(PG5N),$^,)
+
+# quad.a = libpg5 kernel + quad kernel + g5c API (which #includes phantom_g5.c)
+$(OUT)/libref_pg5quad.so: $(PG5N)/gravity_avx2.c $(PG5Q)/quad_avx2.c $(PG5Q)/quad_g5c.c $(PG5N)/rsqrt.c | $(OUT)
+	$(call so,$(abspath $@),$(NEWTON_FLAGS) -Wno-unused-result,-I$(PG5N),$^,)
+
+$(OUT)/libref_pg5table.so: $(PG5T)/gravity_kernel_avx2.c $(PG5T)/phantom_g5.c $(PG5T)/pg5_table.c | $(OUT)
+	$(call so,$(abspath $@),$(TABLE_FLAGS),-I$(PG5T),$^,)
+
+$(OUT)/libref_pg5table_omp.so: $(PG5T)/gravity_kernel_avx2.c $(PG5T)/phantom_g5.c $(PG5T)/pg5_table.c | $(OUT)
+	$(call so,$(abspath $@),$(TABLE_FLAGS) -fopenmp -DENABLE_OPENMP,-I$(PG5T),$^,-fopenmp)
+
+$(OUT)/libref_pg6.so: $(PG6)/phantom_g6.c $(PG6)/gravity_avx2.c $(PG6)/timeprof.c | $(OUT)
+	$(call so,$(abspath $@),$(PG6_FLAGS),-I$(PG6),$^,-fopenmp)
+
+.PHONY: all

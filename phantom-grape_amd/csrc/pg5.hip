@@ -1,0 +1,626 @@
+// libpg5.so: GRAPE-5 API (PG5/newton/libpg5/gp5util.h, phantom_g5.c:35-383,
+// pg5_fortran.c) and the quad_tree cell API (PG5/newton_quad/quad/
+// quad_util.h, quad_g5c.c) on MI355X, plus device-pointer entry points
+// (pg5_dev_*).
+//
+// Board model.  As in the reference each `devid` (0..kMaxDev-1) is an
+// independent board with its own j-memory; the non-MC calls use devid 0
+// (phantom_g5.c:13-28).  Here a board is a HIP stream plus HBM buffers; the
+// j-memory is mirrored in pinned host memory, written by g5_set_xmj and
+// uploaded (only the range written since the last upload) when a force is
+// requested.  g5_calculate_force_on_x* sends all ni i-particles to the GPU in
+// one launch instead of the reference's chunks of g5_get_number_of_pipelines().
+// Different devids may be driven from different host threads at once.
+//
+// Softening: g5_set_eps_to_all sets a value shared by every board, as the
+// reference's static Eps (phantom_g5.c:17,67-70).  Building with
+// -DPG5_SYMMETRIC gives the reference's SYMMETRIC interface, where
+// g5_set_xmj takes eps_j^2 and r^2 = |dx|^2 + eps_i^2 + eps_j^2.
+//
+// rsqrt bias: the reference scales outputs by (1-3b) and -(1-b), where b is
+// the measured bias of the 12-bit rsqrtps (rsqrt.c:42-81).  The GPU uses the
+// ~1 ulp v_rsq_f32, whose bias is negligible, so b = 0 and the output is
+// acc = sum m dx / r^3, pot = + sum m / r including the self pair.
+#include "pgamd_common.h"
+#include "pgamd_api.h"
+#include "pg5_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+
+using namespace pgamd;
+using pg5::Cell;
+using pg5::Outputs5;
+using pg5::Partials5;
+
+#ifdef PG5_SYMMETRIC
+constexpr bool kSym = true;
+#else
+constexpr bool kSym = false;
+#endif
+
+namespace {
+
+constexpr int kMaxDev = 64;             // reference MAXDEV is 24 (phantom_g5.c:13-15)
+constexpr int kDefaultPipes = 4;        // reference NUM_PIPE (phantom_g5.c:11)
+constexpr int kJMemSize = 1 << 24;      // j-memory limit per board
+constexpr int kMaxPipes = 1 << 20;
+
+struct JMem {
+  // Host mirror (pinned) and device copy of one j-memory of float4 records.
+  HostBuf h, he;  // he: eps_j^2 (symmetric build)
+  DevBuf d, de;
+  int hcap = 0;   // records held by h
+  int lo = 0, hi = 0;  // dirty range [lo, hi)
+  void write_range(int a0, int a1) {
+    if (lo >= hi) {
+      lo = a0;
+      hi = a1;
+    } else {
+      lo = std::min(lo, a0);
+      hi = std::max(hi, a1);
+    }
+  }
+};
+
+struct Board {
+  bool init = false;
+  hipStream_t stream = nullptr;
+  JMem jm;             // particle j-memory
+  JMem cm[2];          // cell memories: [0] g5c_*MC, [1] g5c_*MC0
+  int n = 0;           // g5_set_n
+  int nc[2] = {0, 0};  // g5c_set_n
+  // pipeline registers (g5_set_xi / g5_run / g5_get_force)
+  HostBuf ipipe;       // float4 x, y, z, eps2
+  int ni_pipe = 0;
+  HostBuf opipe;       // double acc[3], pot per pipeline
+  // staging of the calculate_force calls
+  HostBuf h_in, h_out;
+  DevBuf d_in, d_out, d_part;
+};
+
+Board g_boards[kMaxDev];
+std::mutex g_init_mutex;
+double g_eps = 0.0;
+int g_npipes = 0;
+
+// launch tuning (autotuner overrides)
+int g_force_bi = 0;
+int g_blocks_per_cu = 8;
+int g_min_jper = 128;
+
+int npipes() {
+  if (g_npipes == 0) {
+    int v = kDefaultPipes;
+    if (const char* s = std::getenv("PGAMD_G5_NPIPES")) {
+      int e = std::atoi(s);
+      if (e > 0 && e <= kMaxPipes) v = e;
+    }
+    g_npipes = v;
+  }
+  return g_npipes;
+}
+
+Board* board(int devid, const char* fn) {
+  if (devid < 0 || devid >= kMaxDev) {
+    api_error(4, "%s: devid %d outside [0, %d)", fn, devid, kMaxDev);
+    return nullptr;
+  }
+  Board& b = g_boards[devid];
+  if (!b.init) {
+    std::lock_guard<std::mutex> lk(g_init_mutex);
+    if (!b.init) {
+      ensure_device();
+      PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+      b.ipipe.reserve(sizeof(float4) * (size_t)npipes());
+      b.opipe.reserve(sizeof(double) * 4 * (size_t)npipes());
+      b.init = true;
+    }
+  }
+  return &b;
+}
+
+void jmem_reserve_host(JMem& m, int need, size_t rec_bytes, bool eps) {
+  if (need <= m.hcap) return;
+  int cap = m.hcap > 0 ? m.hcap : 4096;
+  while (cap < need) cap *= 2;
+  m.h.reserve(rec_bytes * (size_t)cap, true);
+  if (eps) m.he.reserve(sizeof(float) * (size_t)cap, true);
+  m.hcap = cap;
+}
+
+// Upload the dirty range of a j-memory before a kernel reads [0, nread).
+void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
+  m.d.reserve(rec_bytes * (size_t)std::max(nread, m.hcap));
+  if (eps) m.de.reserve(sizeof(float) * (size_t)std::max(nread, m.hcap));
+  if (m.lo < m.hi) {
+    const size_t off = rec_bytes * (size_t)m.lo;
+    PGAMD_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(m.d.ptr) + off,
+                                   static_cast<char*>(m.h.ptr) + off,
+                                   rec_bytes * (size_t)(m.hi - m.lo), hipMemcpyHostToDevice,
+                                   st));
+    if (eps)
+      PGAMD_HIP_CHECK(hipMemcpyAsync(m.de.as<float>() + m.lo, m.he.as<float>() + m.lo,
+                                     sizeof(float) * (size_t)(m.hi - m.lo),
+                                     hipMemcpyHostToDevice, st));
+    m.lo = m.hi = 0;
+  }
+}
+
+struct Plan {
+  int bi, nsplit, jper;
+};
+
+Plan plan(int ni, int nj) {
+  Plan p;
+  p.bi = g_force_bi > 0 ? g_force_bi : (ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64));
+  const int nbi = ceil_div(ni, p.bi);
+  const int target = device_cu_count() * g_blocks_per_cu;
+  int ns = std::max(1, ceil_div(target, nbi));
+  ns = std::min(ns, std::max(1, nj / std::max(1, g_min_jper)));
+  int jper = ceil_div(std::max(nj, 1), ns);
+  jper = (jper + 63) / 64 * 64;
+  p.nsplit = std::max(1, ceil_div(std::max(nj, 1), jper));
+  p.jper = jper;
+  return p;
+}
+
+Partials5 partials(DevBuf& buf, int nsplit, int ni) {
+  Partials5 part{};
+  if (nsplit > 1) {
+    const size_t n = (size_t)nsplit * ni;
+    buf.reserve(n * 4 * sizeof(double));
+    double* d = buf.as<double>();
+    part.ax = d;
+    part.ay = d + n;
+    part.az = d + 2 * n;
+    part.pot = d + 3 * n;
+  }
+  return part;
+}
+
+template <int BI>
+void launch_particles_t(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
+                        const float* ej2, const Partials5& part, const Outputs5& out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL((pg5::pg5_force<BI, kSym>), dim3(ceil_div(ni, BI), p.nsplit),
+                     dim3(pg5::kThreads), 0, st, ni, xi, nj, p.jper, xj, ej2, part, out,
+                     p.nsplit);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+template <int BI>
+void launch_cells_t(const Plan& p, int ni, const float4* xi, int nc, const Cell* cells,
+                    const Partials5& part, const Outputs5& out, hipStream_t st) {
+  hipLaunchKernelGGL((pg5::pg5_quad<BI>), dim3(ceil_div(ni, BI), p.nsplit), dim3(pg5::kThreads),
+                     0, st, ni, xi, nc, p.jper, cells, part, out, p.nsplit);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+// Force of nj particles (cells == nullptr) or nc cells on ni i-particles; all
+// device pointers; asynchronous on st.  Partial-sum scratch lives in `scratch`.
+void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj,
+               const float* ej2, const Cell* cells, const Outputs5& out, hipStream_t st) {
+  if (ni <= 0) return;
+  if (nj <= 0) {
+    PGAMD_HIP_CHECK(hipMemsetAsync(out.acc, 0, sizeof(double) * 3 * (size_t)ni, st));
+    PGAMD_HIP_CHECK(hipMemsetAsync(out.pot, 0, sizeof(double) * (size_t)ni, st));
+    return;
+  }
+  const Plan p = plan(ni, nj);
+  const Partials5 part = partials(scratch, p.nsplit, ni);
+  if (cells) {
+    switch (p.bi) {
+      case 256: launch_cells_t<256>(p, ni, xi, nj, cells, part, out, st); break;
+      case 128: launch_cells_t<128>(p, ni, xi, nj, cells, part, out, st); break;
+      default: launch_cells_t<64>(p, ni, xi, nj, cells, part, out, st); break;
+    }
+  } else {
+    switch (p.bi) {
+      case 256: launch_particles_t<256>(p, ni, xi, nj, xj, ej2, part, out, st); break;
+      case 128: launch_particles_t<128>(p, ni, xi, nj, xj, ej2, part, out, st); break;
+      default: launch_particles_t<64>(p, ni, xi, nj, xj, ej2, part, out, st); break;
+    }
+  }
+  if (p.nsplit > 1) {
+    hipLaunchKernelGGL(pg5::pg5_reduce, dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
+                       p.nsplit, part, out);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+}
+
+// Host-side force evaluation for a board.  i-particles: x[ni][3] and either a
+// per-i eps2 array (eps2 != nullptr, already squared) or the global Eps.
+// cellmem: -1 = particles, 0/1 = cell memory.  Writes a[ni][3] and p[ni] with
+// the sign convention of the API: +pot for particles, -pot for cells.
+void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4* pre, int ni,
+                double (*a)[3], double* p, int cellmem) {
+  if (ni <= 0) return;
+  const size_t in_bytes = sizeof(float4) * (size_t)ni;
+  b.h_in.reserve(in_bytes);
+  b.d_in.reserve(in_bytes);
+  float4* hin = b.h_in.as<float4>();
+  if (pre) {
+    std::memcpy(hin, pre, in_bytes);
+  } else {
+    const float e2all = (float)(g_eps * g_eps);
+    for (int i = 0; i < ni; ++i) {
+      hin[i].x = (float)x[i][0];
+      hin[i].y = (float)x[i][1];
+      hin[i].z = (float)x[i][2];
+      hin[i].w = eps2 ? (float)eps2[i] : e2all;
+    }
+  }
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
+                                 b.stream));
+  const size_t out_bytes = sizeof(double) * 4 * (size_t)ni;
+  b.d_out.reserve(out_bytes);
+  b.h_out.reserve(out_bytes);
+  Outputs5 o{b.d_out.as<double>(), b.d_out.as<double>() + 3 * (size_t)ni};
+  if (cellmem < 0) {
+    const int nj = b.n;
+    jmem_sync(b.jm, sizeof(float4), kSym, nj, b.stream);
+    run_force(b.d_part, ni, b.d_in.as<float4>(), nj, b.jm.d.as<float4>(),
+              kSym ? b.jm.de.as<float>() : nullptr, nullptr, o, b.stream);
+  } else {
+    const int nc = b.nc[cellmem];
+    jmem_sync(b.cm[cellmem], sizeof(Cell), false, nc, b.stream);
+    run_force(b.d_part, ni, b.d_in.as<float4>(), nc, nullptr, nullptr,
+              b.cm[cellmem].d.as<Cell>(), o, b.stream);
+  }
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.h_out.ptr, b.d_out.ptr, out_bytes, hipMemcpyDeviceToHost,
+                                 b.stream));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  const double* ha = b.h_out.as<double>();
+  const double* hp = ha + 3 * (size_t)ni;
+  const double psign = cellmem < 0 ? 1.0 : -1.0;
+  std::memcpy(a, ha, sizeof(double) * 3 * (size_t)ni);
+  for (int i = 0; i < ni; ++i) p[i] = psign * hp[i];
+}
+
+void set_xmj_impl(int devid, int adr, int nj, const double (*xj)[3], const double* mj,
+                  const double* epsj2) {
+  Board* b = board(devid, "g5_set_xmj");
+  if (!b) return;
+  if (adr < 0 || nj < 0 || (long long)adr + nj > kJMemSize) {
+    api_error(2, "g5_set_xmj: addresses [%d, %d) outside the j-memory of %d", adr, adr + nj,
+              kJMemSize);
+    return;
+  }
+  if (nj == 0) return;
+  // The reference reads xj[j] for j in [adr, adr+nj) (phantom_g5.c:193-197).
+  jmem_reserve_host(b->jm, adr + nj, sizeof(float4), kSym);
+  float4* h = b->jm.h.as<float4>();
+  for (int j = adr; j < adr + nj; ++j) {
+    h[j].x = (float)xj[j][0];
+    h[j].y = (float)xj[j][1];
+    h[j].z = (float)xj[j][2];
+    h[j].w = (float)mj[j];
+  }
+  if (kSym)
+    for (int j = adr; j < adr + nj; ++j) b->jm.he.as<float>()[j] = (float)epsj2[j];
+  b->jm.write_range(adr, adr + nj);
+}
+
+void set_xi_impl(int devid, int ni, const double (*xi)[3], const double* eps2) {
+  Board* b = board(devid, "g5_set_xi");
+  if (!b) return;
+  if (ni < 0 || ni > npipes()) {
+    api_error(3, "g5_set_xi: ni=%d exceeds the %d pipelines", ni, npipes());
+    return;
+  }
+  float4* ip = b->ipipe.as<float4>();
+  const float e2all = (float)(g_eps * g_eps);
+  for (int i = 0; i < ni; ++i) {
+    ip[i].x = (float)xi[i][0];
+    ip[i].y = (float)xi[i][1];
+    ip[i].z = (float)xi[i][2];
+    ip[i].w = eps2 ? (float)eps2[i] : e2all;
+  }
+  b->ni_pipe = ni;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ GRAPE-5 API
+
+PGAMD_API int g5_get_number_of_pipelines(void) { return npipes(); }
+PGAMD_API int g5_get_jmemsize(void) { return kJMemSize; }
+
+PGAMD_API void g5_open(void) { ensure_device(); }
+PGAMD_API void g5_close(void) {
+  for (int d = 0; d < kMaxDev; ++d)
+    if (g_boards[d].init) PGAMD_HIP_CHECK(hipStreamSynchronize(g_boards[d].stream));
+}
+
+PGAMD_API void g5_set_eps_to_all(double eps) { g_eps = eps; }
+
+PGAMD_API void g5_set_epsMC(int devid, int ni, double* eps) {
+  Board* b = board(devid, "g5_set_eps");
+  if (!b) return;
+  if (ni < 0 || ni > npipes()) {
+    api_error(3, "g5_set_eps: ni=%d exceeds the %d pipelines", ni, npipes());
+    return;
+  }
+  float4* ip = b->ipipe.as<float4>();
+  for (int i = 0; i < ni; ++i) ip[i].w = (float)(eps[i] * eps[i]);
+}
+PGAMD_API void g5_set_eps(int ni, double* eps) { g5_set_epsMC(0, ni, eps); }
+
+PGAMD_API void g5_set_nMC(int devid, int n) {
+  Board* b = board(devid, "g5_set_n");
+  if (!b) return;
+  if (n < 0 || n > kJMemSize) {
+    api_error(2, "g5_set_n: n=%d outside the j-memory of %d", n, kJMemSize);
+    return;
+  }
+  b->n = n;
+}
+PGAMD_API void g5_set_n(int n) { g5_set_nMC(0, n); }
+
+PGAMD_API void g5_set_xiMC(int devid, int ni, double (*xi)[3]) {
+  set_xi_impl(devid, ni, xi, nullptr);
+}
+PGAMD_API void g5_set_xi(int ni, double (*xi)[3]) { g5_set_xiMC(0, ni, xi); }
+PGAMD_API void g5_set_xeiMC(int devid, int ni, double (*xi)[3], double* eps2) {
+  set_xi_impl(devid, ni, xi, eps2);
+}
+PGAMD_API void g5_set_xei(int ni, double (*xi)[3], double* eps2) {
+  g5_set_xeiMC(0, ni, xi, eps2);
+}
+
+#ifdef PG5_SYMMETRIC
+PGAMD_API void g5_set_xmjMC(int devid, int adr, int nj, double (*xj)[3], double* mj,
+                            double* epsj2) {
+  set_xmj_impl(devid, adr, nj, xj, mj, epsj2);
+}
+PGAMD_API void g5_set_xmj(int adr, int nj, double (*xj)[3], double* mj, double* epsj2) {
+  set_xmj_impl(0, adr, nj, xj, mj, epsj2);
+}
+#else
+PGAMD_API void g5_set_xmjMC(int devid, int adr, int nj, double (*xj)[3], double* mj) {
+  set_xmj_impl(devid, adr, nj, xj, mj, nullptr);
+}
+PGAMD_API void g5_set_xmj(int adr, int nj, double (*xj)[3], double* mj) {
+  set_xmj_impl(0, adr, nj, xj, mj, nullptr);
+}
+#endif
+
+PGAMD_API void g5_runMC(int devid) {
+  Board* b = board(devid, "g5_run");
+  if (!b) return;
+  const int ni = b->ni_pipe;
+  if (ni <= 0) return;
+  double(*a)[3] = reinterpret_cast<double(*)[3]>(b->opipe.as<double>());
+  double* p = b->opipe.as<double>() + 3 * (size_t)ni;
+  board_calc(*b, nullptr, nullptr, b->ipipe.as<float4>(), ni, a, p, -1);
+}
+PGAMD_API void g5_run(void) { g5_runMC(0); }
+
+PGAMD_API void g5_get_forceMC(int devid, int ni, double (*a)[3], double* p) {
+  Board* b = board(devid, "g5_get_force");
+  if (!b) return;
+  if (ni < 0 || ni > b->ni_pipe) {
+    api_error(3, "g5_get_force: ni=%d but %d i-particles were run", ni, b->ni_pipe);
+    return;
+  }
+  const int n = b->ni_pipe;
+  const double* oa = b->opipe.as<double>();
+  const double* op = oa + 3 * (size_t)n;
+  std::memcpy(a, oa, sizeof(double) * 3 * (size_t)ni);
+  std::memcpy(p, op, sizeof(double) * (size_t)ni);
+}
+PGAMD_API void g5_get_force(int ni, double (*a)[3], double* p) { g5_get_forceMC(0, ni, a, p); }
+
+PGAMD_API void g5_calculate_force_on_xMC(int devid, double (*x)[3], double (*a)[3], double* p,
+                                         int ni) {
+  Board* b = board(devid, "g5_calculate_force_on_x");
+  if (!b) return;
+  board_calc(*b, x, nullptr, nullptr, ni, a, p, -1);
+}
+PGAMD_API void g5_calculate_force_on_x(double (*x)[3], double (*a)[3], double* p, int ni) {
+  g5_calculate_force_on_xMC(0, x, a, p, ni);
+}
+PGAMD_API void g5_calculate_force_on_xeMC(int devid, double (*x)[3], double* eps2,
+                                          double (*a)[3], double* p, int ni) {
+  Board* b = board(devid, "g5_calculate_force_on_xe");
+  if (!b) return;
+  board_calc(*b, x, eps2, nullptr, ni, a, p, -1);
+}
+PGAMD_API void g5_calculate_force_on_xe(double (*x)[3], double* eps2, double (*a)[3], double* p,
+                                        int ni) {
+  g5_calculate_force_on_xeMC(0, x, eps2, a, p, ni);
+}
+
+// ------------------------------------------------- Fortran entry points
+// (PG5/newton/libpg5/pg5_fortran.c:1-119): every argument by reference.
+
+PGAMD_API int g5_get_number_of_pipelines_(void) { return g5_get_number_of_pipelines(); }
+PGAMD_API int g5_get_jmemsize_(void) { return g5_get_jmemsize(); }
+PGAMD_API void g5_open_(void) { g5_open(); }
+PGAMD_API void g5_close_(void) { g5_close(); }
+PGAMD_API void g5_set_eps_to_all_(double* eps) { g5_set_eps_to_all(*eps); }
+PGAMD_API void g5_set_n_(int* n) { g5_set_n(*n); }
+PGAMD_API void g5_set_nMC_(int* devid, int* n) { g5_set_nMC(*devid, *n); }
+PGAMD_API void g5_set_xi_(int* ni, double (*xi)[3]) { g5_set_xi(*ni, xi); }
+PGAMD_API void g5_set_xei_(int* ni, double (*xi)[3], double* eps2) { g5_set_xei(*ni, xi, eps2); }
+PGAMD_API void g5_set_xiMC_(int* devid, int* ni, double (*xi)[3]) {
+  g5_set_xiMC(*devid, *ni, xi);
+}
+PGAMD_API void g5_set_xeiMC_(int* devid, int* ni, double (*xi)[3], double* eps2) {
+  g5_set_xeiMC(*devid, *ni, xi, eps2);
+}
+#ifdef PG5_SYMMETRIC
+PGAMD_API void g5_set_xmj_(int* adr, int* nj, double (*xj)[3], double* mj, double* eps2) {
+  g5_set_xmj(*adr, *nj, xj, mj, eps2);
+}
+PGAMD_API void g5_set_xmjMC_(int* devid, int* adr, int* nj, double (*xj)[3], double* mj,
+                             double* eps2) {
+  g5_set_xmjMC(*devid, *adr, *nj, xj, mj, eps2);
+}
+#else
+PGAMD_API void g5_set_xmj_(int* adr, int* nj, double (*xj)[3], double* mj) {
+  g5_set_xmj(*adr, *nj, xj, mj);
+}
+PGAMD_API void g5_set_xmjMC_(int* devid, int* adr, int* nj, double (*xj)[3], double* mj) {
+  g5_set_xmjMC(*devid, *adr, *nj, xj, mj);
+}
+#endif
+PGAMD_API void g5_run_(void) { g5_run(); }
+PGAMD_API void g5_runMC_(int* devid) { g5_runMC(*devid); }
+PGAMD_API void g5_get_force_(int* ni, double (*a)[3], double* p) { g5_get_force(*ni, a, p); }
+PGAMD_API void g5_get_forceMC_(int* devid, int* ni, double (*a)[3], double* p) {
+  g5_get_forceMC(*devid, *ni, a, p);
+}
+PGAMD_API void g5_calculate_force_on_x_(double (*x)[3], double (*a)[3], double* p, int* ni) {
+  g5_calculate_force_on_x(x, a, p, *ni);
+}
+PGAMD_API void g5_calculate_force_on_xe_(double (*x)[3], double* eps2, double (*a)[3],
+                                         double* p, int* ni) {
+  g5_calculate_force_on_xe(x, eps2, a, p, *ni);
+}
+PGAMD_API void g5_calculate_force_on_xMC_(int* devid, double (*x)[3], double (*a)[3], double* p,
+                                          int* ni) {
+  g5_calculate_force_on_xMC(*devid, x, a, p, *ni);
+}
+PGAMD_API void g5_calculate_force_on_xeMC_(int* devid, double (*x)[3], double* eps2,
+                                           double (*a)[3], double* p, int* ni) {
+  g5_calculate_force_on_xeMC(*devid, x, eps2, a, p, *ni);
+}
+
+// ------------------------------------------------------------- quad_tree API
+// Cells: COM xj[.][3], mass mj[.], traceless quadrupole qj[.][6] =
+// {xx, xy, xz, yy, yz, zz} of sum m (3 d d - |d|^2 I) (source.cpp:215-248).
+// Potential is returned negative: p = -sum (M/r + dx.Q.dx / (2 r^5))
+// (quad_g5c.c:285-295 through g5_get_forceMC's -(1-b) scaling).
+
+namespace {
+void g5c_set_xmj_impl(int devid, int mem, int adr, int nj, const double (*xj)[3],
+                      const double* mj, const double (*qj)[6]) {
+  Board* b = board(devid, "g5c_set_xmj");
+  if (!b) return;
+  if (adr < 0 || nj < 0 || (long long)adr + nj > kJMemSize) {
+    api_error(2, "g5c_set_xmj: addresses [%d, %d) outside the cell memory of %d", adr,
+              adr + nj, kJMemSize);
+    return;
+  }
+  if (nj == 0) return;
+  JMem& m = b->cm[mem];
+  jmem_reserve_host(m, adr + nj, sizeof(Cell), false);
+  Cell* h = m.h.as<Cell>();
+  for (int j = adr; j < adr + nj; ++j) {
+    Cell c;
+    c.xm = make_float4((float)xj[j][0], (float)xj[j][1], (float)xj[j][2], (float)mj[j]);
+    c.qa = make_float4((float)qj[j][0], (float)qj[j][1], (float)qj[j][2], (float)qj[j][3]);
+    c.qb = make_float4((float)qj[j][4], (float)qj[j][5], 0.f, 0.f);
+    h[j] = c;
+  }
+  m.write_range(adr, adr + nj);
+}
+void g5c_set_n_impl(int devid, int mem, int n) {
+  Board* b = board(devid, "g5c_set_n");
+  if (!b) return;
+  if (n < 0 || n > kJMemSize) {
+    api_error(2, "g5c_set_n: n=%d outside the cell memory of %d", n, kJMemSize);
+    return;
+  }
+  b->nc[mem] = n;
+}
+}  // namespace
+
+PGAMD_API void g5c_set_nMC(int devid, int n) {
+  g5c_set_n_impl(devid, 0, n);
+  g5c_set_n_impl(devid, 1, n);
+}
+PGAMD_API void g5c_set_xmjMC(int devid, int adr, int nj, double (*xj)[3], double* mj,
+                             double (*qj)[6]) {
+  g5c_set_xmj_impl(devid, 0, adr, nj, xj, mj, qj);
+}
+PGAMD_API void g5c_set_xmjMC0(int devid, int adr, int nj, double (*xj)[3], double* mj,
+                              double (*qj)[6]) {
+  g5c_set_xmj_impl(devid, 1, adr, nj, xj, mj, qj);
+}
+PGAMD_API void g5c_runMC(int devid) {
+  Board* b = board(devid, "g5c_run");
+  if (!b) return;
+  const int ni = b->ni_pipe;
+  if (ni <= 0) return;
+  double(*a)[3] = reinterpret_cast<double(*)[3]>(b->opipe.as<double>());
+  double* p = b->opipe.as<double>() + 3 * (size_t)ni;
+  board_calc(*b, nullptr, nullptr, b->ipipe.as<float4>(), ni, a, p, 0);
+}
+PGAMD_API void g5c_runMC0(int devid) {
+  Board* b = board(devid, "g5c_run");
+  if (!b) return;
+  const int ni = b->ni_pipe;
+  if (ni <= 0) return;
+  double(*a)[3] = reinterpret_cast<double(*)[3]>(b->opipe.as<double>());
+  double* p = b->opipe.as<double>() + 3 * (size_t)ni;
+  board_calc(*b, nullptr, nullptr, b->ipipe.as<float4>(), ni, a, p, 1);
+}
+PGAMD_API void g5c_calculate_force_on_xMC(int devid, double (*x)[3], double (*a)[3], double* p,
+                                          int ni) {
+  Board* b = board(devid, "g5c_calculate_force_on_x");
+  if (!b) return;
+  board_calc(*b, x, nullptr, nullptr, ni, a, p, 0);
+}
+PGAMD_API void g5c_calculate_force_on_xMC0(int devid, double (*x)[3], double (*a)[3], double* p,
+                                           int ni) {
+  Board* b = board(devid, "g5c_calculate_force_on_x");
+  if (!b) return;
+  board_calc(*b, x, nullptr, nullptr, ni, a, p, 1);
+}
+
+// ------------------------------------------------- device-pointer entry points
+
+// Force of nj j-particles on ni i-particles, all in device memory:
+//   xi[ni] float4 {x, y, z, eps_i^2}, xj[nj] float4 {x, y, z, m},
+//   epsj2[nj] (symmetric build only, else ignored), acc[ni][3], pot[ni] FP64
+//   with pot = + sum m / r (self pair included, no sign change).
+// Asynchronous on `stream` (nullptr = the board's stream); uses board devid's
+// partial-sum scratch, so concurrent calls on one devid race.
+PGAMD_API int pg5_dev_force(int devid, int ni, const void* xi, int nj, const void* xj,
+                            const float* epsj2, double* acc, double* pot, void* stream) {
+  Board* b = board(devid, "pg5_dev_force");
+  if (!b) return -1;
+  if (ni < 0 || nj < 0) {
+    api_error(3, "pg5_dev_force: negative ni (%d) or nj (%d)", ni, nj);
+    return -1;
+  }
+  Outputs5 o{acc, pot};
+  run_force(b->d_part, ni, static_cast<const float4*>(xi), nj, static_cast<const float4*>(xj),
+            epsj2, nullptr, o, stream ? (hipStream_t)stream : b->stream);
+  return 0;
+}
+
+// Cell force of nc cells (device array of 12 floats each: x, y, z, M, Qxx,
+// Qxy, Qxz, Qyy, Qyz, Qzz, 0, 0) on ni i-particles; pot = + sum (phi_p+phi_q).
+PGAMD_API int pg5_dev_quad(int devid, int ni, const void* xi, int nc, const void* cells,
+                           double* acc, double* pot, void* stream) {
+  Board* b = board(devid, "pg5_dev_quad");
+  if (!b) return -1;
+  if (ni < 0 || nc < 0) {
+    api_error(3, "pg5_dev_quad: negative ni (%d) or nc (%d)", ni, nc);
+    return -1;
+  }
+  Outputs5 o{acc, pot};
+  run_force(b->d_part, ni, static_cast<const float4*>(xi), nc, nullptr, nullptr,
+            static_cast<const Cell*>(cells), o, stream ? (hipStream_t)stream : b->stream);
+  return 0;
+}
+
+PGAMD_API void* pg5_dev_stream(int devid) {
+  Board* b = board(devid, "pg5_dev_stream");
+  return b ? (void*)b->stream : nullptr;
+}
+
+PGAMD_API int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
+  if (bi != 0 && bi != 64 && bi != 128 && bi != 256) {
+    api_error(3, "pg5_dev_set_launch: bi=%d not in {0,64,128,256}", bi);
+    return -1;
+  }
+  g_force_bi = bi;
+  if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
+  if (min_jper > 0) g_min_jper = min_jper;
+  return 0;
+}

@@ -1,0 +1,539 @@
+// libpg5_table.so: GRAPE-5 API with an arbitrary cut-off force shape by table
+// interpolation (PG5/table/libpg5/gp5util.h, phantom_g5.c:43-200,
+// pg5_table.h, pg5_table.c:31-113), on MI355X.  Default shape: the S2
+// (Hockney-Eastwood) short-range TreePM force, eps_PP = 0.1/64,
+// eps_PM = r_cut = 3/64 (pppm.h:4-6).
+//
+// Board model as in libpg5.so: per-devid boards (stream + HBM j-memory,
+// pinned host mirror uploaded by dirty range).  Positions are stored scaled by
+// xscale (phantom_g5.c:96-128); the kernel indexes the table by the float bits
+// of s = 2 + |dx_s|^2.  The potential is not computed (returned as 0,
+// phantom_g5.c:145-154).  Softening and eta are accepted and ignored, as in
+// the reference (the softening is inside the tabulated shape).
+#include "pgamd_common.h"
+#include "pgamd_api.h"
+#include "table_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+using namespace pgamd;
+
+// Table of the default resolution, exported like the reference's global
+// (pg5_table.h:31): Force_table[k][0] = value, [k][1] = slope in s.
+extern "C" {
+__attribute__((visibility("default"))) float Force_table[512][2];
+}
+
+namespace {
+
+constexpr int kExpBit = 4;              // pg5_table.h:8
+constexpr int kDefaultFrcBit = 5;       // pg5_table.h:9
+constexpr int kMaxDev = 64;
+constexpr int kDefaultPipes = 4;        // phantom_g5.c:9
+constexpr int kJMemSize = 1 << 24;
+constexpr double kMesh = 64.0;          // pppm.h:4
+constexpr double kSftPM = 3.0 / kMesh;  // pppm.h:5
+constexpr double kSftPP = 0.1 / kMesh;  // pppm.h:6
+
+struct Table {
+  int frc_bit = kDefaultFrcBit;
+  int ntab = 1 << (kExpBit + kDefaultFrcBit);
+  std::vector<float> host;  // 2*ntab
+  DevBuf dev;
+  bool dirty = true;
+  double xscale = 0.0;
+  float r2scale = 0.f;
+  double (*func)(double) = nullptr;  // last shape given to pg5_gen_force_table
+  double rcut = 0.0;
+  double sft_pp = kSftPP, sft_pm = kSftPM;
+  bool is_s2 = true;
+};
+
+struct Board {
+  bool init = false;
+  hipStream_t stream = nullptr;
+  HostBuf hj;
+  int hcap = 0;
+  int lo = 0, hi = 0;
+  DevBuf dj;
+  int n = 0;
+  HostBuf ipipe, opipe;
+  int ni_pipe = 0;
+  HostBuf h_in, h_out;
+  DevBuf d_in, d_out, d_part;
+};
+
+Table g_tab;
+Board g_boards[kMaxDev];
+std::mutex g_mutex;  // guards the table and board creation
+int g_force_bi = 0;
+int g_blocks_per_cu = 8;
+int g_min_jper = 128;
+
+int npipes() {
+  static int v = [] {
+    int d = kDefaultPipes;
+    if (const char* s = std::getenv("PGAMD_G5_NPIPES")) {
+      int e = std::atoi(s);
+      if (e > 0) d = e;
+    }
+    return d;
+  }();
+  return v;
+}
+
+// S2 force shape in r^-3 units, the reference's piecewise polynomial in
+// eta = 2 r / eps (pg5_table.c:8-22) with the same Horner order, so that the
+// table entries are the same floats.
+double s2_reff(double sft, double r) {
+  const double eta = 2.0 * r / sft;
+  if (eta < 1.0)
+    return (224. + eta * eta * (-224. + eta * (70. + eta * (48. - eta * 21.)))) * 2.e0 /
+           (35.0 * std::pow(sft, 3));
+  if (eta < 2.0)
+    return (12. / (eta * eta) - 224. +
+            eta * (896. + eta * (-840. + eta * (224. + eta * (70. + eta * (-48. + eta * 7.)))))) *
+           2.e0 / (35.0 * eta * std::pow(sft, 3));
+  return 1.0 / std::pow(r, 3);
+}
+
+double s2_force(double r) { return s2_reff(g_tab.sft_pp, r) - s2_reff(g_tab.sft_pm, r); }
+
+union Pack32 {
+  float f;
+  uint32_t u;
+};
+
+// pg5_gen_force_table (pg5_table.c:43-113): entries at float-bit ticks of s
+// from 2, r2 = (s - 2) / r2scale, value = func(sqrtf(r2)), forward-difference
+// slope, last slope 0.  Expressions keep the reference's C types.
+void build_table(double (*func)(double), double rcut, int frc_bit) {
+  Table& t = g_tab;
+  t.frc_bit = frc_bit;
+  t.ntab = 1 << (kExpBit + frc_bit);
+  t.host.assign(2 * (size_t)t.ntab, 0.f);
+  const uint32_t tick = 1u << (23 - frc_bit);
+  const float r2max = (float)(rcut * rcut);
+  const float fmax = (float)((double)(1 << (1 << kExpBit)) * (2.0 - 1.0 / (1 << frc_bit)));
+  const float r2scale = (fmax - 2.0f) / r2max;
+  t.r2scale = r2scale;
+  t.xscale = (double)sqrtf(r2scale);
+  Pack32 m;
+  m.f = 2.0f;
+  for (int i = 0; i < t.ntab; i++, m.u += tick) {
+    const float f = m.f;
+    const float r2 = (float)(((double)f - 2.0) / (double)r2scale);
+    const float r = sqrtf(r2);
+    t.host[2 * i] = (float)func((double)r);
+  }
+  m.f = 2.0f;
+  for (int i = 0; i < t.ntab - 1; i++) {
+    const float x0 = m.f;
+    m.u += tick;
+    const float x1 = m.f;
+    t.host[2 * i + 1] = (t.host[2 * (i + 1)] - t.host[2 * i]) / (x1 - x0);
+  }
+  t.host[2 * (t.ntab - 1) + 1] = 0.0f;
+  t.func = func;
+  t.rcut = rcut;
+  t.dirty = true;
+  if (t.ntab <= 512) std::memcpy(Force_table, t.host.data(), sizeof(float) * 2 * t.ntab);
+}
+
+void ensure_table() {
+  if (g_tab.host.empty()) {
+    g_tab.is_s2 = true;
+    build_table(s2_force, g_tab.sft_pm, g_tab.frc_bit);
+  }
+}
+
+// Upload the table if it changed (callers hold g_mutex).
+const float2* table_dev(hipStream_t st) {
+  Table& t = g_tab;
+  if (t.dirty) {
+    t.dev.reserve(sizeof(float) * t.host.size());
+    PGAMD_HIP_CHECK(hipMemcpyAsync(t.dev.ptr, t.host.data(), sizeof(float) * t.host.size(),
+                                   hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    t.dirty = false;
+  }
+  return t.dev.as<float2>();
+}
+
+Board* board(int devid, const char* fn) {
+  if (devid < 0 || devid >= kMaxDev) {
+    api_error(4, "%s: devid %d outside [0, %d)", fn, devid, kMaxDev);
+    return nullptr;
+  }
+  Board& b = g_boards[devid];
+  if (!b.init) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (!b.init) {
+      ensure_device();
+      PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+      b.ipipe.reserve(sizeof(float4) * (size_t)npipes());
+      b.opipe.reserve(sizeof(double) * 3 * (size_t)npipes());
+      b.init = true;
+    }
+  }
+  return &b;
+}
+
+struct Plan {
+  int bi, nsplit, jper;
+};
+
+Plan plan(int ni, int nj) {
+  Plan p;
+  p.bi = g_force_bi > 0 ? g_force_bi : (ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64));
+  const int nbi = ceil_div(ni, p.bi);
+  const int target = device_cu_count() * g_blocks_per_cu;
+  int ns = std::max(1, ceil_div(target, nbi));
+  ns = std::min(ns, std::max(1, nj / std::max(1, g_min_jper)));
+  int jper = ceil_div(std::max(nj, 1), ns);
+  jper = (jper + 63) / 64 * 64;
+  p.nsplit = std::max(1, ceil_div(std::max(nj, 1), jper));
+  p.jper = jper;
+  return p;
+}
+
+template <int BI>
+void launch_t(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
+              const float2* tab, double ascale, const pgt::Part3& part, double* acc,
+              hipStream_t st) {
+  const size_t lds = sizeof(float2) * (size_t)g_tab.ntab;
+  hipLaunchKernelGGL((pgt::table_force<BI>), dim3(ceil_div(ni, BI), p.nsplit),
+                     dim3(pgt::kThreads), lds, st, ni, xi, nj, p.jper, xj, tab, g_tab.ntab,
+                     g_tab.frc_bit, ascale, part, acc, p.nsplit);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+// Scaled i (float4) and j (float4) device arrays -> acc[ni][3] (device).
+void run_table(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj, double* acc,
+               hipStream_t st) {
+  if (ni <= 0) return;
+  if (nj <= 0) {
+    PGAMD_HIP_CHECK(hipMemsetAsync(acc, 0, sizeof(double) * 3 * (size_t)ni, st));
+    return;
+  }
+  const float2* tab = table_dev(st);
+  // Reference scales the FP32 sum by float(1/xscale) (phantom_g5.c:46-47).
+  const double ascale = (double)(float)(1.0 / g_tab.xscale);
+  const Plan p = plan(ni, nj);
+  pgt::Part3 part{};
+  if (p.nsplit > 1) {
+    const size_t n = (size_t)p.nsplit * ni;
+    scratch.reserve(n * 3 * sizeof(double));
+    part.ax = scratch.as<double>();
+    part.ay = part.ax + n;
+    part.az = part.ay + n;
+  }
+  switch (p.bi) {
+    case 256: launch_t<256>(p, ni, xi, nj, xj, tab, ascale, part, acc, st); break;
+    case 128: launch_t<128>(p, ni, xi, nj, xj, tab, ascale, part, acc, st); break;
+    default: launch_t<64>(p, ni, xi, nj, xj, tab, ascale, part, acc, st); break;
+  }
+  if (p.nsplit > 1) {
+    hipLaunchKernelGGL(pgt::table_reduce, dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
+                       p.nsplit, part, ascale, acc);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+}
+
+inline float4 scale_i(const double* x, double xs) {
+  // (float)x * Xscale evaluated in double, stored as float (phantom_g5.c:102-104)
+  return make_float4((float)((double)(float)x[0] * xs), (float)((double)(float)x[1] * xs),
+                     (float)((double)(float)x[2] * xs), 0.f);
+}
+
+void board_calc(Board& b, const double (*x)[3], const float4* pre, int ni, double (*a)[3],
+                double* p) {
+  if (ni <= 0) return;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  const size_t in_bytes = sizeof(float4) * (size_t)ni;
+  b.h_in.reserve(in_bytes);
+  b.d_in.reserve(in_bytes);
+  float4* hin = b.h_in.as<float4>();
+  if (pre)
+    std::memcpy(hin, pre, in_bytes);
+  else
+    for (int i = 0; i < ni; ++i) hin[i] = scale_i(x[i], g_tab.xscale);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
+                                 b.stream));
+  const int nj = b.n;
+  b.dj.reserve(sizeof(float4) * (size_t)std::max(nj, b.hcap));
+  if (b.lo < b.hi) {
+    PGAMD_HIP_CHECK(hipMemcpyAsync(b.dj.as<float4>() + b.lo, b.hj.as<float4>() + b.lo,
+                                   sizeof(float4) * (size_t)(b.hi - b.lo),
+                                   hipMemcpyHostToDevice, b.stream));
+    b.lo = b.hi = 0;
+  }
+  const size_t out_bytes = sizeof(double) * 3 * (size_t)ni;
+  b.d_out.reserve(out_bytes);
+  b.h_out.reserve(out_bytes);
+  run_table(b.d_part, ni, b.d_in.as<float4>(), nj, b.dj.as<float4>(), b.d_out.as<double>(),
+            b.stream);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.h_out.ptr, b.d_out.ptr, out_bytes, hipMemcpyDeviceToHost,
+                                 b.stream));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  std::memcpy(a, b.h_out.ptr, out_bytes);
+  for (int i = 0; i < ni; ++i) p[i] = 0.0;
+}
+
+void jmem_reserve(Board& b, int need) {
+  if (need <= b.hcap) return;
+  int cap = b.hcap > 0 ? b.hcap : 4096;
+  while (cap < need) cap *= 2;
+  b.hj.reserve(sizeof(float4) * (size_t)cap, true);
+  b.hcap = cap;
+}
+
+void mark(Board& b, int a0, int a1) {
+  if (b.lo >= b.hi) {
+    b.lo = a0;
+    b.hi = a1;
+  } else {
+    b.lo = std::min(b.lo, a0);
+    b.hi = std::max(b.hi, a1);
+  }
+}
+
+bool check_range(const char* fn, int adr, int nj) {
+  if (adr < 0 || nj < 0 || (long long)adr + nj > kJMemSize) {
+    api_error(2, "%s: addresses [%d, %d) outside the j-memory of %d", fn, adr, adr + nj,
+              kJMemSize);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ table control
+
+PGAMD_API void pg5_set_xscale(double xscale) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_tab.xscale = xscale;
+}
+
+PGAMD_API void pg5_gen_force_table(double (*force_func)(double), double rcut) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_tab.is_s2 = false;
+  build_table(force_func, rcut, g_tab.frc_bit);
+}
+
+PGAMD_API void pg5_gen_s2_force_table(double sft_for_PP, double sft_for_PM) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_tab.sft_pp = sft_for_PP;
+  g_tab.sft_pm = sft_for_PM;
+  g_tab.is_s2 = true;
+  build_table(s2_force, sft_for_PM, g_tab.frc_bit);
+}
+
+// Table resolution: 2^frc_bit entries per octave of s (reference: 5, i.e. 512
+// entries; 6 gives 1024).  Rebuilds the current shape.  No counterpart in the
+// reference, where FRC_BIT is a compile-time constant (pg5_table.h:7-15).
+PGAMD_API int pg5_set_table_frc_bit(int frc_bit) {
+  if (frc_bit < 2 || frc_bit > 9) {
+    api_error(3, "pg5_set_table_frc_bit: %d not in [2, 9]", frc_bit);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mutex);
+  double (*f)(double) = g_tab.func ? g_tab.func : s2_force;
+  const double rc = g_tab.func ? g_tab.rcut : g_tab.sft_pm;
+  build_table(f, rc, frc_bit);
+  return 0;
+}
+
+// Copies the current table (2*ntab floats) into out if non-null; returns ntab.
+PGAMD_API int pg5_get_table(float* out, double* xscale, float* r2scale) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  if (out) std::memcpy(out, g_tab.host.data(), sizeof(float) * g_tab.host.size());
+  if (xscale) *xscale = g_tab.xscale;
+  if (r2scale) *r2scale = g_tab.r2scale;
+  return g_tab.ntab;
+}
+
+// ------------------------------------------------------------------ GRAPE-5 API
+
+PGAMD_API int g5_get_number_of_pipelines(void) { return npipes(); }
+PGAMD_API int g5_get_jmemsize(void) { return kJMemSize; }
+
+PGAMD_API void g5_open(void) {
+  ensure_device();
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_tab.sft_pp = kSftPP;
+  g_tab.sft_pm = kSftPM;
+  g_tab.is_s2 = true;
+  build_table(s2_force, kSftPM, g_tab.frc_bit);
+}
+PGAMD_API void g5_close(void) {
+  for (int d = 0; d < kMaxDev; ++d)
+    if (g_boards[d].init) PGAMD_HIP_CHECK(hipStreamSynchronize(g_boards[d].stream));
+}
+
+// Accepted and ignored, as in the reference (phantom_g5.c:69-84).
+PGAMD_API void g5_set_eta(double) {}
+PGAMD_API void g5_set_eps_to_all(double) {}
+PGAMD_API void g5_set_range(double, double, double) {}
+PGAMD_API void g5_set_cutoff_table(double (*)(double), double, double, double (*)(double),
+                                   double, double) {}
+
+PGAMD_API void g5_set_nMC(int devid, int n) {
+  Board* b = board(devid, "g5_set_n");
+  if (!b) return;
+  if (n < 0 || n > kJMemSize) {
+    api_error(2, "g5_set_n: n=%d outside the j-memory of %d", n, kJMemSize);
+    return;
+  }
+  b->n = n;
+}
+PGAMD_API void g5_set_n(int n) { g5_set_nMC(0, n); }
+
+PGAMD_API void g5_set_xiMC(int devid, int ni, double (*xi)[3]) {
+  Board* b = board(devid, "g5_set_xi");
+  if (!b) return;
+  if (ni < 0 || ni > npipes()) {
+    api_error(3, "g5_set_xi: ni=%d exceeds the %d pipelines", ni, npipes());
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  for (int i = 0; i < ni; ++i) b->ipipe.as<float4>()[i] = scale_i(xi[i], g_tab.xscale);
+  b->ni_pipe = ni;
+}
+PGAMD_API void g5_set_xi(int ni, double (*xi)[3]) { g5_set_xiMC(0, ni, xi); }
+
+// j-memory: {x*xs, y*xs, z*xs, m} in float (phantom_g5.c:112-128).  Like the
+// reference, xj and mj are indexed by the absolute address adr..adr+nj-1.
+PGAMD_API void g5_set_xmjMC(int devid, int adr, int nj, double (*xj)[3], double* mj) {
+  Board* b = board(devid, "g5_set_xmj");
+  if (!b || !check_range("g5_set_xmj", adr, nj) || nj == 0) return;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  jmem_reserve(*b, adr + nj);
+  const float xs = (float)g_tab.xscale;
+  float4* h = b->hj.as<float4>();
+  for (int j = adr; j < adr + nj; ++j)
+    h[j] = make_float4((float)xj[j][0] * xs, (float)xj[j][1] * xs, (float)xj[j][2] * xs,
+                       (float)mj[j]);
+  mark(*b, adr, adr + nj);
+}
+PGAMD_API void g5_set_xmj(int adr, int nj, double (*xj)[3], double* mj) {
+  g5_set_xmjMC(0, adr, nj, xj, mj);
+}
+
+// Declared but never defined by the reference (gp5util.h:72-73); provided
+// here with the obvious meaning: update positions or masses only.
+PGAMD_API void g5_set_xj(int adr, int nj, double (*xj)[3]) {
+  Board* b = board(0, "g5_set_xj");
+  if (!b || !check_range("g5_set_xj", adr, nj) || nj == 0) return;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  jmem_reserve(*b, adr + nj);
+  const float xs = (float)g_tab.xscale;
+  float4* h = b->hj.as<float4>();
+  for (int j = adr; j < adr + nj; ++j) {
+    h[j].x = (float)xj[j][0] * xs;
+    h[j].y = (float)xj[j][1] * xs;
+    h[j].z = (float)xj[j][2] * xs;
+  }
+  mark(*b, adr, adr + nj);
+}
+PGAMD_API void g5_set_mj(int adr, int nj, double* mj) {
+  Board* b = board(0, "g5_set_mj");
+  if (!b || !check_range("g5_set_mj", adr, nj) || nj == 0) return;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  jmem_reserve(*b, adr + nj);
+  float4* h = b->hj.as<float4>();
+  for (int j = adr; j < adr + nj; ++j) h[j].w = (float)mj[j];
+  mark(*b, adr, adr + nj);
+}
+
+PGAMD_API void g5_runMC(int devid) {
+  Board* b = board(devid, "g5_run");
+  if (!b) return;
+  const int ni = b->ni_pipe;
+  if (ni <= 0) return;
+  double(*a)[3] = reinterpret_cast<double(*)[3]>(b->opipe.as<double>());
+  std::vector<double> p((size_t)ni);
+  board_calc(*b, nullptr, b->ipipe.as<float4>(), ni, a, p.data());
+}
+PGAMD_API void g5_run(void) { g5_runMC(0); }
+
+PGAMD_API void g5_get_forceMC(int devid, int ni, double (*a)[3], double* pot) {
+  Board* b = board(devid, "g5_get_force");
+  if (!b) return;
+  if (ni < 0 || ni > b->ni_pipe) {
+    api_error(3, "g5_get_force: ni=%d but %d i-particles were run", ni, b->ni_pipe);
+    return;
+  }
+  std::memcpy(a, b->opipe.ptr, sizeof(double) * 3 * (size_t)ni);
+  for (int i = 0; i < ni; ++i) pot[i] = 0.0;
+}
+PGAMD_API void g5_get_force(int ni, double (*a)[3], double* p) { g5_get_forceMC(0, ni, a, p); }
+
+PGAMD_API void g5_calculate_force_on_xMC(int devid, double (*x)[3], double (*a)[3], double* p,
+                                         int ni) {
+  Board* b = board(devid, "g5_calculate_force_on_x");
+  if (!b) return;
+  board_calc(*b, x, nullptr, ni, a, p);
+}
+PGAMD_API void g5_calculate_force_on_x(double (*x)[3], double (*a)[3], double* p, int ni) {
+  g5_calculate_force_on_xMC(0, x, a, p, ni);
+}
+
+// ------------------------------------------------- device-pointer entry points
+
+// Scaled positions (device float4 {x*xs, y*xs, z*xs, m} for j, {.., 0} for i)
+// -> acc[ni][3] (device, FP64, unscaled).  Async on stream (nullptr = board's).
+PGAMD_API int pg5t_dev_force(int devid, int ni, const void* xi, int nj, const void* xj,
+                             double* acc, void* stream) {
+  Board* b = board(devid, "pg5t_dev_force");
+  if (!b) return -1;
+  if (ni < 0 || nj < 0) {
+    api_error(3, "pg5t_dev_force: negative ni (%d) or nj (%d)", ni, nj);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  run_table(b->d_part, ni, static_cast<const float4*>(xi), nj, static_cast<const float4*>(xj),
+            acc, stream ? (hipStream_t)stream : b->stream);
+  return 0;
+}
+
+// Evaluates the current table at n values of s (device arrays), synchronously.
+PGAMD_API int pg5t_dev_eval(int n, const float* s, float* out) {
+  Board* b = board(0, "pg5t_dev_eval");
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  const float2* tab = table_dev(b->stream);
+  if (n > 0) {
+    hipLaunchKernelGGL(pgt::table_eval, dim3(ceil_div(n, 256)), dim3(256), 0, b->stream, n, s,
+                       tab, g_tab.ntab, g_tab.frc_bit, out);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+PGAMD_API int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
+  if (bi != 0 && bi != 64 && bi != 128 && bi != 256) {
+    api_error(3, "pg5t_dev_set_launch: bi=%d not in {0,64,128,256}", bi);
+    return -1;
+  }
+  g_force_bi = bi;
+  if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
+  if (min_jper > 0) g_min_jper = min_jper;
+  return 0;
+}
+
+PGAMD_API void* pg5t_dev_stream(int devid) {
+  Board* b = board(devid, "pg5t_dev_stream");
+  return b ? (void*)b->stream : nullptr;
+}

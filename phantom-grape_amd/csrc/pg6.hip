@@ -1,0 +1,805 @@
+// libpg6.so: GRAPE-6 API (PG6/libpg6/gp6util.h, phantom_g6.c:18-566) on
+// MI355X, plus device-pointer entry points (pg6_dev_*) that work on the same
+// board context without host round trips.
+//
+// Board model.  The reference keeps one global context (PG6/libpg6/gravity.c:
+// 26-47): a j-particle memory written by g6_set_j_particle, a predicted copy
+// rebuilt by g6calc_firsthalf when g6_set_ti has been called, and i-particles
+// handed to g6calc_lasthalf*.  Here the j-memory lives in HBM as SoA arrays;
+// g6_set_j_particle queues 120-byte records in pinned host memory (a later
+// write to the same address replaces the earlier one), firsthalf uploads the
+// queue in one copy and scatters it with a kernel, then predicts.  lasthalf*
+// uploads the i-set in one copy, runs the force kernel (split over j-ranges
+// when ni is small so every CU has work) and returns the results in one copy.
+// The gpid argument is ignored, as in the reference.
+#include "pgamd_common.h"
+#include "pgamd_api.h"
+#include "pg6_kernels.h"
+#include "pg6_nblist.h"
+
+#include <algorithm>
+#include <vector>
+
+using namespace pgamd;
+using pg6::JPred;
+using pg6::JRecord;
+using pg6::JState;
+using pg6::Outputs;
+using pg6::Partials;
+
+namespace {
+
+constexpr int kDefaultPipes = 48;        // g6_npipes() of the reference (avx_type.h:1)
+constexpr int kMaxLen = 1024;            // neighbour list length (avx_type.h:2)
+constexpr int kJMax = 1 << 26;           // j-memory limit
+constexpr int kInitialJCap = 1 << 18;    // reference JMEMSIZE (gravity.c:14)
+
+struct LaunchPlan {
+  int bi;      // i-particles per block
+  int nsplit;  // j-ranges
+  int jper;    // j-particles per range
+};
+
+struct Board {
+  bool open = false;
+  hipStream_t stream = nullptr;
+  int npipes = kDefaultPipes;
+  int cap = 0;  // j capacity
+  DevBuf js_buf, jp_buf;
+  JState js{};
+  JPred jp{};
+  // queued j records
+  HostBuf pend;
+  int npend = 0;
+  std::vector<int> slot;  // address -> queue slot, -1 if none
+  DevBuf pend_dev;
+  double ti = 0.0;
+  int predict_flag = 0;
+  int nj_pred = -1;  // j count of the last prediction
+  // i-set and outputs of the host API
+  HostBuf h_in, h_out;
+  DevBuf d_in, d_out, d_part;
+  // neighbour lists of the last lasthalf*n call
+  DevBuf d_nb;
+  std::vector<int> nbl, nblen;
+  int nb_ni = 0;
+  int nblerror = 0;
+  // launch tuning overrides (0 = automatic)
+  int force_bi = 0;
+  int blocks_per_cu = 8;
+  int min_jper = 128;
+};
+
+Board g_board;
+std::mutex g_mutex;
+
+// Optional per-launch timing of the force kernel: a HIP event pair recorded on
+// the launch stream immediately around each pg6_force launch.
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // 2 per launch
+  int used = 0;
+  void start(int pairs) {
+    for (hipEvent_t e : ev) PGAMD_HIP_CHECK(hipEventDestroy(e));
+    ev.assign(2 * (size_t)pairs, nullptr);
+    for (hipEvent_t& e : ev) PGAMD_HIP_CHECK(hipEventCreate(&e));
+    used = 0;
+    on = true;
+  }
+  void before(hipStream_t st) {
+    if (on && 2 * used + 1 < (int)ev.size()) PGAMD_HIP_CHECK(hipEventRecord(ev[2 * used], st));
+  }
+  void after(hipStream_t st) {
+    if (on && 2 * used + 1 < (int)ev.size()) {
+      PGAMD_HIP_CHECK(hipEventRecord(ev[2 * used + 1], st));
+      ++used;
+    }
+  }
+};
+KernelTimer g_timer;
+
+void board_alloc_j(Board& b, int cap) {
+  // Grow j-state and prediction arrays, preserving the j-state contents.
+  DevBuf njs, njp;
+  const size_t jsb = (size_t)cap * (14 * sizeof(double) + sizeof(int));
+  const size_t jpb = (size_t)cap * (3 * sizeof(double) + sizeof(float4) + sizeof(int));
+  njs.reserve(jsb);
+  njp.reserve(jpb);
+  PGAMD_HIP_CHECK(hipMemsetAsync(njs.ptr, 0, jsb, b.stream));
+  PGAMD_HIP_CHECK(hipMemsetAsync(njp.ptr, 0, jpb, b.stream));
+  JState js;
+  double* d = njs.as<double>();
+  double** f[14] = {&js.x, &js.y, &js.z, &js.vx, &js.vy, &js.vz, &js.ax,
+                    &js.ay, &js.az, &js.kx, &js.ky, &js.kz, &js.m, &js.t};
+  for (int k = 0; k < 14; ++k) *f[k] = d + (size_t)k * cap;
+  js.idx = reinterpret_cast<int*>(d + (size_t)14 * cap);
+  JPred jp;
+  double* e = njp.as<double>();
+  jp.x = e;
+  jp.y = e + cap;
+  jp.z = e + 2 * (size_t)cap;
+  jp.vm = reinterpret_cast<float4*>(e + 3 * (size_t)cap);
+  jp.id = reinterpret_cast<int*>(jp.vm + cap);
+  if (b.cap > 0) {
+    double** o[14] = {&b.js.x, &b.js.y, &b.js.z, &b.js.vx, &b.js.vy, &b.js.vz, &b.js.ax,
+                      &b.js.ay, &b.js.az, &b.js.kx, &b.js.ky, &b.js.kz, &b.js.m, &b.js.t};
+    for (int k = 0; k < 14; ++k)
+      PGAMD_HIP_CHECK(hipMemcpyAsync(*f[k], *o[k], (size_t)b.cap * sizeof(double),
+                                     hipMemcpyDeviceToDevice, b.stream));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(js.idx, b.js.idx, (size_t)b.cap * sizeof(int),
+                                   hipMemcpyDeviceToDevice, b.stream));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  }
+  b.js_buf.release();
+  b.jp_buf.release();
+  b.js_buf = njs;
+  b.jp_buf = njp;
+  b.js = js;
+  b.jp = jp;
+  b.cap = cap;
+  b.slot.assign((size_t)cap, -1);
+  b.npend = 0;
+  b.nj_pred = -1;
+}
+
+// Requires b.open.  Returns false (after reporting) if addr is out of range.
+bool board_reserve(Board& b, int need) {
+  if (need <= b.cap) return true;
+  if (need > kJMax) {
+    api_error(2, "j-particle address %d exceeds the j-memory limit %d", need - 1, kJMax);
+    return false;
+  }
+  // Flush queued records before the arrays move.
+  int cap = b.cap > 0 ? b.cap : kInitialJCap;
+  while (cap < need) cap *= 2;
+  if (cap > kJMax) cap = kJMax;
+  std::vector<JRecord> keep;
+  if (b.npend > 0) keep.assign(b.pend.as<JRecord>(), b.pend.as<JRecord>() + b.npend);
+  board_alloc_j(b, cap);
+  for (const JRecord& r : keep) {
+    b.slot[r.addr] = b.npend;
+    b.pend.as<JRecord>()[b.npend++] = r;
+  }
+  return true;
+}
+
+void board_open(Board& b) {
+  if (b.open) return;
+  ensure_device();
+  PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  if (const char* s = std::getenv("PGAMD_G6_NPIPES")) {
+    int v = std::atoi(s);
+    if (v > 0) b.npipes = v;
+  }
+  b.open = true;
+  board_alloc_j(b, kInitialJCap);
+  b.pend.reserve(sizeof(JRecord) * 1024);
+}
+
+void board_close(Board& b) {
+  if (!b.open) return;
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  b.js_buf.release();
+  b.jp_buf.release();
+  b.pend_dev.release();
+  b.d_in.release();
+  b.d_out.release();
+  b.d_part.release();
+  b.d_nb.release();
+  b.pend.release();
+  b.h_in.release();
+  b.h_out.release();
+  PGAMD_HIP_CHECK(hipStreamDestroy(b.stream));
+  b = Board();
+}
+
+bool board_check_open(const char* fn) {
+  if (!g_board.open) {
+    api_error(1, "%s called before g6_open", fn);
+    return false;
+  }
+  return true;
+}
+
+void board_queue_j(Board& b, int padr, int pidx, double t, double m, const double* acc2,
+                   const double* jrk6, const double* vel, const double* pos) {
+  int s = b.slot[padr];
+  if (s < 0) {
+    b.pend.reserve(sizeof(JRecord) * (size_t)(b.npend + 1), true);
+    s = b.npend++;
+    b.slot[padr] = s;
+  }
+  JRecord& r = b.pend.as<JRecord>()[s];
+  for (int k = 0; k < 3; ++k) {
+    r.pos[k] = pos[k];
+    r.vel[k] = vel[k];
+    r.acc2[k] = acc2[k];
+    r.jrk6[k] = jrk6[k];
+  }
+  r.mass = m;
+  r.time = t;
+  r.idx = pidx;
+  r.addr = padr;
+}
+
+void board_flush_j(Board& b) {
+  if (b.npend == 0) return;
+  const size_t bytes = sizeof(JRecord) * (size_t)b.npend;
+  b.pend_dev.reserve(bytes);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.pend_dev.ptr, b.pend.ptr, bytes, hipMemcpyHostToDevice,
+                                 b.stream));
+  hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
+                     b.npend, b.pend_dev.as<JRecord>(), b.js);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  // The pinned queue is reused by the next set_j_particle: wait for the copy.
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  const JRecord* r = b.pend.as<JRecord>();
+  for (int k = 0; k < b.npend; ++k) b.slot[r[k].addr] = -1;
+  b.npend = 0;
+}
+
+void board_predict(Board& b, int nj, hipStream_t st) {
+  if (nj <= 0) return;
+  hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
+                     b.js, b.jp);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  b.nj_pred = nj;
+}
+
+LaunchPlan plan_launch(const Board& b, int ni, int nj) {
+  LaunchPlan p;
+  if (b.force_bi > 0) {
+    p.bi = b.force_bi;
+  } else {
+    p.bi = ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64);
+  }
+  const int nbi = ceil_div(ni, p.bi);
+  const int target = device_cu_count() * b.blocks_per_cu;
+  int ns = std::max(1, ceil_div(target, nbi));
+  const int maxs = std::max(1, nj / std::max(1, b.min_jper));
+  ns = std::min(ns, maxs);
+  int jper = ceil_div(nj, ns);
+  jper = (jper + 63) / 64 * 64;
+  ns = std::max(1, ceil_div(nj, jper));
+  p.nsplit = ns;
+  p.jper = jper;
+  return p;
+}
+
+template <int BI, bool NN>
+void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const double* ivel,
+                    const int* iid, float eps2, int nj, const JPred& jp, const Partials& part,
+                    const Outputs& out, hipStream_t st) {
+  dim3 grid(ceil_div(ni, BI), p.nsplit);
+  g_timer.before(st);
+  hipLaunchKernelGGL((pg6::pg6_force<BI, NN>), grid, dim3(pg6::kThreads), 0, st, ni, ipos, ivel,
+                     iid, eps2, nj, p.jper, jp, part, out, p.nsplit);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  g_timer.after(st);
+  if (p.nsplit > 1) {
+    hipLaunchKernelGGL((pg6::pg6_reduce<NN>), dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
+                       p.nsplit, part, jp.id, out);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+}
+
+// Runs the force kernels for ni i-particles (device arrays) against the first
+// nj predicted j-particles.  out.nnb == nullptr selects the kernel without
+// nearest-neighbour tracking.
+void board_force(Board& b, int ni, int nj, const double* ipos, const double* ivel,
+                 const int* iid, double eps2, const Outputs& out, hipStream_t st) {
+  if (ni <= 0) return;
+  const LaunchPlan p = plan_launch(b, ni, nj > 0 ? nj : 1);
+  Partials part{};
+  if (p.nsplit > 1) {
+    const size_t n = (size_t)p.nsplit * ni;
+    b.d_part.reserve(n * (7 * sizeof(double) + sizeof(float) + sizeof(int)));
+    double* d = b.d_part.as<double>();
+    part.ax = d;
+    part.ay = d + n;
+    part.az = d + 2 * n;
+    part.pot = d + 3 * n;
+    part.jx = d + 4 * n;
+    part.jy = d + 5 * n;
+    part.jz = d + 6 * n;
+    part.rmax = reinterpret_cast<float*>(d + 7 * n);
+    part.nn = reinterpret_cast<int*>(part.rmax + n);
+  }
+  const bool nn = out.nnb != nullptr;
+  const float e2 = (float)eps2;
+  const int njr = nj > 0 ? nj : 0;
+#define PG6_LAUNCH(BI)                                                                     \
+  do {                                                                                     \
+    if (nn)                                                                                \
+      launch_force_t<BI, true>(p, ni, ipos, ivel, iid, e2, njr, b.jp, part, out, st);      \
+    else                                                                                   \
+      launch_force_t<BI, false>(p, ni, ipos, ivel, iid, e2, njr, b.jp, part, out, st);     \
+  } while (0)
+  switch (p.bi) {
+    case 256: PG6_LAUNCH(256); break;
+    case 128: PG6_LAUNCH(128); break;
+    default: PG6_LAUNCH(64); break;
+  }
+#undef PG6_LAUNCH
+}
+
+// Host-API lasthalf: flags select nearest neighbour, rnnb and neighbour lists.
+int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double (*pos)[3],
+                  const double (*vel)[3], double eps2, const double* h2, double (*acc)[3],
+                  double (*jrk)[3], double* pot, int* nnb, float* rnnb, bool want_nn,
+                  bool want_nblist) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open(fn)) return -1;
+  Board& b = g_board;
+  if (ni < 0 || nj < 0) {
+    api_error(3, "%s: negative ni (%d) or nj (%d)", fn, ni, nj);
+    return -1;
+  }
+  if (nj > b.cap) {
+    api_error(2, "%s: nj=%d exceeds the j-particles set (capacity %d)", fn, nj, b.cap);
+    return -1;
+  }
+  if (ni == 0) return 0;
+  // Pending j records or a missing prediction: do what firsthalf would.
+  board_flush_j(b);
+  if (b.predict_flag || b.nj_pred < nj) {
+    board_predict(b, nj, b.stream);
+    b.predict_flag = 0;
+  }
+  // i-set: pos[3] double, vel[3] double, id int, h2 float
+  const size_t in_bytes = (size_t)ni * (6 * sizeof(double) + sizeof(int) + sizeof(float));
+  b.h_in.reserve(in_bytes);
+  b.d_in.reserve(in_bytes);
+  double* hp = b.h_in.as<double>();
+  std::memcpy(hp, pos, sizeof(double) * 3 * (size_t)ni);
+  std::memcpy(hp + 3 * (size_t)ni, vel, sizeof(double) * 3 * (size_t)ni);
+  int* hid = reinterpret_cast<int*>(hp + 6 * (size_t)ni);
+  std::memcpy(hid, pidx, sizeof(int) * (size_t)ni);
+  float* hh2 = reinterpret_cast<float*>(hid + ni);
+  for (int i = 0; i < ni; ++i) hh2[i] = want_nblist && h2 ? (float)h2[i] : 0.f;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
+                                 b.stream));
+  const double* dpos = b.d_in.as<double>();
+  const double* dvel = dpos + 3 * (size_t)ni;
+  const int* did = reinterpret_cast<const int*>(dvel + 3 * (size_t)ni);
+  const float* dh2 = reinterpret_cast<const float*>(did + ni);
+
+  const size_t out_bytes = (size_t)ni * (7 * sizeof(double) + sizeof(int) + sizeof(float));
+  b.d_out.reserve(out_bytes);
+  b.h_out.reserve(out_bytes);
+  Outputs o;
+  o.acc = b.d_out.as<double>();
+  o.jrk = o.acc + 3 * (size_t)ni;
+  o.pot = o.jrk + 3 * (size_t)ni;
+  o.nnb = want_nn ? reinterpret_cast<int*>(o.pot + ni) : nullptr;
+  o.rnnb = reinterpret_cast<float*>(reinterpret_cast<int*>(o.pot + ni) + ni);
+  board_force(b, ni, nj, dpos, dvel, did, eps2, o, b.stream);
+
+  if (want_nblist) {
+    b.d_nb.reserve((size_t)ni * (kMaxLen + 1) * sizeof(int));
+    int* dnbl = b.d_nb.as<int>();
+    int* dlen = dnbl + (size_t)ni * kMaxLen;
+    hipLaunchKernelGGL(pg6::pg6_nblist, dim3(ni), dim3(pg6::kThreads), 0, b.stream, ni, dpos,
+                       did, dh2, (float)eps2, nj, b.jp, kMaxLen, dnbl, dlen);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    b.nbl.resize((size_t)ni * kMaxLen);
+    b.nblen.resize((size_t)ni);
+    PGAMD_HIP_CHECK(hipMemcpyAsync(b.nblen.data(), dlen, sizeof(int) * (size_t)ni,
+                                   hipMemcpyDeviceToHost, b.stream));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(b.nbl.data(), dnbl, sizeof(int) * (size_t)ni * kMaxLen,
+                                   hipMemcpyDeviceToHost, b.stream));
+  }
+  PGAMD_HIP_CHECK(hipMemcpyAsync(b.h_out.ptr, b.d_out.ptr, out_bytes, hipMemcpyDeviceToHost,
+                                 b.stream));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  const double* ha = b.h_out.as<double>();
+  const double* hj = ha + 3 * (size_t)ni;
+  const double* hpo = hj + 3 * (size_t)ni;
+  const int* hn = reinterpret_cast<const int*>(hpo + ni);
+  const float* hr = reinterpret_cast<const float*>(hn + ni);
+  std::memcpy(acc, ha, sizeof(double) * 3 * (size_t)ni);
+  std::memcpy(jrk, hj, sizeof(double) * 3 * (size_t)ni);
+  std::memcpy(pot, hpo, sizeof(double) * (size_t)ni);
+  if (want_nn && nnb) std::memcpy(nnb, hn, sizeof(int) * (size_t)ni);
+  if (want_nn && rnnb) std::memcpy(rnnb, hr, sizeof(float) * (size_t)ni);
+  if (want_nblist) {
+    b.nb_ni = ni;
+    for (int i = 0; i < ni; ++i)
+      if (b.nblen[i] > kMaxLen) b.nblerror = 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- GRAPE-6 API
+
+PGAMD_API void g6_open(int gpid) {
+  (void)gpid;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  board_open(g_board);
+}
+PGAMD_API void g6_open_(int* gpid) { g6_open(*gpid); }
+
+PGAMD_API void g6_close(int gpid) {
+  (void)gpid;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  board_close(g_board);
+}
+PGAMD_API void g6_close_(int* gpid) { g6_close(*gpid); }
+
+// Hardware-specific settings of the GRAPE-6 board: no-ops, as in the reference
+// (PG6/libpg6/phantom_g6.c:36-88,468-478).
+PGAMD_API void g6_set_tunit(int) {}
+PGAMD_API void g6_set_tunit_(int*) {}
+PGAMD_API void g6_set_xunit(int) {}
+PGAMD_API void g6_set_xunit_(int*) {}
+PGAMD_API void g6_reset(int) {}
+PGAMD_API void g6_reset_(int*) {}
+PGAMD_API void g6_reset_fofpga(int) {}
+PGAMD_API void g6_reset_fofpga_(int*) {}
+PGAMD_API void g6_setup_njdata(int, int) {}
+PGAMD_API void g6_setup_njdata_(int*, int*) {}
+
+PGAMD_API int g6_npipes(void) {
+  if (const char* s = std::getenv("PGAMD_G6_NPIPES")) {
+    int v = std::atoi(s);
+    if (v > 0) return v;
+  }
+  return g_board.open ? g_board.npipes : kDefaultPipes;
+}
+PGAMD_API int g6_npipes_(void) { return g6_npipes(); }
+
+PGAMD_API void g6_set_j_particle(int gpid, int padr, int pidx, double t, double dt, double mss,
+                                 double* snp, double* jrk, double* acc, double* vel,
+                                 double* pos) {
+  (void)gpid;
+  (void)dt;
+  (void)snp;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("g6_set_j_particle")) return;
+  if (padr < 0) {
+    api_error(2, "g6_set_j_particle: negative address %d", padr);
+    return;
+  }
+  if (!board_reserve(g_board, padr + 1)) return;
+  board_queue_j(g_board, padr, pidx, t, mss, acc, jrk, vel, pos);
+}
+PGAMD_API void g6_set_j_particle_(int* gpid, int* padr, int* pidx, double* t, double* dt,
+                                  double* mss, double* snp, double* jrk, double* acc,
+                                  double* vel, double* pos) {
+  g6_set_j_particle(*gpid, *padr, *pidx, *t, *dt, *mss, snp, jrk, acc, vel, pos);
+}
+
+PGAMD_API void g6_set_ti(int gpid, double ti) {
+  (void)gpid;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_board.ti = ti;
+  g_board.predict_flag = 1;
+}
+PGAMD_API void g6_set_ti_(int* gpid, double* ti) { g6_set_ti(*gpid, *ti); }
+
+PGAMD_API void g6calc_firsthalf(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                                double (*vel)[3], double (*acc)[3], double (*jrk)[3],
+                                double* pot, double eps2, double* h2) {
+  (void)gpid; (void)ni; (void)pidx; (void)pos; (void)vel; (void)acc; (void)jrk; (void)pot;
+  (void)eps2; (void)h2;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("g6calc_firsthalf")) return;
+  Board& b = g_board;
+  if (nj < 0 || nj > b.cap) {
+    api_error(2, "g6calc_firsthalf: nj=%d outside the j-memory (capacity %d)", nj, b.cap);
+    return;
+  }
+  board_flush_j(b);
+  if (b.predict_flag) {
+    board_predict(b, nj, b.stream);
+    b.predict_flag = 0;
+  }
+  b.nblerror = 0;
+}
+PGAMD_API void g6calc_firsthalf_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                 double (*vel)[3], double (*acc)[3], double (*jrk)[3],
+                                 double* pot, double* eps2, double* h2) {
+  g6calc_firsthalf(*gpid, *nj, *ni, pidx, pos, vel, acc, jrk, pot, *eps2, h2);
+}
+
+PGAMD_API int g6calc_lasthalf(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                              double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                              double (*jrk)[3], double* pot) {
+  (void)gpid;
+  int rc = lasthalf_impl("g6calc_lasthalf", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                         nullptr, nullptr, false, false);
+  return rc < 0 ? rc : 0;
+}
+PGAMD_API int g6calc_lasthalf_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                               double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                               double (*jrk)[3], double* pot) {
+  return g6calc_lasthalf(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot);
+}
+
+PGAMD_API int g6calc_lasthalf2(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                               double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                               double (*jrk)[3], double* pot, int* nnb) {
+  (void)gpid;
+  return lasthalf_impl("g6calc_lasthalf2", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                       nnb, nullptr, true, false);
+}
+PGAMD_API int g6calc_lasthalf2_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                                double (*jrk)[3], double* pot, int* nnb) {
+  return g6calc_lasthalf2(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot, nnb);
+}
+
+PGAMD_API int g6calc_lasthalf2p(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                                double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                                double (*jrk)[3], double* pot, int* nnb, float* rnnb) {
+  (void)gpid;
+  return lasthalf_impl("g6calc_lasthalf2p", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                       nnb, rnnb, true, false);
+}
+PGAMD_API int g6calc_lasthalf2p_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                 double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                                 double (*jrk)[3], double* pot, int* nnb, float* rnnb) {
+  return g6calc_lasthalf2p(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot, nnb, rnnb);
+}
+
+PGAMD_API int g6calc_lasthalfn(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                               double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                               double (*jrk)[3], double* pot) {
+  (void)gpid;
+  return lasthalf_impl("g6calc_lasthalfn", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                       nullptr, nullptr, false, true);
+}
+PGAMD_API int g6calc_lasthalfn_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                                double (*jrk)[3], double* pot) {
+  return g6calc_lasthalfn(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot);
+}
+
+PGAMD_API int g6calc_lasthalf2n(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                                double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                                double (*jrk)[3], double* pot, int* nnb) {
+  (void)gpid;
+  return lasthalf_impl("g6calc_lasthalf2n", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                       nnb, nullptr, true, true);
+}
+PGAMD_API int g6calc_lasthalf2n_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                 double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                                 double (*jrk)[3], double* pot, int* nnb) {
+  return g6calc_lasthalf2n(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot, nnb);
+}
+
+PGAMD_API int g6calc_lasthalf2np(int gpid, int nj, int ni, int* pidx, double (*pos)[3],
+                                 double (*vel)[3], double eps2, double* h2, double (*acc)[3],
+                                 double (*jrk)[3], double* pot, int* nnb, float* rnnb) {
+  (void)gpid;
+  return lasthalf_impl("g6calc_lasthalf2np", nj, ni, pidx, pos, vel, eps2, h2, acc, jrk, pot,
+                       nnb, rnnb, true, true);
+}
+PGAMD_API int g6calc_lasthalf2np_(int* gpid, int* nj, int* ni, int* pidx, double (*pos)[3],
+                                  double (*vel)[3], double* eps2, double* h2, double (*acc)[3],
+                                  double (*jrk)[3], double* pot, int* nnb, float* rnnb) {
+  return g6calc_lasthalf2np(*gpid, *nj, *ni, pidx, pos, vel, *eps2, h2, acc, jrk, pot, nnb,
+                            rnnb);
+}
+
+// 0 = every list of the last lasthalf*n call fitted; 1 = one overflowed
+// (PG6/libpg6/phantom_g6.c:482-486, gravity.c:905-906).
+PGAMD_API int g6_read_neighbour_list(int gpid) {
+  (void)gpid;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  return g_board.nblerror;
+}
+PGAMD_API int g6_read_neighbour_list_(int* gpid) { return g6_read_neighbour_list(*gpid); }
+
+// Copies the list of i-particle `ipipe` of the last lasthalf*n call.
+// Returns 1 if it is longer than maxlen (nothing copied), else 0
+// (PG6/libpg6/gravity.c:135-147).
+PGAMD_API int g6_get_neighbour_list(int gpid, int ipipe, int maxlen, int* nblen, int* nbl) {
+  (void)gpid;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  Board& b = g_board;
+  if (ipipe < 0 || ipipe >= b.nb_ni) {
+    api_error(3, "g6_get_neighbour_list: ipipe=%d outside the last call's %d i-particles",
+              ipipe, b.nb_ni);
+    return 1;
+  }
+  const int len = b.nblen[ipipe];
+  if (len > maxlen || len > kMaxLen) return 1;
+  *nblen = len;
+  std::memcpy(nbl, b.nbl.data() + (size_t)ipipe * kMaxLen, sizeof(int) * (size_t)len);
+  return 0;
+}
+PGAMD_API int g6_get_neighbour_list_(int* gpid, int* ipipe, int* maxlen, int* nblen, int* nbl) {
+  return g6_get_neighbour_list(*gpid, *ipipe, *maxlen, nblen, nbl);
+}
+
+// Debug dumps with the reference's file names and formats
+// (PG6/libpg6/phantom_g6.c:508-561).
+PGAMD_API void g6_debugfunc(int tmp) {
+  FILE* fp = std::fopen("dump.dat", "a");
+  if (!fp) return;
+  std::fprintf(fp, "%d\n", tmp);
+  std::fclose(fp);
+}
+PGAMD_API void g6_debugfunc_(int* tmp) { g6_debugfunc(*tmp); }
+PGAMD_API void g6_debugfunc_double(double tmp) {
+  FILE* fp = std::fopen("dump.dat", "a");
+  if (!fp) return;
+  std::fprintf(fp, "%+e\n", tmp);
+  std::fclose(fp);
+}
+PGAMD_API void g6_debugfunc_double_(double* tmp) { g6_debugfunc_double(*tmp); }
+PGAMD_API void g6_dump(double tim, int n, double* m, double (*x)[3], double (*v)[3]) {
+  char out[64];
+  std::snprintf(out, sizeof(out), "dump.dat.%05d", (int)tim);
+  FILE* fp = std::fopen(out, "w");
+  if (!fp) return;
+  for (int i = 0; i < n; i++) {
+    std::fprintf(fp, "%+.16e", m[i]);
+    std::fprintf(fp, " %+.16e %+.16e %+.16e", x[i][0], x[i][1], x[i][2]);
+    std::fprintf(fp, " %+.16e %+.16e %+.16e", v[i][0], v[i][1], v[i][2]);
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+}
+PGAMD_API void g6_dump_(double* tim, int* n, double* m, double (*x)[3], double (*v)[3]) {
+  g6_dump(*tim, *n, m, x, v);
+}
+
+// ------------------------------------------------- device-pointer entry points
+
+// Bulk load of j-particles 0..nj-1 from host arrays (same layout and
+// conventions as nj calls of g6_set_j_particle: acc2 = a/2, jrk6 = j/6).
+PGAMD_API int pg6_dev_set_j(int nj, const double* pos, const double* vel, const double* acc2,
+                            const double* jrk6, const double* mass, const double* tj,
+                            const int* idx) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_set_j")) return -1;
+  if (nj < 0) {
+    api_error(3, "pg6_dev_set_j: negative nj %d", nj);
+    return -1;
+  }
+  if (!board_reserve(g_board, nj)) return -1;
+  for (int j = 0; j < nj; ++j)
+    board_queue_j(g_board, j, idx[j], tj[j], mass[j], acc2 + 3 * (size_t)j,
+                  jrk6 + 3 * (size_t)j, vel + 3 * (size_t)j, pos + 3 * (size_t)j);
+  board_flush_j(g_board);
+  return 0;
+}
+
+// Predicts j-particles 0..nj-1 to time ti on `stream` (nullptr = library stream).
+PGAMD_API int pg6_dev_predict(int nj, double ti, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_predict")) return -1;
+  if (nj < 0 || nj > g_board.cap) {
+    api_error(2, "pg6_dev_predict: nj=%d outside the j-memory (capacity %d)", nj, g_board.cap);
+    return -1;
+  }
+  g_board.ti = ti;
+  g_board.predict_flag = 0;
+  board_predict(g_board, nj, stream ? (hipStream_t)stream : g_board.stream);
+  return 0;
+}
+
+// Force on ni i-particles held in device memory against the first nj
+// predicted j-particles.  All pointers are device pointers:
+//   pos[ni][3], vel[ni][3] (FP64), id[ni]; acc[ni][3], jrk[ni][3], pot[ni]
+//   (FP64); nnb[ni], rnnb[ni] may be null (then no nearest-neighbour search).
+// Asynchronous on `stream` (nullptr = library stream).  Calls that overlap in
+// time on different streams share the partial-sum workspace and race.
+PGAMD_API int pg6_dev_force(int nj, int ni, const double* pos, const double* vel, const int* id,
+                            double eps2, double* acc, double* jrk, double* pot, int* nnb,
+                            float* rnnb, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_force")) return -1;
+  if (ni < 0 || nj < 0 || nj > g_board.cap) {
+    api_error(2, "pg6_dev_force: ni=%d nj=%d (capacity %d)", ni, nj, g_board.cap);
+    return -1;
+  }
+  Outputs o{acc, jrk, pot, nnb, rnnb};
+  if (nnb && !rnnb) {
+    api_error(3, "pg6_dev_force: nnb given without rnnb");
+    return -1;
+  }
+  board_force(g_board, ni, nj, pos, vel, id, eps2, o,
+              stream ? (hipStream_t)stream : g_board.stream);
+  return 0;
+}
+
+PGAMD_API int pg6_dev_sync(void) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_sync")) return -1;
+  PGAMD_HIP_CHECK(hipStreamSynchronize(g_board.stream));
+  return 0;
+}
+
+PGAMD_API void* pg6_dev_stream(void) { return g_board.open ? (void*)g_board.stream : nullptr; }
+
+// Launch-shape override used by the autotuner: bi in {0 (auto), 64, 128, 256},
+// blocks_per_cu > 0 sets the j-split target, min_jper > 0 the smallest j-range.
+PGAMD_API int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (bi != 0 && bi != 64 && bi != 128 && bi != 256) {
+    api_error(3, "pg6_dev_set_launch: bi=%d not in {0,64,128,256}", bi);
+    return -1;
+  }
+  g_board.force_bi = bi;
+  if (blocks_per_cu > 0) g_board.blocks_per_cu = blocks_per_cu;
+  if (min_jper > 0) g_board.min_jper = min_jper;
+  return 0;
+}
+
+// The launch shape the library would use for (ni, nj): {bi, nsplit, jper}.
+PGAMD_API int pg6_dev_plan(int ni, int nj, int* out3) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_plan")) return -1;
+  LaunchPlan p = plan_launch(g_board, ni > 0 ? ni : 1, nj > 0 ? nj : 1);
+  out3[0] = p.bi;
+  out3[1] = p.nsplit;
+  out3[2] = p.jper;
+  return 0;
+}
+
+// Copies the predicted j-particles 0..nj-1 to host memory: pos[nj][3] FP64,
+// vel[nj][3] FP32 (as the force kernel reads them), mass[nj] FP32 (may be
+// null).  Synchronous.  For checking the predictor.
+PGAMD_API int pg6_dev_read_predicted(int nj, double* pos, float* vel, float* mass) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_read_predicted")) return -1;
+  Board& b = g_board;
+  if (nj < 0 || nj > b.cap) {
+    api_error(2, "pg6_dev_read_predicted: nj=%d outside the j-memory (capacity %d)", nj, b.cap);
+    return -1;
+  }
+  if (nj == 0) return 0;
+  std::vector<double> x((size_t)nj), y((size_t)nj), z((size_t)nj);
+  std::vector<float4> vm((size_t)nj);
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  PGAMD_HIP_CHECK(hipMemcpy(x.data(), b.jp.x, sizeof(double) * nj, hipMemcpyDeviceToHost));
+  PGAMD_HIP_CHECK(hipMemcpy(y.data(), b.jp.y, sizeof(double) * nj, hipMemcpyDeviceToHost));
+  PGAMD_HIP_CHECK(hipMemcpy(z.data(), b.jp.z, sizeof(double) * nj, hipMemcpyDeviceToHost));
+  PGAMD_HIP_CHECK(hipMemcpy(vm.data(), b.jp.vm, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+  for (int j = 0; j < nj; ++j) {
+    pos[3 * j] = x[j];
+    pos[3 * j + 1] = y[j];
+    pos[3 * j + 2] = z[j];
+    vel[3 * j] = vm[j].x;
+    vel[3 * j + 1] = vm[j].y;
+    vel[3 * j + 2] = vm[j].z;
+    if (mass) mass[j] = vm[j].w;
+  }
+  return 0;
+}
+
+// Kernel timing: pg6_dev_timing_start(n) records HIP events around each of the
+// next n force-kernel launches (on their launch stream);
+// pg6_dev_timing_read(&ms, &count) waits for them and returns the summed
+// kernel time in milliseconds and the number of launches timed, then stops.
+PGAMD_API int pg6_dev_timing_start(int max_launches) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_timing_start")) return -1;
+  if (max_launches <= 0 || max_launches > (1 << 16)) {
+    api_error(3, "pg6_dev_timing_start: %d launches not in [1, 65536]", max_launches);
+    return -1;
+  }
+  g_timer.start(max_launches);
+  return 0;
+}
+
+PGAMD_API int pg6_dev_timing_read(double* ms, int* count) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  double tot = 0.0;
+  for (int k = 0; k < g_timer.used; ++k) {
+    PGAMD_HIP_CHECK(hipEventSynchronize(g_timer.ev[2 * k + 1]));
+    float t = 0.f;
+    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, g_timer.ev[2 * k], g_timer.ev[2 * k + 1]));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (count) *count = g_timer.used;
+  g_timer.on = false;
+  g_timer.used = 0;
+  return 0;
+}

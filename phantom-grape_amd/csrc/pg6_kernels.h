@@ -1,0 +1,302 @@
+// GRAPE-6 (Hermite) device kernels for gfx950.
+//
+//  * pg6_scatter_j   : applies the j-particle records queued by
+//                      g6_set_j_particle to the device j-state (SoA, FP64).
+//  * pg6_predict     : 3rd-order Taylor predictor of every j-particle to the
+//                      block time, FP64 position, FP32 velocity
+//                      (PG6/libpg6/gravity.c:19-20,149-193).
+//  * pg6_force<BI,NN>: acc, jerk, pot (+ nearest neighbour) of BI i-particles
+//                      per block against one j-range
+//                      (PG6/libpg6/gravity.c:260-453, 455-678).
+//  * pg6_reduce      : fixed-order sum of the per-j-range partials.
+//
+// Arithmetic (matches the reference's precision split, gravity.c:313-421):
+// Δx is formed in FP64 and rounded to FP32; pair math is FP32 with the
+// hardware v_rsq_f32; acc and pot are summed in FP32 over one LDS tile and
+// folded into FP64 accumulators per tile (jerk too, where the reference keeps
+// an FP32 sum: a strict accuracy gain at no measurable cost).
+// The self pair is excluded by particle id (gravity.c:348-351).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <climits>
+
+namespace pg6 {
+
+constexpr int kThreads = 256;  // threads per block: 4 wave64
+constexpr int kJTile = 256;    // j-particles staged in LDS per tile
+
+// One queued j-particle record, as uploaded by g6calc_firsthalf.  The host
+// passes acc/2 and jerk/6 (PG6/banana/banana01.c:199-212), stored as given.
+struct JRecord {
+  double pos[3], vel[3], acc2[3], jrk6[3];
+  double mass, time;
+  int idx, addr;
+};
+static_assert(sizeof(JRecord) == 120, "JRecord layout");
+
+// Device j-state, SoA, capacity `cap`.
+struct JState {
+  double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *kx, *ky, *kz, *m, *t;
+  int* idx;
+};
+
+// Predicted j-particles, SoA.
+struct JPred {
+  double *x, *y, *z;
+  float4* vm;  // predicted velocity (FP32) and mass (FP32)
+  int* id;
+};
+
+// Per-j-range partial results, SoA over [split][i].
+struct Partials {
+  double *ax, *ay, *az, *pot, *jx, *jy, *jz;
+  float* rmax;
+  int* nn;  // j address of the nearest neighbour, -1 if none
+};
+
+// Final outputs in the API's layout: acc[i][3], jrk[i][3], pot[i], nnb[i], rnnb[i].
+struct Outputs {
+  double* acc;
+  double* jrk;
+  double* pot;
+  int* nnb;
+  float* rnnb;
+};
+
+__global__ void pg6_scatter_j(int n, const JRecord* __restrict__ rec, JState js) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const JRecord r = rec[k];
+  const int a = r.addr;
+  js.x[a] = r.pos[0];  js.y[a] = r.pos[1];  js.z[a] = r.pos[2];
+  js.vx[a] = r.vel[0]; js.vy[a] = r.vel[1]; js.vz[a] = r.vel[2];
+  js.ax[a] = r.acc2[0]; js.ay[a] = r.acc2[1]; js.az[a] = r.acc2[2];
+  js.kx[a] = r.jrk6[0]; js.ky[a] = r.jrk6[1]; js.kz[a] = r.jrk6[2];
+  js.m[a] = r.mass;
+  js.t[a] = r.time;
+  js.idx[a] = r.idx;
+}
+
+// Predictor, evaluated left to right as the reference's macro
+// predict(dt,x,v,a2,j6) = x + v*dt + a2*dt*dt + j6*dt*dt*dt (gravity.c:19-20),
+// velocity = v + (2 a2) dt + (3 j6) dt dt (gravity.c:171-176), with FMA
+// contraction disabled so the result is the exactly rounded IEEE sequence.
+__device__ inline double pg6_pred_pos(double dt, double x, double v, double a2, double j6) {
+#pragma clang fp contract(off)
+  return x + v * dt + a2 * dt * dt + j6 * dt * dt * dt;
+}
+__device__ inline double pg6_pred_vel(double dt, double v, double a2, double j6) {
+#pragma clang fp contract(off)
+  return v + (2.0 * a2) * dt + (3.0 * j6) * dt * dt;
+}
+
+__global__ void pg6_predict(int nj, double ti, JState js, JPred jp) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const double dt = ti - js.t[j];
+  jp.x[j] = pg6_pred_pos(dt, js.x[j], js.vx[j], js.ax[j], js.kx[j]);
+  jp.y[j] = pg6_pred_pos(dt, js.y[j], js.vy[j], js.ay[j], js.ky[j]);
+  jp.z[j] = pg6_pred_pos(dt, js.z[j], js.vz[j], js.az[j], js.kz[j]);
+  float4 vm;
+  vm.x = (float)pg6_pred_vel(dt, js.vx[j], js.ax[j], js.kx[j]);
+  vm.y = (float)pg6_pred_vel(dt, js.vy[j], js.ay[j], js.ky[j]);
+  vm.z = (float)pg6_pred_vel(dt, js.vz[j], js.az[j], js.kz[j]);
+  vm.w = (float)js.m[j];
+  jp.vm[j] = vm;
+  jp.id[j] = js.idx[j];
+}
+
+// Force + jerk + potential (+ nearest neighbour) kernel.
+//
+// Block = 256 threads.  BI i-particles per block (64, 128 or 256); the block's
+// G = 256/BI wave groups split each LDS tile's j-particles between them
+// (group g takes j = g, g+G, ...), and are combined in fixed order at the end.
+// blockIdx.y selects the j-range [y*jper, min(nj,(y+1)*jper)).
+// With nsplit == 1 the kernel writes final outputs, otherwise partials.
+template <int BI, bool NN>
+__global__ __launch_bounds__(kThreads) void pg6_force(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp,
+    Partials part, Outputs out, int nsplit) {
+  constexpr int G = kThreads / BI;
+  __shared__ double s_x[kJTile], s_y[kJTile], s_z[kJTile];
+  __shared__ float4 s_vm[kJTile];
+  __shared__ int s_id[kJTile];
+
+  const int tid = threadIdx.x;
+  const int il = tid % BI;
+  const int g = tid / BI;
+  const int i = blockIdx.x * BI + il;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;
+
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const float vxi = (float)ivel[3 * ic + 0];
+  const float vyi = (float)ivel[3 * ic + 1];
+  const float vzi = (float)ivel[3 * ic + 2];
+  const int idi = iid[ic];
+
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min(nj, j0 + jper);
+
+  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
+  double jx = 0.0, jy = 0.0, jz = 0.0;
+  float rmax = 0.f;  // largest 1/r seen = nearest neighbour
+  int nn = -1;
+
+  for (int jt = j0; jt < j1; jt += kJTile) {
+    const int nt = min(kJTile, j1 - jt);
+    __syncthreads();
+    if (tid < nt) {
+      const int j = jt + tid;
+      s_x[tid] = jp.x[j];
+      s_y[tid] = jp.y[j];
+      s_z[tid] = jp.z[j];
+      s_vm[tid] = jp.vm[j];
+      s_id[tid] = jp.id[j];
+    }
+    __syncthreads();
+
+    float fax = 0.f, fay = 0.f, faz = 0.f, fpot = 0.f;
+    float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+#pragma unroll 4
+    for (int jj = g; jj < nt; jj += G) {
+      const float dx = (float)(s_x[jj] - xi);
+      const float dy = (float)(s_y[jj] - yi);
+      const float dz = (float)(s_z[jj] - zi);
+      const float4 vm = s_vm[jj];
+      const float dvx = vm.x - vxi;
+      const float dvy = vm.y - vyi;
+      const float dvz = vm.z - vzi;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+      const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+      const bool self = s_id[jj] == idi;
+      const float rinv = self ? 0.f : __builtin_amdgcn_rsqf(r2);
+      const float rinv2 = rinv * rinv;
+      const float mr = vm.w * rinv;
+      const float mr3 = mr * rinv2;
+      const float beta = -3.f * rv * rinv2;
+      fpot -= mr;
+      fax = fmaf(mr3, dx, fax);
+      fay = fmaf(mr3, dy, fay);
+      faz = fmaf(mr3, dz, faz);
+      fjx = fmaf(mr3, fmaf(beta, dx, dvx), fjx);
+      fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
+      fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
+      if (NN) {
+        const bool closer = rinv > rmax;
+        rmax = closer ? rinv : rmax;
+        nn = closer ? jt + jj : nn;
+      }
+    }
+    ax += (double)fax;
+    ay += (double)fay;
+    az += (double)faz;
+    pot += (double)fpot;
+    jx += (double)fjx;
+    jy += (double)fjy;
+    jz += (double)fjz;
+  }
+
+  // Combine the G wave groups in fixed order g = 0..G-1.
+  if (G > 1) {
+    __shared__ double r_d[7][kThreads];
+    __shared__ float r_f[kThreads];
+    __shared__ int r_n[kThreads];
+    __syncthreads();
+    r_d[0][tid] = ax; r_d[1][tid] = ay; r_d[2][tid] = az; r_d[3][tid] = pot;
+    r_d[4][tid] = jx; r_d[5][tid] = jy; r_d[6][tid] = jz;
+    r_f[tid] = rmax;
+    r_n[tid] = nn;
+    __syncthreads();
+    if (g == 0) {
+      for (int q = 1; q < G; ++q) {
+        const int t = q * BI + il;
+        ax += r_d[0][t]; ay += r_d[1][t]; az += r_d[2][t]; pot += r_d[3][t];
+        jx += r_d[4][t]; jy += r_d[5][t]; jz += r_d[6][t];
+        if (NN) {
+          const float rq = r_f[t];
+          const int nq = r_n[t];
+          // Larger 1/r wins; on a tie the lower j address wins.
+          if (rq > rmax || (rq == rmax && nq >= 0 && (nn < 0 || nq < nn))) {
+            rmax = rq;
+            nn = nq;
+          }
+        }
+      }
+    }
+  }
+  if (g != 0 || !ivalid) return;
+
+  if (nsplit == 1) {
+    out.acc[3 * i + 0] = ax;
+    out.acc[3 * i + 1] = ay;
+    out.acc[3 * i + 2] = az;
+    out.jrk[3 * i + 0] = jx;
+    out.jrk[3 * i + 1] = jy;
+    out.jrk[3 * i + 2] = jz;
+    out.pot[i] = pot;
+    if (NN) {
+      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+      out.rnnb[i] = rmax > 0.f ? 1.f / rmax : 0.f;
+    }
+  } else {
+    const size_t o = (size_t)blockIdx.y * ni + i;
+    part.ax[o] = ax;
+    part.ay[o] = ay;
+    part.az[o] = az;
+    part.pot[o] = pot;
+    part.jx[o] = jx;
+    part.jy[o] = jy;
+    part.jz[o] = jz;
+    if (NN) {
+      part.rmax[o] = rmax;
+      part.nn[o] = nn;
+    }
+  }
+}
+
+// Sums the nsplit partials of each i in split order (deterministic).
+template <bool NN>
+__global__ void pg6_reduce(int ni, int nsplit, Partials part, const int* __restrict__ jid,
+                           Outputs out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+  float rmax = 0.f;
+  int nn = -1;
+  for (int s = 0; s < nsplit; ++s) {
+    const size_t o = (size_t)s * ni + i;
+    ax += part.ax[o];
+    ay += part.ay[o];
+    az += part.az[o];
+    pot += part.pot[o];
+    jx += part.jx[o];
+    jy += part.jy[o];
+    jz += part.jz[o];
+    if (NN) {
+      // Splits cover increasing j addresses: strict '>' keeps the lowest address on ties.
+      const float r = part.rmax[o];
+      if (r > rmax) {
+        rmax = r;
+        nn = part.nn[o];
+      }
+    }
+  }
+  out.acc[3 * i + 0] = ax;
+  out.acc[3 * i + 1] = ay;
+  out.acc[3 * i + 2] = az;
+  out.jrk[3 * i + 0] = jx;
+  out.jrk[3 * i + 1] = jy;
+  out.jrk[3 * i + 2] = jz;
+  out.pot[i] = pot;
+  if (NN) {
+    out.nnb[i] = nn >= 0 ? jid[nn] : -1;
+    out.rnnb[i] = rmax > 0.f ? 1.f / rmax : 0.f;
+  }
+}
+
+}  // namespace pg6

@@ -1,0 +1,296 @@
+"""Python mirror of the MI355X Phantom-GRAPE C ABI.
+
+The product is the three shared libraries built from ``csrc/``:
+
+* ``libpg5.so``       GRAPE-5 API ``g5_*`` + quad_tree ``g5c_*`` + ``pg5_dev_*``
+  (reference ``PG5/newton/libpg5/gp5util.h``, ``PG5/newton_quad/quad/quad_util.h``)
+* ``libpg5_sym.so``   the same, built with symmetric softening (reference
+  ``-DSYMMETRIC``, ``PG5/newton/setting.mk:6,17-19``)
+* ``libpg5_table.so`` GRAPE-5 tabulated-cutoff API + ``pg5_*`` table calls +
+  learned-cutoff and batched TreePM entry points
+  (reference ``PG5/table/libpg5/gp5util.h``, ``pg5_table.h``)
+* ``libpg6.so``       GRAPE-6 API ``g6_*`` / ``g6calc_*`` + ``pg6_dev_*``
+  (reference ``PG6/libpg6/gp6util.h``)
+
+This module loads them with ctypes (one private handle each, so the ``g5_*``
+names of ``libpg5.so`` and ``libpg5_table.so`` never clash) and wraps the calls
+for numpy arrays.  Function names, argument meaning and order follow the C API.
+There is no CPU fallback: if a library is missing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+LIBS = ("pg5", "pg5_sym", "pg5_table", "pg6")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(LIBDIR, f"lib{name}.so")
+
+
+_handles: dict[str, C.CDLL] = {}
+
+
+def load(name: str) -> C.CDLL:
+    """Load ``lib<name>.so`` from the package's lib/ directory (built in-tree)."""
+    if name in _handles:
+        return _handles[name]
+    path = lib_path(name)
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run __graft_entry__.build() (hipcc for gfx950); "
+            "this package has no CPU fallback")
+    lib = C.CDLL(path, mode=os.RTLD_LOCAL | os.RTLD_NOW)
+    _handles[name] = lib
+    return lib
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def last_error(lib: C.CDLL) -> tuple[int, str]:
+    buf = C.create_string_buffer(512)
+    code = lib.pgamd_last_error(buf, 512)
+    return code, buf.value.decode()
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+# ----------------------------------------------------------------- GRAPE-5
+class G5:
+    """GRAPE-5 Newtonian board (libpg5.so).  Potential is returned with the
+    API's sign: ``+sum m/r`` including the self term (``phantom_g5.c:45-60``);
+    callers negate it and remove ``m/eps`` as ``direct.c:165-172`` does."""
+
+    def __init__(self, symmetric: bool = False):
+        self.lib = load("pg5_sym" if symmetric else "pg5")
+        self.symmetric = symmetric
+        L = self.lib
+        _sig(L, "g5_open", None)
+        _sig(L, "g5_close", None)
+        _sig(L, "g5_get_number_of_pipelines", C.c_int)
+        _sig(L, "g5_get_jmemsize", C.c_int)
+        _sig(L, "g5_set_eps_to_all", None, C.c_double)
+        _sig(L, "g5_set_n", None, C.c_int)
+        _sig(L, "g5_set_nMC", None, C.c_int, C.c_int)
+        if symmetric:
+            _sig(L, "g5_set_xmjMC", None, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp)
+        else:
+            _sig(L, "g5_set_xmjMC", None, C.c_int, C.c_int, C.c_int, _dp, _dp)
+        _sig(L, "g5_set_xiMC", None, C.c_int, C.c_int, _dp)
+        _sig(L, "g5_set_xeiMC", None, C.c_int, C.c_int, _dp, _dp)
+        _sig(L, "g5_set_epsMC", None, C.c_int, C.c_int, _dp)
+        _sig(L, "g5_runMC", None, C.c_int)
+        _sig(L, "g5_get_forceMC", None, C.c_int, C.c_int, _dp, _dp)
+        _sig(L, "g5_calculate_force_on_xMC", None, C.c_int, _dp, _dp, _dp, C.c_int)
+        _sig(L, "g5_calculate_force_on_xeMC", None, C.c_int, _dp, _dp, _dp, _dp, C.c_int)
+        _sig(L, "g5c_set_nMC", None, C.c_int, C.c_int)
+        _sig(L, "g5c_set_xmjMC", None, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp)
+        _sig(L, "g5c_set_xmjMC0", None, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp)
+        _sig(L, "g5c_calculate_force_on_xMC", None, C.c_int, _dp, _dp, _dp, C.c_int)
+        _sig(L, "g5c_calculate_force_on_xMC0", None, C.c_int, _dp, _dp, _dp, C.c_int)
+        _sig(L, "g5c_runMC", None, C.c_int)
+        _sig(L, "g5c_runMC0", None, C.c_int)
+        _sig(L, "pg5_dev_force", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp,
+             _vp)
+        _sig(L, "pg5_dev_quad", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
+        _sig(L, "pg5_dev_stream", _vp, C.c_int)
+        _sig(L, "pg5_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
+        _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
+        _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
+        L.g5_open()
+
+    def set_xmj(self, adr, x, m, epsj2=None, devid=0):
+        """``g5_set_xmj(adr, nj, xj, mj)``: x is indexed from ``adr`` (the
+        reference reads ``xj[j]`` for ``j`` in ``[adr, adr+nj)``)."""
+        x = f64(x)
+        m = f64(m)
+        nj = len(m) - adr
+        if self.symmetric:
+            e = f64(epsj2)
+            self.lib.g5_set_xmjMC(devid, adr, nj, dptr(x), dptr(m), dptr(e))
+        else:
+            self.lib.g5_set_xmjMC(devid, adr, nj, dptr(x), dptr(m))
+
+    def set_eps_to_all(self, eps):
+        self.lib.g5_set_eps_to_all(float(eps))
+
+    def set_n(self, n, devid=0):
+        self.lib.g5_set_nMC(devid, int(n))
+
+    def calculate_force_on_x(self, x, devid=0, eps2=None):
+        x = f64(x)
+        ni = x.shape[0]
+        a = np.zeros((ni, 3))
+        p = np.zeros(ni)
+        if eps2 is None:
+            self.lib.g5_calculate_force_on_xMC(devid, dptr(x), dptr(a), dptr(p), ni)
+        else:
+            e = f64(eps2)
+            self.lib.g5_calculate_force_on_xeMC(devid, dptr(x), dptr(e), dptr(a), dptr(p), ni)
+        return a, p
+
+    def set_cells(self, x, m, q, devid=0, mem=0):
+        x, m, q = f64(x), f64(m), f64(q)
+        f = self.lib.g5c_set_xmjMC if mem == 0 else self.lib.g5c_set_xmjMC0
+        f(devid, 0, len(m), dptr(x), dptr(m), dptr(q))
+        self.lib.g5c_set_nMC(devid, len(m))
+
+    def cell_force_on_x(self, x, devid=0, mem=0):
+        x = f64(x)
+        ni = x.shape[0]
+        a = np.zeros((ni, 3))
+        p = np.zeros(ni)
+        f = self.lib.g5c_calculate_force_on_xMC if mem == 0 else self.lib.g5c_calculate_force_on_xMC0
+        f(devid, dptr(x), dptr(a), dptr(p), ni)
+        return a, p
+
+
+# ----------------------------------------------------------------- GRAPE-6
+class G6:
+    """GRAPE-6 board (libpg6.so).  j-particle inputs carry acc/2 and jerk/6
+    (``PG6/banana/banana01.c:199-212``); outputs are acc, jerk and
+    ``pot = -sum m/r`` with the self pair excluded by id."""
+
+    def __init__(self):
+        self.lib = L = load("pg6")
+        _sig(L, "g6_open", None, C.c_int)
+        _sig(L, "g6_close", None, C.c_int)
+        _sig(L, "g6_npipes", C.c_int)
+        _sig(L, "g6_set_j_particle", None, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+             C.c_double, _dp, _dp, _dp, _dp, _dp)
+        _sig(L, "g6_set_ti", None, C.c_int, C.c_double)
+        _sig(L, "g6calc_firsthalf", None, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp,
+             _dp, C.c_double, _dp)
+        for nm in ("g6calc_lasthalf", "g6calc_lasthalfn"):
+            _sig(L, nm, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, _dp, _dp,
+                 _dp, _dp)
+        for nm in ("g6calc_lasthalf2", "g6calc_lasthalf2n"):
+            _sig(L, nm, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, _dp, _dp,
+                 _dp, _dp, _ip)
+        for nm in ("g6calc_lasthalf2p", "g6calc_lasthalf2np"):
+            _sig(L, nm, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, _dp, _dp,
+                 _dp, _dp, _ip, _fp)
+        _sig(L, "g6_read_neighbour_list", C.c_int, C.c_int)
+        _sig(L, "g6_get_neighbour_list", C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip)
+        _sig(L, "pg6_dev_set_j", C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip)
+        _sig(L, "pg6_dev_predict", C.c_int, C.c_int, C.c_double, _vp)
+        _sig(L, "pg6_dev_force", C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_double, _vp,
+             _vp, _vp, _vp, _vp, _vp)
+        _sig(L, "pg6_dev_sync", C.c_int)
+        _sig(L, "pg6_dev_stream", _vp)
+        _sig(L, "pg6_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg6_dev_plan", C.c_int, C.c_int, C.c_int, _ip)
+        _sig(L, "pg6_dev_read_predicted", C.c_int, C.c_int, _dp, _fp, _fp)
+        _sig(L, "pg6_dev_timing_start", C.c_int, C.c_int)
+        _sig(L, "pg6_dev_timing_read", C.c_int, _dp, _ip)
+        _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
+        _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
+        L.g6_open(0)
+
+    def close(self):
+        self.lib.g6_close(0)
+
+    def set_j_particles(self, pos, vel, acc2, jrk6, mass, tj, idx, adr0=0):
+        """One ``g6_set_j_particle`` call per particle, addresses adr0.."""
+        pos, vel, acc2, jrk6 = f64(pos), f64(vel), f64(acc2), f64(jrk6)
+        z = np.zeros(3)
+        for k in range(len(mass)):
+            self.lib.g6_set_j_particle(0, adr0 + k, int(idx[k]), float(tj[k]), 0.0,
+                                       float(mass[k]), dptr(z), dptr(jrk6[k]), dptr(acc2[k]),
+                                       dptr(vel[k]), dptr(pos[k]))
+
+    def set_j_bulk(self, pos, vel, acc2, jrk6, mass, tj, idx):
+        return self.lib.pg6_dev_set_j(len(mass), dptr(f64(pos)), dptr(f64(vel)), dptr(f64(acc2)),
+                                      dptr(f64(jrk6)), dptr(f64(mass)), dptr(f64(tj)),
+                                      iptr(i32(idx)))
+
+    def set_ti(self, t):
+        self.lib.g6_set_ti(0, float(t))
+
+    def calc(self, nj, pos, vel, idx, eps2, h2=None, variant="2p"):
+        """g6calc_firsthalf followed by g6calc_lasthalf<variant> over all i in
+        one call.  Returns dict of outputs."""
+        pos, vel, idx = f64(pos), f64(vel), i32(idx)
+        ni = len(idx)
+        h2a = f64(h2 if h2 is not None else np.zeros(ni))
+        acc = np.zeros((ni, 3))
+        jrk = np.zeros((ni, 3))
+        pot = np.zeros(ni)
+        nnb = np.zeros(ni, np.int32)
+        rnnb = np.zeros(ni, np.float32)
+        dummy = np.zeros((ni, 3))
+        self.lib.g6calc_firsthalf(0, nj, ni, iptr(idx), dptr(pos), dptr(vel), dptr(dummy),
+                                  dptr(dummy), dptr(pot), float(eps2), dptr(h2a))
+        L = self.lib
+        args = (0, nj, ni, iptr(idx), dptr(pos), dptr(vel), float(eps2), dptr(h2a), dptr(acc),
+                dptr(jrk), dptr(pot))
+        if variant == "":
+            rc = L.g6calc_lasthalf(*args)
+        elif variant == "2":
+            rc = L.g6calc_lasthalf2(*args, iptr(nnb))
+        elif variant == "2p":
+            rc = L.g6calc_lasthalf2p(*args, iptr(nnb), fptr(rnnb))
+        elif variant == "n":
+            rc = L.g6calc_lasthalfn(*args)
+        elif variant == "2n":
+            rc = L.g6calc_lasthalf2n(*args, iptr(nnb))
+        elif variant == "2np":
+            rc = L.g6calc_lasthalf2np(*args, iptr(nnb), fptr(rnnb))
+        else:
+            raise ValueError(variant)
+        return dict(rc=rc, acc=acc, jrk=jrk, pot=pot, nnb=nnb, rnnb=rnnb)
+
+    def read_predicted(self, nj):
+        x = np.zeros((nj, 3))
+        v = np.zeros((nj, 3), np.float32)
+        m = np.zeros(nj, np.float32)
+        rc = self.lib.pg6_dev_read_predicted(nj, dptr(x), fptr(v), fptr(m))
+        assert rc == 0
+        return x, v, m
+
+    def plan(self, ni, nj):
+        o = np.zeros(3, np.int32)
+        self.lib.pg6_dev_plan(ni, nj, iptr(o))
+        return tuple(int(v) for v in o)
+
+    def neighbour_list(self, ipipe, maxlen=1024):
+        n = np.zeros(1, np.int32)
+        buf = np.zeros(max(maxlen, 1), np.int32)
+        rc = self.lib.g6_get_neighbour_list(0, ipipe, maxlen, iptr(n), iptr(buf))
+        return rc, buf[: n[0]].copy()

@@ -1,0 +1,180 @@
+"""GRAPE-6 parity on the GPU: libpg6.so through its C ABI against the CPU
+oracle and the reference's own outputs (golden fixture)."""
+import numpy as np
+import pytest
+
+import ics
+import oracle as O
+from conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g6():
+    import pgamd
+    return pgamd.G6()
+
+
+def plummer_state(n, seed, eps):
+    m, x, v = ics.plummer(n, seed)
+    idx = np.arange(n, dtype=np.int32)
+    o = O.pg6_force(x, v, idx, eps * eps, x, v, m, idx)
+    rng = np.random.default_rng(seed + 100)
+    tj = -rng.integers(0, 4, n) / 512.0
+    return m, x, v, o["acc"] / 2, o["jrk"] / 6, tj, idx
+
+
+def test_predictor_bit_exact(g6):
+    m, x, v, a2, j6, tj, idx = plummer_state(3000, 1, 1 / 256)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    ti = 1.0 / 256
+    g6.set_ti(ti)
+    g6.calc(len(m), x[:1], v[:1], idx[:1], 1e-4)  # firsthalf predicts
+    xp, vp, mp = g6.read_predicted(len(m))
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    np.testing.assert_array_equal(xp, xo)
+    np.testing.assert_array_equal(vp, vo.astype(np.float32))
+    np.testing.assert_array_equal(mp, m.astype(np.float32))
+
+
+def test_matches_reference_fixture(g6):
+    g = golden("pg6_hermite.npz")
+    n = len(g["m"])
+    idx = np.arange(n, dtype=np.int32)
+    g6.set_j_particles(g["x"], g["v"], g["acc2"], g["jrk6"], g["m"], g["tj"], idx)
+    g6.set_ti(float(g["ti"]))
+    r = g6.calc(n, g["xi"], g["vi"], idx[g["isel"]], float(g["eps2"]), variant="2p")
+    assert r["rc"] == 0
+    assert np.percentile(rel_err(r["acc"], g["acc"]), 99) < 1e-6
+    assert np.percentile(rel_err(r["jrk"], g["jrk"]), 99) < 2e-5
+    assert rel_err(r["pot"], g["pot"]).max() < 1e-6
+    np.testing.assert_array_equal(r["nnb"], g["nnb"])
+    assert rel_err(r["rnnb"], g["rnnb"]).max() < 1e-5
+    # neighbour lists, same call sequence as the reference run
+    rn = g6.calc(n, g["xi"], g["vi"], idx[g["isel"]], float(g["eps2"]), h2=g["h2"], variant="n")
+    assert rn["rc"] == 0 and g6.lib.g6_read_neighbour_list(0) == 0
+    got = []
+    for q in range(len(g["isel"])):
+        rc, l = g6.neighbour_list(q)
+        assert rc == 0
+        got.append(l)
+    np.testing.assert_array_equal([len(l) for l in got], g["nblen"])
+    np.testing.assert_array_equal(np.concatenate(got), g["nbl"])
+
+
+@pytest.mark.parametrize("ni", [1, 7, 48, 64, 500, 4096])
+def test_force_vs_fp64_oracle(g6, ni):
+    n = 4096
+    eps = 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 2, eps)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    ti = 1.0 / 512
+    g6.set_ti(ti)
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    rng = np.random.default_rng(ni)
+    isel = rng.choice(n, ni, replace=False)
+    r = g6.calc(n, xo[isel], vo[isel], idx[isel], eps * eps, variant="2p")
+    # oracle with the velocities the GPU uses (FP32-rounded j velocities)
+    o = O.pg6_force(xo[isel], vo[isel], idx[isel], eps * eps, xo,
+                    vo.astype(np.float32).astype(np.float64), m, idx)
+    ea = rel_err(r["acc"], o["acc"])
+    ej = rel_err(r["jrk"], o["jrk"])
+    assert np.percentile(ea, 99) < 1e-6, np.percentile(ea, 99)
+    assert np.percentile(ej, 99) < 1e-5, np.percentile(ej, 99)
+    assert rel_err(r["pot"], o["pot"]).max() < 1e-6
+    assert rel_err(r["rnnb"], o["rnnb"]).max() < 1e-5
+    # nearest neighbour: identical except where the two nearest are FP32-tied
+    bad = np.nonzero(r["nnb"] != o["nnb"])[0]
+    for i in bad:
+        d = xo - xo[isel[i]]
+        r2 = (d * d).sum(1) + eps * eps
+        assert abs(r2[r["nnb"][i]] - r2[o["nnb"][i]]) < 1e-6 * r2[o["nnb"][i]]
+
+
+def test_variants_agree_and_are_deterministic(g6):
+    n = 2048
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 3, 1 / 64)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    g6.set_ti(0.0)
+    sel = np.arange(0, n, 9)
+    outs = {k: g6.calc(n, x[sel], v[sel], idx[sel], 1 / 64 ** 2, variant=k)
+            for k in ("", "2", "2p", "n", "2n", "2np")}
+    for k, o in outs.items():
+        np.testing.assert_array_equal(o["acc"], outs["2p"]["acc"], err_msg=k)
+        np.testing.assert_array_equal(o["jrk"], outs["2p"]["jrk"], err_msg=k)
+        np.testing.assert_array_equal(o["pot"], outs["2p"]["pot"], err_msg=k)
+    np.testing.assert_array_equal(outs["2"]["nnb"], outs["2p"]["nnb"])
+    again = g6.calc(n, x[sel], v[sel], idx[sel], 1 / 64 ** 2, variant="2p")
+    for k in ("acc", "jrk", "pot", "nnb", "rnnb"):
+        np.testing.assert_array_equal(again[k], outs["2p"][k])
+
+
+def test_chunked_calls_match_one_call(g6):
+    """The reference caller (banana hermite.c) sends <= g6_npipes() i per call."""
+    n = 2048
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 4, 1 / 128)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    g6.set_ti(0.0)
+    npipes = g6.lib.g6_npipes()
+    assert npipes == 48
+    one = g6.calc(n, x, v, idx, 1 / 128 ** 2)
+    parts = [g6.calc(n, x[o:o + npipes], v[o:o + npipes], idx[o:o + npipes], 1 / 128 ** 2)
+             for o in range(0, n, npipes)]
+    acc = np.concatenate([p["acc"] for p in parts])
+    assert rel_err(acc, one["acc"]).max() < 1e-6
+
+
+def test_j_updates_last_write_wins_and_partial(g6):
+    n = 1024
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 5, 1 / 64)
+    g6.set_j_particles(x, v, a2, j6, m, tj, idx)
+    # move particle 10 twice before the next calculation: the second write wins
+    x2 = x.copy()
+    x2[10] += 0.25
+    g6.set_j_particles(x[10:11] + 5.0, v[10:11], a2[10:11], j6[10:11], m[10:11], tj[10:11],
+                       idx[10:11], adr0=10)
+    g6.set_j_particles(x2[10:11], v[10:11], a2[10:11], j6[10:11], m[10:11], tj[10:11],
+                       idx[10:11], adr0=10)
+    g6.set_ti(0.0)
+    r = g6.calc(n, x[:64], v[:64], idx[:64], 1 / 64 ** 2)
+    xo, vo = O.pg6_predict(0.0, x2, v, a2, j6, tj)
+    o = O.pg6_force(x[:64], v[:64], idx[:64], 1 / 64 ** 2, xo, vo, m, idx)
+    assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 1e-6
+
+
+def test_neighbour_list_overflow_is_reported(g6):
+    n = 3000
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 6, 1 / 64)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    g6.set_ti(0.0)
+    h2 = np.array([100.0, 1e-6])  # first i: every j is a neighbour (> 1024)
+    r = g6.calc(n, x[:2], v[:2], idx[:2], 1 / 64 ** 2, h2=h2, variant="n")
+    assert r["rc"] == 0
+    assert g6.lib.g6_read_neighbour_list(0) == 1
+    rc, _ = g6.neighbour_list(0, maxlen=1024)
+    assert rc == 1
+    rc, l = g6.neighbour_list(1, maxlen=1024)
+    assert rc == 0 and len(l) == 0
+    # h2 = 0 means no neighbours (gravity.c:689-696)
+    r = g6.calc(n, x[:2], v[:2], idx[:2], 1 / 64 ** 2, h2=np.zeros(2), variant="n")
+    assert g6.lib.g6_read_neighbour_list(0) == 0
+    assert g6.neighbour_list(0)[1].size == 0
+
+
+def test_empty_and_error_paths(g6):
+    r = g6.calc(0, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32), 0.01)
+    assert r["rc"] == 0
+    m, x, v, a2, j6, tj, idx = plummer_state(256, 7, 1 / 64)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    g6.set_ti(0.0)
+    r = g6.calc(0, x[:5], v[:5], idx[:5], 0.01)  # nj = 0: no force
+    assert np.all(r["acc"] == 0) and np.all(r["pot"] == 0)
+    import pgamd
+    g6.lib.pgamd_set_abort_on_error(0)
+    try:
+        rc, l = g6.neighbour_list(10 ** 6)
+        code, msg = pgamd.last_error(g6.lib)
+        assert rc == 1 and code != 0 and "ipipe" in msg
+    finally:
+        g6.lib.pgamd_set_abort_on_error(1)

@@ -1,0 +1,294 @@
+"""Pins the CPU oracle (oracle/pg_oracle.c) before it is trusted as the
+checker of the GPU path: against the reference's own outputs (golden fixtures
+made by running the compiled reference on its own input files), against the
+compiled reference live where it is built, and against what the mathematics
+fixes (closed forms, invariants, finite differences, brute force).
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import ics
+import oracle as O
+from conftest import golden, rel_err
+
+
+# ------------------------------------------------------------------- GRAPE-5
+@pytest.mark.parametrize("name", ["pl1k", "pl1k_1", "pl1k_3"])
+def test_pg5_oracle_matches_reference_fixture(name):
+    g = golden("pg5_direct.npz")
+    m, x, eps = g[f"{name}_m"], g[f"{name}_x"], float(g[f"{name}_eps"])
+    a, p = O.pg5_force(x, eps * eps, x, m)
+    # the reference uses the 12-bit rsqrtps estimate plus a bias correction
+    # (PG5/newton/libpg5/rsqrt.c:42-81, phantom_g5.c:45-60): its own error
+    # against FP64 is about 1e-4 per particle
+    e = rel_err(g[f"{name}_acc"], a)
+    assert np.median(e) < 1e-4 and e.max() < 3e-3
+    ep = rel_err(g[f"{name}_pot"], p)
+    assert np.median(ep) < 5e-5 and ep.max() < 1e-3
+
+
+def test_pg5_symmetric_oracle_matches_reference_fixture():
+    g = golden("pg5_symmetric.npz")
+    a, p = O.pg5_force(g["x"], g["eps2"], g["x"], g["m"], eps2j=g["eps2"])
+    assert np.median(rel_err(g["acc"], a)) < 1e-4
+    assert np.median(rel_err(g["pot"], p)) < 5e-5
+    # a wrong softening (eps_i^2 only) is detected
+    a2, _ = O.pg5_force(g["x"], g["eps2"], g["x"], g["m"])
+    assert np.median(rel_err(g["acc"], a2)) > 1e-3
+
+
+def test_pg5_two_body_closed_form():
+    xj = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, 1.2]])
+    mj = np.array([2.0, 0.5])
+    eps = 0.1
+    a, p = O.pg5_force(xj[:1], eps * eps, xj, mj)
+    d = xj[1] - xj[0]
+    r2 = d @ d + eps * eps
+    np.testing.assert_allclose(a[0], 0.5 * d / r2 ** 1.5, rtol=1e-15)
+    np.testing.assert_allclose(p[0], 2.0 / eps + 0.5 / np.sqrt(r2), rtol=1e-15)
+
+
+def test_pg5_momentum_and_gradient():
+    m, x, _ = ics.plummer(512, 3)
+    eps = 1.0 / 64
+    a, p = O.pg5_force(x, eps * eps, x, m)
+    # Newton's third law: sum m a = 0 when the i-set is the j-set
+    assert np.abs((m[:, None] * a).sum(0)).max() < 1e-13 * (m[:, None] * np.abs(a)).sum()
+    # a = +grad_x(p) (p = + sum m/r)
+    h = 1e-6
+    for i in (0, 17, 300):
+        for k in range(3):
+            xp = x[i:i + 1].copy()
+            xm = x[i:i + 1].copy()
+            xp[0, k] += h
+            xm[0, k] -= h
+            _, pp = O.pg5_force(xp, eps * eps, x, m)
+            _, pm = O.pg5_force(xm, eps * eps, x, m)
+            np.testing.assert_allclose((pp - pm) / (2 * h), a[i, k], rtol=1e-6,
+                                       atol=1e-8 * np.abs(a[i]).max())
+
+
+def test_pg5_against_compiled_reference():
+    if not O.ref_available("pg5"):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    m, x, _ = ics.plummer(2048, 8)
+    ra, rp = O.RefPG5().force(x, x, m, 1.0 / 64)
+    a, p = O.pg5_force(x, (1.0 / 64) ** 2, x, m)
+    assert np.percentile(rel_err(ra, a), 99) < 1e-3
+    assert np.median(rel_err(rp, p)) < 5e-5
+
+
+# ------------------------------------------------------------------- GRAPE-6
+def test_pg6_predictor_is_the_exact_polynomial():
+    """With binary-fraction inputs every operation is exact, so the result must
+    equal the rational polynomial x + v dt + a2 dt^2 + j6 dt^3."""
+    rng = np.random.default_rng(1)
+    n = 64
+    q = lambda: rng.integers(-64, 64, (n, 3)) / 64.0  # noqa: E731
+    x, v, a2, j6 = q(), q(), q(), q()
+    tj = -rng.integers(0, 8, n) / 16.0
+    ti = 0.125
+    xp, vp = O.pg6_predict(ti, x, v, a2, j6, tj)
+    for j in range(0, n, 7):
+        dt = Fraction(ti) - Fraction(tj[j])
+        for k in range(3):
+            X = (Fraction(x[j, k]) + Fraction(v[j, k]) * dt + Fraction(a2[j, k]) * dt ** 2
+                 + Fraction(j6[j, k]) * dt ** 3)
+            V = Fraction(v[j, k]) + 2 * Fraction(a2[j, k]) * dt + 3 * Fraction(j6[j, k]) * dt ** 2
+            assert xp[j, k] == float(X) and vp[j, k] == float(V)
+
+
+def test_pg6_oracle_matches_reference_fixture():
+    """Reference run on its own banana input (PG6/banana/init/pl001k.init),
+    j-particles predicted from t_j to t_i: pins predictor and force together."""
+    g = golden("pg6_hermite.npz")
+    xp, vp = O.pg6_predict(float(g["ti"]), g["x"], g["v"], g["acc2"], g["jrk6"], g["tj"])
+    idx = np.arange(len(g["m"]), dtype=np.int32)
+    isel = g["isel"]
+    np.testing.assert_array_equal(xp[isel], g["xi"])
+    o = O.pg6_force(g["xi"], g["vi"], idx[isel], float(g["eps2"]), xp, vp, g["m"], idx)
+    ea = rel_err(g["acc"], o["acc"])
+    ej = rel_err(g["jrk"], o["jrk"])
+    ep = rel_err(g["pot"], o["pot"])
+    # reference: FP32 pair math with one Newton step on rsqrtps, FP32 jerk sums
+    assert np.percentile(ea, 99) < 1e-6 and ea.max() < 5e-6
+    assert np.percentile(ej, 99) < 1e-5
+    assert ep.max() < 1e-6  # pot is returned through a float field
+    assert np.mean(g["nnb"] == o["nnb"]) == 1.0
+    assert rel_err(g["rnnb"], o["rnnb"]).max() < 1e-6
+    # a predictor without the jerk term is detected
+    xq, vq = O.pg6_predict(float(g["ti"]), g["x"], g["v"], g["acc2"], 0 * g["jrk6"], g["tj"])
+    o2 = O.pg6_force(g["xi"], g["vi"], idx[isel], float(g["eps2"]), xq, vq, g["m"], idx)
+    assert np.percentile(rel_err(g["jrk"], o2["jrk"]), 99) > 1e-5
+
+
+def test_pg6_neighbour_lists_match_reference_fixture():
+    g = golden("pg6_hermite.npz")
+    xp, _ = O.pg6_predict(float(g["ti"]), g["x"], g["v"], g["acc2"], g["jrk6"], g["tj"])
+    idx = np.arange(len(g["m"]), dtype=np.int32)
+    lists, nblen = O.pg6_nblist(g["xi"], idx[g["isel"]], g["h2"], float(g["eps2"]), xp, idx)
+    np.testing.assert_array_equal(nblen, g["nblen"])
+    np.testing.assert_array_equal(np.concatenate(lists), g["nbl"])
+
+
+def test_pg6_two_body_closed_form():
+    xi = np.array([[0.1, 0.2, 0.3]])
+    vi = np.array([[0.0, 0.5, 0.0]])
+    xj = np.array([[0.1, 0.2, 0.3], [1.1, 0.2, -0.7]])
+    vj = np.array([[9.0, 9.0, 9.0], [0.3, 0.0, 0.1]])
+    mj = np.array([5.0, 0.25])
+    eps2 = 0.01
+    o = O.pg6_force(xi, vi, [0], eps2, xj, vj, mj, [0, 1])
+    d = xj[1] - xi[0]
+    dv = vj[1] - vi[0]
+    r2 = d @ d + eps2
+    a = 0.25 * d / r2 ** 1.5
+    jk = 0.25 * (dv / r2 ** 1.5 - 3 * (d @ dv) * d / r2 ** 2.5)
+    np.testing.assert_allclose(o["acc"][0], a, rtol=1e-14)
+    np.testing.assert_allclose(o["jrk"][0], jk, rtol=1e-13)
+    np.testing.assert_allclose(o["pot"][0], -0.25 / np.sqrt(r2), rtol=1e-15)
+    assert o["nnb"][0] == 1 and abs(o["rnnb"][0] - np.sqrt(r2)) < 1e-15
+
+
+def test_pg6_jerk_is_the_time_derivative_of_acc():
+    m, x, v = ics.plummer(256, 4)
+    idx = np.arange(256, dtype=np.int32)
+    eps2 = (1.0 / 64) ** 2
+    o = O.pg6_force(x, v, idx, eps2, x, v, m, idx)
+    h = 1e-6
+    op = O.pg6_force(x + h * v, v, idx, eps2, x + h * v, v, m, idx)
+    om = O.pg6_force(x - h * v, v, idx, eps2, x - h * v, v, m, idx)
+    fd = (op["acc"] - om["acc"]) / (2 * h)
+    assert np.percentile(rel_err(fd, o["jrk"]), 99) < 1e-5
+
+
+def test_pg6_against_compiled_reference():
+    if not O.ref_available("pg6"):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    m, x, v = ics.plummer(1024, 9)
+    n = len(m)
+    idx = np.arange(n, dtype=np.int32)
+    eps2 = (1.0 / 256) ** 2
+    R = O.RefPG6()
+    z = np.zeros((n, 3))
+    R.set_j(x, v, z, z, m, np.zeros(n), idx)
+    r = R.calc(n, 0.0, x, v, idx, eps2)
+    o = O.pg6_force(x, v, idx, eps2, x, v, m, idx)
+    assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 1e-6
+    assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 2e-5
+    assert np.mean(r["nnb"] == o["nnb"]) == 1.0
+
+
+# --------------------------------------------------------------------- table
+def test_s2_shape_is_continuous_and_cut():
+    pp, pm = 0.1 / 128, 3.0 / 128
+    for sft in (pp, pm):
+        for eta in (1.0, 2.0):
+            r = eta * sft / 2
+            lo = O.lib().or_s2_reff(sft, r * (1 - 1e-12))
+            hi = O.lib().or_s2_reff(sft, r * (1 + 1e-12))
+            assert abs(lo - hi) < 1e-9 * abs(hi)
+    r = np.array([pm * 1.0001, pm * 2, 0.5])
+    np.testing.assert_allclose(O.s2_force(pp, pm, r), 0.0, atol=1e-9)
+    # unsoftened Newtonian limit between the two softening lengths
+    r = np.array([pp * 3, pp * 5])
+    f = O.s2_force(pp, pm, r)
+    assert np.all(f > 0) and np.all(f < 1 / r ** 3)
+
+
+@pytest.mark.parametrize("key,eps", [("tab64", (0.1 / 64, 3.0 / 64)),
+                                     ("tab128", (0.1 / 128, 3.0 / 128))])
+def test_table_build_is_bit_exact_with_reference(key, eps):
+    g = golden("pg5_table.npz")
+    tab, _, _ = O.table_build(*eps)
+    np.testing.assert_array_equal(tab.view(np.uint32), g[key].astype(np.float32).view(np.uint32))
+
+
+def test_table_lookup_hits_entries_and_interpolates():
+    tab, r2s, xs = O.table_build(0.1 / 128, 3.0 / 128)
+    # at the float-bit ticks s_k the lookup returns the entry exactly
+    ticks = (np.arange(0, 512, 3, dtype=np.uint32) << 18) + np.uint32(0x40000000)
+    s = ticks.view(np.float32)
+    np.testing.assert_array_equal(O.table_lookup(tab, s), tab[0::2][np.arange(0, 512, 3)])
+    # beyond the cut-off the last entry (about 0) is used
+    assert O.table_lookup(tab, [1e9])[0] == tab[2 * 511]
+
+
+def test_table_force_emulation_matches_reference_fixture():
+    g = golden("pg5_table.npz")
+    tab, r2s, xs = O.table_build(0.1 / 128, 3.0 / 128)
+    a = O.table_force_emul(g["xi"], g["xj"], g["mj"], tab, xs)
+    ref = g["acc"]
+    scale = np.linalg.norm(ref, axis=1).max()
+    # identical per-pair terms; the reference sums them in FP32
+    e = rel_err(ref, a, floor=1e-3 * scale)
+    assert np.percentile(e, 99) < 1e-5, np.percentile(e, 99)
+    # and the closed-form S2 force is close, within the table's own error
+    f = O.table_force_f64(g["xi"], g["xj"], g["mj"], 0.1 / 128, 3.0 / 128)
+    ef = rel_err(f, a, floor=1e-3 * scale)
+    assert np.median(ef) < 5e-3
+
+
+def test_table_against_compiled_reference():
+    if not O.ref_available("pg5table"):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    T = O.RefTable()
+    T.gen_s2(0.1 / 128, 3.0 / 128)
+    tab, _, xs = O.table_build(0.1 / 128, 3.0 / 128)
+    np.testing.assert_array_equal(T.table().view(np.uint32), tab.view(np.uint32))
+    m, x = ics.clumpy_box(4096, 2)
+    sel = x[:, 0] < 0.05
+    xj, mj = x[sel], m[sel]
+    if len(mj) % 2:
+        xj, mj = xj[:-1], mj[:-1]
+    xi = xj[::5]
+    ra = T.force(xi, xj, mj)
+    a = O.table_force_emul(xi, xj, mj, tab, xs)
+    scale = np.linalg.norm(ra, axis=1).max()
+    assert np.percentile(rel_err(ra, a, floor=1e-3 * scale), 99) < 1e-5
+
+
+# ---------------------------------------------------------------- quadrupole
+def test_quad_moments_traceless_and_definition():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(20, 3))
+    m = rng.uniform(0.5, 1.5, 20)
+    c, M, q = O.quad_moments(x, m)
+    assert abs(q[0] + q[3] + q[5]) < 1e-12
+    d = x - c
+    Q = 3 * np.einsum("n,ni,nj->ij", m, d, d) - np.eye(3) * (m * (d * d).sum(1)).sum()
+    np.testing.assert_allclose(q, [Q[0, 0], Q[0, 1], Q[0, 2], Q[1, 1], Q[1, 2], Q[2, 2]],
+                               atol=1e-12)
+
+
+def test_quad_far_field_converges_to_direct_sum():
+    """Monopole + quadrupole of a particle cloud reproduces the direct sum at
+    distance D with residual O((size/D)^4) (the dipole vanishes about the
+    centre of mass), while monopole-only leaves O((size/D)^2)."""
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(50, 3)) * [1.0, 0.6, 0.3]
+    m = rng.uniform(0.5, 1.5, 50)
+    c, M, q = O.quad_moments(x, m)
+    prev = None
+    for D in (20.0, 40.0):
+        xi = c + D * np.array([[0.6, 0.64, 0.48], [-0.8, 0.0, 0.6]])
+        ad, pd = O.pg5_force(xi, 0.0, x, m)
+        aq, pq = O.quad_force(xi, 0.0, c[None], [M], q[None])
+        am, pm = O.pg5_force(xi, 0.0, c[None], np.array([M]))
+        eq = rel_err(aq, ad).max()
+        em = rel_err(am, ad).max()
+        assert eq < 0.05 * em
+        np.testing.assert_allclose(pq, pd, rtol=5 * (2.0 / D) ** 3)
+        if prev is not None:
+            assert eq < prev / 8  # at least third order
+        prev = eq
+
+
+def test_quad_oracle_matches_reference_fixture():
+    g = golden("pg5_quad.npz")
+    a, p = O.quad_force(g["xi"], float(g["eps"]) ** 2, g["xc"], g["mc"], g["qc"])
+    # reference: raw 12-bit rsqrtps, FP32 sums; the g5c potential is negative
+    assert np.median(rel_err(g["acc"], a)) < 3e-4
+    np.testing.assert_allclose(g["pot"], -p, rtol=3e-3)
