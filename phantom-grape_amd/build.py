@@ -27,6 +27,12 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+# host-only C++ libraries (no GPU code): name -> sources
+HOST_TARGETS = {
+    "pgtree": ["treebuild.cpp"],
+}
+
 # library name -> (sources, extra flags)
 TARGETS = {
     "pg5": (["pg5.hip", "pgamd_common.cpp"], []),
@@ -37,6 +43,10 @@ TARGETS = {
 
 
 def _cmd(name: str) -> list[str]:
+    if name in HOST_TARGETS:
+        out = os.path.join(LIBDIR, f"lib{name}.so")
+        return [CXX, "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-fvisibility=hidden",
+                "-o", out + ".tmp", *[os.path.join(CSRC, s) for s in HOST_TARGETS[name]]]
     srcs, extra = TARGETS[name]
     out = os.path.join(LIBDIR, f"lib{name}.so")
     return [HIPCC, *FLAGS, *extra, "-I", CSRC, "-o", out + ".tmp",
@@ -45,10 +55,11 @@ def _cmd(name: str) -> list[str]:
 
 def build(names=None, verbose=False) -> None:
     os.makedirs(LIBDIR, exist_ok=True)
-    names = list(names or TARGETS)
+    names = list(names or (list(TARGETS) + list(HOST_TARGETS)))
     procs = []
     for n in names:
-        if not all(os.path.exists(os.path.join(CSRC, s)) for s in TARGETS[n][0]):
+        srcs = HOST_TARGETS[n] if n in HOST_TARGETS else TARGETS[n][0]
+        if not all(os.path.exists(os.path.join(CSRC, s)) for s in srcs):
             raise FileNotFoundError(f"sources of lib{n}.so missing")
         cmd = _cmd(n)
         if verbose:

@@ -24,6 +24,7 @@
 #include "pgamd_common.h"
 #include "pgamd_api.h"
 #include "pg5_kernels.h"
+#include "bh_batch.h"
 
 #include <algorithm>
 #include <cmath>
@@ -51,8 +52,10 @@ struct JMem {
   HostBuf h, he;  // he: eps_j^2 (symmetric build)
   DevBuf d, de;
   int hcap = 0;   // records held by h
+  int hmax = 0;   // one past the highest address ever written
   int lo = 0, hi = 0;  // dirty range [lo, hi)
   void write_range(int a0, int a1) {
+    hmax = std::max(hmax, a1);
     if (lo >= hi) {
       lo = a0;
       hi = a1;
@@ -76,7 +79,7 @@ struct Board {
   HostBuf opipe;       // double acc[3], pot per pipeline
   // staging of the calculate_force calls
   HostBuf h_in, h_out;
-  DevBuf d_in, d_out, d_part;
+  DevBuf d_in, d_out, d_part, d_counter;
 };
 
 Board g_boards[kMaxDev];
@@ -131,8 +134,14 @@ void jmem_reserve_host(JMem& m, int need, size_t rec_bytes, bool eps) {
 
 // Upload the dirty range of a j-memory before a kernel reads [0, nread).
 void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
-  m.d.reserve(rec_bytes * (size_t)std::max(nread, m.hcap));
-  if (eps) m.de.reserve(sizeof(float) * (size_t)std::max(nread, m.hcap));
+  const size_t need = rec_bytes * (size_t)std::max(nread, m.hcap);
+  if (need > m.d.bytes) {
+    // A grown device buffer starts empty: upload everything written so far.
+    m.d.reserve(need);
+    if (eps) m.de.reserve(sizeof(float) * (size_t)std::max(nread, m.hcap));
+    m.lo = 0;
+    m.hi = m.hmax;
+  }
   if (m.lo < m.hi) {
     const size_t off = rec_bytes * (size_t)m.lo;
     PGAMD_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(m.d.ptr) + off,
@@ -622,5 +631,161 @@ PGAMD_API int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
   g_force_bi = bi;
   if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
   if (min_jper > 0) g_min_jper = min_jper;
+  return 0;
+}
+
+// --------------------------------------------------- batched tree-list force
+
+namespace {
+int g_bh_bpc = 0;  // persistent blocks per CU (0 = default 4)
+
+struct BhArgs {
+  int ngroups;
+  const int *ioff, *icnt, *ilist, *poff, *pcnt, *plist, *coff, *ccnt, *clist, *order;
+  const float4* pos;
+  const Cell* cells;
+  float eps2;
+  double *acc, *pot;
+};
+
+void launch_bh(const BhArgs& a, int* counter, int bpc, hipStream_t st) {
+  hipLaunchKernelGGL(pgbh::zero_counter, dim3(1), dim3(64), 0, st, counter);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  if (a.ngroups <= 0) return;
+  const int blocks = std::max(1, std::min(a.ngroups, device_cu_count() * bpc));
+  hipLaunchKernelGGL(pgbh::bh_batch, dim3(blocks), dim3(pgbh::kThreads), 0, st, a.ngroups,
+                     a.order, a.ioff, a.icnt, a.ilist, a.poff, a.pcnt, a.plist, a.coff, a.ccnt,
+                     a.clist, a.pos, a.cells, a.eps2, a.acc, a.pot, counter);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+int bh_bpc() { return g_bh_bpc > 0 ? g_bh_bpc : 4; }
+
+struct BhGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf counter;
+};
+
+BhArgs bh_args(int ngroups, const int* ioff, const int* icnt, const int* ilist, const int* poff,
+               const int* pcnt, const int* plist, const int* coff, const int* ccnt,
+               const int* clist, const int* order, const void* pos, const void* cells,
+               double eps2, double* acc, double* pot) {
+  return BhArgs{ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist, order,
+                static_cast<const float4*>(pos), static_cast<const Cell*>(cells), (float)eps2,
+                acc, pot};
+}
+}  // namespace
+
+// Tree-list force of every i-group in one persistent launch (bh_batch.h).
+// Device arrays: pos[np] float4 {x,y,z,m}; cells[nc] 12 floats {x,y,z,M,
+// Qxx,Qxy,Qxz,Qyy,Qyz,Qzz,0,0}; per group g the i-list ilist[ioff[g]..
+// +icnt[g]), particle list plist[poff[g]..+pcnt[g]) and cell list
+// clist[coff[g]..+ccnt[g]) (indices into pos / cells); order nullable.
+// Writes acc[i][3] and pot[i] = + sum m/r + sum (phi_p + phi_q) for every
+// listed i.  Asynchronous on stream; calls on one devid share its counter.
+PGAMD_API int pg5_bh_batch(int devid, int ngroups, const int* ioff, const int* icnt,
+                           const int* ilist, const int* poff, const int* pcnt, const int* plist,
+                           const int* coff, const int* ccnt, const int* clist, const int* order,
+                           const void* pos, const void* cells, double eps2, double* acc,
+                           double* pot, void* stream) {
+  Board* b = board(devid, "pg5_bh_batch");
+  if (!b) return -1;
+  if (ngroups < 0) {
+    api_error(3, "pg5_bh_batch: ngroups=%d", ngroups);
+    return -1;
+  }
+  b->d_counter.reserve(256);
+  launch_bh(bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist, order, pos,
+                    cells, eps2, acc, pot),
+            b->d_counter.as<int>(), bh_bpc(), stream ? (hipStream_t)stream : b->stream);
+  return 0;
+}
+
+// The same launch captured into a HIP graph (own counter); replay with
+// pg5_bh_graph_launch, free with pg5_bh_graph_destroy.
+PGAMD_API void* pg5_bh_graph(int devid, int ngroups, const int* ioff, const int* icnt,
+                             const int* ilist, const int* poff, const int* pcnt,
+                             const int* plist, const int* coff, const int* ccnt,
+                             const int* clist, const int* order, const void* pos,
+                             const void* cells, double eps2, double* acc, double* pot) {
+  Board* b = board(devid, "pg5_bh_graph");
+  if (!b) return nullptr;
+  BhGraph* g = new BhGraph();
+  g->counter.reserve(256);
+  hipStream_t cs;
+  PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  launch_bh(bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist, order, pos,
+                    cells, eps2, acc, pot),
+            g->counter.as<int>(), bh_bpc(), cs);
+  PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
+  PGAMD_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  PGAMD_HIP_CHECK(hipStreamDestroy(cs));
+  return g;
+}
+
+PGAMD_API int pg5_bh_graph_launch(void* h, void* stream) {
+  if (!h) {
+    api_error(3, "pg5_bh_graph_launch: null graph");
+    return -1;
+  }
+  Board* b = board(0, "pg5_bh_graph_launch");
+  PGAMD_HIP_CHECK(hipGraphLaunch(static_cast<BhGraph*>(h)->exec,
+                                 stream ? (hipStream_t)stream : b->stream));
+  return 0;
+}
+
+PGAMD_API int pg5_bh_graph_destroy(void* h) {
+  if (!h) return 0;
+  BhGraph* g = static_cast<BhGraph*>(h);
+  PGAMD_HIP_CHECK(hipDeviceSynchronize());
+  PGAMD_HIP_CHECK(hipGraphExecDestroy(g->exec));
+  PGAMD_HIP_CHECK(hipGraphDestroy(g->graph));
+  g->counter.release();
+  delete g;
+  return 0;
+}
+
+// Autotuner: times the launch for {1, 2, 3, 4, 6, 8, 12, 16} persistent
+// blocks per CU (one warm-up, mean of reps), keeps the fastest; ms[8]
+// (nullable) receives the timings.
+PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int* icnt,
+                              const int* ilist, const int* poff, const int* pcnt,
+                              const int* plist, const int* coff, const int* ccnt,
+                              const int* clist, const int* order, const void* pos,
+                              const void* cells, double eps2, double* acc, double* pot, int reps,
+                              int* best, float* ms) {
+  Board* b = board(devid, "pg5_bh_autotune");
+  if (!b) return -1;
+  const BhArgs a = bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist,
+                           order, pos, cells, eps2, acc, pot);
+  b->d_counter.reserve(256);
+  const int cand[8] = {1, 2, 3, 4, 6, 8, 12, 16};
+  hipEvent_t e0, e1;
+  PGAMD_HIP_CHECK(hipEventCreate(&e0));
+  PGAMD_HIP_CHECK(hipEventCreate(&e1));
+  float bestt = 1e30f;
+  int bestc = bh_bpc();
+  reps = std::max(1, reps);
+  for (int k = 0; k < 8; ++k) {
+    launch_bh(a, b->d_counter.as<int>(), cand[k], b->stream);
+    PGAMD_HIP_CHECK(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; ++r) launch_bh(a, b->d_counter.as<int>(), cand[k], b->stream);
+    PGAMD_HIP_CHECK(hipEventRecord(e1, b->stream));
+    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    t /= reps;
+    if (ms) ms[k] = t;
+    if (t < bestt) {
+      bestt = t;
+      bestc = cand[k];
+    }
+  }
+  PGAMD_HIP_CHECK(hipEventDestroy(e0));
+  PGAMD_HIP_CHECK(hipEventDestroy(e1));
+  g_bh_bpc = bestc;
+  if (best) *best = bestc;
   return 0;
 }

@@ -13,9 +13,12 @@
 #include "pgamd_common.h"
 #include "pgamd_api.h"
 #include "table_kernels.h"
+#include "treepm_batch.h"
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
+#include <random>
 #include <vector>
 
 using namespace pgamd;
@@ -56,16 +59,29 @@ struct Board {
   hipStream_t stream = nullptr;
   HostBuf hj;
   int hcap = 0;
+  int hmax = 0;  // one past the highest address ever written
   int lo = 0, hi = 0;
   DevBuf dj;
   int n = 0;
   HostBuf ipipe, opipe;
   int ni_pipe = 0;
   HostBuf h_in, h_out;
-  DevBuf d_in, d_out, d_part;
+  DevBuf d_in, d_out, d_part, d_counter;
+};
+
+// Learned cut-off (mlp_kernels.h): parameters, the radii it was trained for.
+struct Mlp {
+  std::vector<float> params;  // mlp::kParams
+  bool valid = false;
+  double rc = 0.0, r0 = 0.0;
+  DevBuf dev;
+  bool dirty = true;
 };
 
 Table g_tab;
+Mlp g_mlp;
+int g_cut_mode = 0;  // 0: table, 1: learned MLP
+int g_batch_bpc[2] = {0, 0};  // persistent blocks per CU per mode (0 = default)
 Board g_boards[kMaxDev];
 std::mutex g_mutex;  // guards the table and board creation
 int g_force_bi = 0;
@@ -210,12 +226,19 @@ void launch_t(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
+void run_mlp_dropin(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj,
+                    double* acc, hipStream_t st);
+
 // Scaled i (float4) and j (float4) device arrays -> acc[ni][3] (device).
 void run_table(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj, double* acc,
                hipStream_t st) {
   if (ni <= 0) return;
   if (nj <= 0) {
     PGAMD_HIP_CHECK(hipMemsetAsync(acc, 0, sizeof(double) * 3 * (size_t)ni, st));
+    return;
+  }
+  if (g_cut_mode == 1) {
+    run_mlp_dropin(scratch, ni, xi, nj, xj, acc, st);
     return;
   }
   const float2* tab = table_dev(st);
@@ -264,7 +287,13 @@ void board_calc(Board& b, const double (*x)[3], const float4* pre, int ni, doubl
   PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
                                  b.stream));
   const int nj = b.n;
-  b.dj.reserve(sizeof(float4) * (size_t)std::max(nj, b.hcap));
+  const size_t need = sizeof(float4) * (size_t)std::max(nj, b.hcap);
+  if (need > b.dj.bytes) {
+    // A grown device buffer starts empty: upload everything written so far.
+    b.dj.reserve(need);
+    b.lo = 0;
+    b.hi = b.hmax;
+  }
   if (b.lo < b.hi) {
     PGAMD_HIP_CHECK(hipMemcpyAsync(b.dj.as<float4>() + b.lo, b.hj.as<float4>() + b.lo,
                                    sizeof(float4) * (size_t)(b.hi - b.lo),
@@ -292,6 +321,7 @@ void jmem_reserve(Board& b, int need) {
 }
 
 void mark(Board& b, int a0, int a1) {
+  b.hmax = std::max(b.hmax, a1);
   if (b.lo >= b.hi) {
     b.lo = a0;
     b.hi = a1;
@@ -308,6 +338,102 @@ bool check_range(const char* fn, int adr, int nj) {
     return false;
   }
   return true;
+}
+
+
+const float* mlp_dev(hipStream_t st) {
+  Mlp& m = g_mlp;
+  if (m.dirty) {
+    m.dev.reserve(sizeof(float) * mlp::kParams);
+    PGAMD_HIP_CHECK(hipMemcpyAsync(m.dev.ptr, m.params.data(), sizeof(float) * mlp::kParams,
+                                   hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    m.dirty = false;
+  }
+  return m.dev.as<float>();
+}
+
+// Core length of the learned target Y = f (r^3 + r0^3): the PP softening for
+// the S2 shape, r_c/30 for a user shape.
+double mlp_r0() { return g_tab.is_s2 ? g_tab.sft_pp : g_tab.rcut / 30.0; }
+
+pgb::MlpCut mlp_cut(hipStream_t st) {
+  pgb::MlpCut mc;
+  const double xs = g_tab.xscale;
+  mc.w = mlp_dev(st);
+  mc.inv_rc_s = (float)(1.0 / (g_mlp.rc * xs));
+  mc.rc2_s = (float)((g_mlp.rc * xs) * (g_mlp.rc * xs));
+  mc.r03_s = (float)std::pow(g_mlp.r0 * xs, 3);
+  mc.xs3 = (float)(xs * xs * xs);
+  return mc;
+}
+
+// Launches the persistent batch kernel (counter reset + kernel) on st.
+void launch_batch(int mode, int ngroups, const int* order, const int* ioff, const int* icnt,
+                  const int* ilist, const int* joff, const int* jcnt, const int* jlist,
+                  const float4* pos, double* acc, int* counter, int bpc, hipStream_t st) {
+  hipLaunchKernelGGL(pgb::zero_counter, dim3(1), dim3(64), 0, st, counter);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  if (ngroups <= 0) return;
+  const int blocks = std::max(1, std::min(ngroups, device_cu_count() * bpc));
+  const double ascale = (double)(float)(1.0 / g_tab.xscale);
+  pgb::TableCut tc{table_dev(st), g_tab.ntab, g_tab.frc_bit};
+  if (mode == 1) {
+    pgb::MlpCut mc = mlp_cut(st);
+    hipLaunchKernelGGL((pgb::treepm_batch<true>), dim3(blocks), dim3(pgb::kThreads), 0, st,
+                       ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, pos, tc, mc, ascale,
+                       acc, counter);
+  } else {
+    pgb::MlpCut mc{};
+    const size_t lds = sizeof(float2) * (size_t)g_tab.ntab;
+    hipLaunchKernelGGL((pgb::treepm_batch<false>), dim3(blocks), dim3(pgb::kThreads), lds, st,
+                       ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, pos, tc, mc, ascale,
+                       acc, counter);
+  }
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+int default_bpc(int mode) {
+  if (g_batch_bpc[mode] > 0) return g_batch_bpc[mode];
+  return mode == 1 ? 4 : 8;
+}
+
+// Drop-in force through the batch kernel (used in MLP mode): device i and j
+// float4 arrays are placed in one array [j..., i...] with index lists.
+void run_mlp_dropin(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj,
+                    double* acc, hipStream_t st) {
+  const int ng = ceil_div(ni, pgb::kThreads);
+  const size_t pos_b = sizeof(float4) * (size_t)(ni + nj);
+  const size_t acc_b = sizeof(double) * 3 * (size_t)(ni + nj);
+  const size_t idx_n = (size_t)ni + nj + 4 * (size_t)ng + 1;
+  scratch.reserve(pos_b + acc_b + sizeof(int) * idx_n + 256);
+  char* base = static_cast<char*>(scratch.ptr);
+  float4* pos = reinterpret_cast<float4*>(base);
+  double* accw = reinterpret_cast<double*>(base + pos_b);
+  int* ilist = reinterpret_cast<int*>(base + pos_b + acc_b);
+  int* jlist = ilist + ni;
+  int* ioff = jlist + nj;
+  int* icnt = ioff + ng;
+  int* joff = icnt + ng;
+  int* jcnt = joff + ng;
+  int* counter = jcnt + ng;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(pos, xj, sizeof(float4) * (size_t)nj, hipMemcpyDeviceToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(pos + nj, xi, sizeof(float4) * (size_t)ni, hipMemcpyDeviceToDevice, st));
+  std::vector<int> h((size_t)ni + nj + 4 * (size_t)ng);
+  for (int i = 0; i < ni; ++i) h[i] = nj + i;
+  for (int j = 0; j < nj; ++j) h[(size_t)ni + j] = j;
+  for (int g = 0; g < ng; ++g) {
+    h[(size_t)ni + nj + g] = g * pgb::kThreads;
+    h[(size_t)ni + nj + ng + g] = std::min(pgb::kThreads, ni - g * pgb::kThreads);
+    h[(size_t)ni + nj + 2 * ng + g] = 0;
+    h[(size_t)ni + nj + 3 * ng + g] = nj;
+  }
+  PGAMD_HIP_CHECK(hipMemcpyAsync(ilist, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
+  launch_batch(1, ng, nullptr, ioff, icnt, ilist, joff, jcnt, jlist, pos, accw, counter,
+               default_bpc(1), st);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(acc, accw + 3 * (size_t)nj, sizeof(double) * 3 * (size_t)ni,
+                                 hipMemcpyDeviceToDevice, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -536,4 +662,385 @@ PGAMD_API int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
 PGAMD_API void* pg5t_dev_stream(int devid) {
   Board* b = board(devid, "pg5t_dev_stream");
   return b ? (void*)b->stream : nullptr;
+}
+
+// ------------------------------------------------------- learned cut-off
+
+// 0 = table (default, the reference's method), 1 = learned MLP (requires a
+// trained or loaded network).  Applies to g5_calculate_force_on_x* and
+// pg5t_dev_force; the batch calls take the mode as an argument.
+PGAMD_API int pg5_set_cutoff_mode(int mode) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (mode != 0 && mode != 1) {
+    api_error(3, "pg5_set_cutoff_mode: mode %d not in {0, 1}", mode);
+    return -1;
+  }
+  if (mode == 1 && !g_mlp.valid) {
+    api_error(5, "pg5_set_cutoff_mode: no learned cut-off (train or load one first)");
+    return -1;
+  }
+  g_cut_mode = mode;
+  return 0;
+}
+
+PGAMD_API int pg5_get_cutoff_mode(void) { return g_cut_mode; }
+
+// Loads a network: p[321] (layout in mlp_kernels.h), trained for cut-off
+// radius rc and core length r0 (physical units).
+PGAMD_API int pg5_mlp_set_params(const float* p, double rc, double r0) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!(rc > 0.0) || !(r0 >= 0.0)) {
+    api_error(3, "pg5_mlp_set_params: rc=%g r0=%g", rc, r0);
+    return -1;
+  }
+  g_mlp.params.assign(p, p + mlp::kParams);
+  g_mlp.rc = rc;
+  g_mlp.r0 = r0;
+  g_mlp.valid = true;
+  g_mlp.dirty = true;
+  return 0;
+}
+
+// Copies the current network (p may be null); returns 1 if one is loaded.
+PGAMD_API int pg5_mlp_get_params(float* p, double* rc, double* r0) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!g_mlp.valid) return 0;
+  if (p) std::memcpy(p, g_mlp.params.data(), sizeof(float) * mlp::kParams);
+  if (rc) *rc = g_mlp.rc;
+  if (r0) *r0 = g_mlp.r0;
+  return 1;
+}
+
+namespace {
+struct TrainBufs {
+  DevBuf v, t, wt, w, m, u, g, loss, gsum;
+};
+
+// One gradient evaluation (+ optional Adam step) on n samples at device
+// offset `off` of the training set; returns the summed loss through gsum.
+void mlp_step(TrainBufs& tb, int n, int off, int adam, int step, float lr, hipStream_t st) {
+  hipLaunchKernelGGL(mlp::mlp_sample_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, n,
+                     tb.v.as<float>() + off, tb.t.as<float>() + off, tb.wt.as<float>() + off,
+                     tb.w.as<float>(), tb.g.as<float>(), tb.loss.as<float>());
+  PGAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(mlp::mlp_reduce_adam, dim3(mlp::kParams + 1), dim3(256), 0, st, n,
+                     tb.g.as<float>(), tb.loss.as<float>(), tb.w.as<float>(), tb.m.as<float>(),
+                     tb.u.as<float>(), tb.gsum.as<float>(), adam, step, lr, 0.9f, 0.999f, 1e-8f);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+void tb_alloc(TrainBufs& tb, int nset, int batch) {
+  tb.v.reserve(sizeof(float) * nset);
+  tb.t.reserve(sizeof(float) * nset);
+  tb.wt.reserve(sizeof(float) * nset);
+  tb.w.reserve(sizeof(float) * mlp::kParams);
+  tb.m.reserve(sizeof(float) * mlp::kParams);
+  tb.u.reserve(sizeof(float) * mlp::kParams);
+  tb.g.reserve(sizeof(float) * (size_t)mlp::kParams * batch);
+  tb.loss.reserve(sizeof(float) * batch);
+  tb.gsum.reserve(sizeof(float) * (mlp::kParams + 1));
+}
+
+void tb_release(TrainBufs& tb) {
+  for (DevBuf* d : {&tb.v, &tb.t, &tb.wt, &tb.w, &tb.m, &tb.u, &tb.g, &tb.loss, &tb.gsum})
+    d->release();
+}
+}  // namespace
+
+// Summed gradient and loss of the weighted MSE sum_s wt_s (Y(v_s) - t_s)^2 / n
+// for host arrays v, t, wt (n samples) at parameters p[321], computed by the
+// training kernels on the GPU.  For checking them against a CPU reference.
+PGAMD_API int pg5_mlp_grad(int n, const float* v, const float* t, const float* wt,
+                           const float* p, float* grad, double* loss) {
+  Board* b = board(0, "pg5_mlp_grad");
+  if (!b) return -1;
+  if (n <= 0) {
+    api_error(3, "pg5_mlp_grad: n=%d", n);
+    return -1;
+  }
+  TrainBufs tb;
+  tb_alloc(tb, n, n);
+  hipStream_t st = b->stream;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.v.ptr, v, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.t.ptr, t, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.wt.ptr, wt, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, p, sizeof(float) * mlp::kParams,
+                                 hipMemcpyHostToDevice, st));
+  mlp_step(tb, n, 0, 0, 1, 0.f, st);
+  std::vector<float> gs(mlp::kParams + 1);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
+                                 hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  std::memcpy(grad, gs.data(), sizeof(float) * mlp::kParams);
+  if (loss) *loss = gs[mlp::kParams];
+  tb_release(tb);
+  return 0;
+}
+
+// Trains the network on the GPU for the current cut-off shape (the function
+// the table was generated from, pg5_gen_force_table / pg5_gen_s2_force_table):
+// 65536 seeded samples of r in [0, r_c) (half uniform in v = sqrt(r/r_c),
+// half uniform in volume), target Y = f(r) (r^3 + r0^3) evaluated in FP64,
+// weight 1 / (|Y| + 0.05)^2 (relative accuracy where Y is O(1)); Adam with a
+// cosine learning-rate decay to lr/100 over `steps` mini-batches of `batch`.
+// Deterministic for a given seed.  *loss_out = weighted MSE on the full set.
+PGAMD_API int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, double* loss_out) {
+  Board* b = board(0, "pg5_mlp_train");
+  if (!b) return -1;
+  if (steps <= 0 || batch <= 0 || batch > (1 << 16) || !(lr > 0.0)) {
+    api_error(3, "pg5_mlp_train: steps=%d batch=%d lr=%g", steps, batch, lr);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_table();
+  double (*func)(double) = g_tab.func ? g_tab.func : s2_force;
+  const double rc = g_tab.rcut;
+  const double r0 = mlp_r0();
+  const double r03 = r0 * r0 * r0;
+  const int nset = 1 << 16;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::vector<float> hv(nset), ht(nset), hw(nset);
+  for (int k = 0; k < nset; ++k) {
+    double r, v;
+    if (k & 1) {
+      r = rc * std::cbrt(U(rng));
+      v = std::sqrt(r / rc);
+    } else {
+      v = U(rng);
+      r = rc * v * v;
+    }
+    const double y = func(r) * (r * r * r + r03);
+    hv[k] = (float)v;
+    ht[k] = (float)y;
+    hw[k] = (float)(1.0 / ((std::fabs(y) + 0.05) * (std::fabs(y) + 0.05)));
+  }
+  // initial parameters
+  std::vector<float> p(mlp::kParams, 0.f);
+  std::uniform_real_distribution<float> Uf(-1.f, 1.f);
+  for (int j = 0; j < mlp::kH; ++j) {
+    p[mlp::kW1 + j] = 4.f * Uf(rng);
+    p[mlp::kB1 + j] = 2.f * Uf(rng);
+  }
+  const float lim = std::sqrt(6.f / (2 * mlp::kH));
+  for (int k = 0; k < mlp::kH * mlp::kH; ++k) p[mlp::kW2 + k] = lim * Uf(rng);
+  for (int k = 0; k < mlp::kH; ++k) p[mlp::kW3 + k] = lim * Uf(rng);
+  double mean = 0.0;
+  for (float y : ht) mean += y;
+  p[mlp::kB3] = (float)(mean / nset);
+
+  TrainBufs tb;
+  tb_alloc(tb, nset, std::max(batch, 4096));
+  hipStream_t st = b->stream;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.v.ptr, hv.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.t.ptr, ht.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.wt.ptr, hw.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, p.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
+  std::vector<float> zero(mlp::kParams, 0.f);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.m.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.u.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
+  const int span = nset - batch + 1;
+  for (int step = 1; step <= steps; ++step) {
+    const int off = (int)(((long long)(step - 1) * batch) % span);
+    const double c = 0.5 * (1.0 + std::cos(M_PI * (double)step / steps));
+    const float lr_t = (float)(lr * (0.01 + 0.99 * c));
+    mlp_step(tb, batch, off, 1, step, lr_t, st);
+  }
+  // full-set loss, in chunks of 4096
+  double tot = 0.0;
+  std::vector<float> gs(mlp::kParams + 1);
+  for (int off = 0; off < nset; off += 4096) {
+    const int n = std::min(4096, nset - off);
+    mlp_step(tb, n, off, 0, 1, 0.f, st);
+    PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
+                                   hipMemcpyDeviceToHost, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    tot += (double)gs[mlp::kParams] * n / nset;
+  }
+  PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), tb.w.ptr, sizeof(float) * mlp::kParams,
+                                 hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  tb_release(tb);
+  g_mlp.params = p;
+  g_mlp.rc = rc;
+  g_mlp.r0 = r0;
+  g_mlp.valid = true;
+  g_mlp.dirty = true;
+  if (loss_out) *loss_out = tot;
+  return 0;
+}
+
+// Learned f(r) at n physical radii (device arrays), synchronously.
+PGAMD_API int pg5_mlp_eval_dev(int n, const float* r, float* f) {
+  Board* b = board(0, "pg5_mlp_eval_dev");
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!g_mlp.valid) {
+    api_error(5, "pg5_mlp_eval_dev: no learned cut-off");
+    return -1;
+  }
+  const float* w = mlp_dev(b->stream);
+  if (n > 0) {
+    hipLaunchKernelGGL(mlp::mlp_eval_f, dim3(ceil_div(n, 256)), dim3(256), 0, b->stream, n, r, w,
+                       (float)g_mlp.rc, (float)(g_mlp.r0 * g_mlp.r0 * g_mlp.r0), f);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------ batched TreePM
+
+namespace {
+bool batch_check(const char* fn, int mode, int ngroups) {
+  if (mode != 0 && mode != 1) {
+    api_error(3, "%s: mode %d not in {0, 1}", fn, mode);
+    return false;
+  }
+  if (mode == 1 && !g_mlp.valid) {
+    api_error(5, "%s: no learned cut-off (train or load one first)", fn);
+    return false;
+  }
+  if (ngroups < 0) {
+    api_error(3, "%s: ngroups=%d", fn, ngroups);
+    return false;
+  }
+  return true;
+}
+
+struct BatchGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf counter;
+};
+}  // namespace
+
+// Short-range force of every group in one persistent launch (treepm_batch.h):
+// pos[np] float4 scaled {x*xs, y*xs, z*xs, m}; group g: i-particles
+// ilist[ioff[g] .. +icnt[g]), j-particles jlist[joff[g] .. +jcnt[g]); order
+// (nullable) = processing order of the groups; acc[np][3] FP64 written for
+// the listed i.  mode 0 = table, 1 = learned MLP.  All device pointers;
+// asynchronous on stream.  Calls on one devid share its work counter.
+PGAMD_API int pg5t_batch_force(int devid, int ngroups, const int* ioff, const int* icnt,
+                               const int* ilist, const int* joff, const int* jcnt,
+                               const int* jlist, const int* order, const void* pos, double* acc,
+                               int mode, void* stream) {
+  Board* b = board(devid, "pg5t_batch_force");
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!batch_check("pg5t_batch_force", mode, ngroups)) return -1;
+  ensure_table();
+  hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+  b->d_counter.reserve(256);
+  launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
+               static_cast<const float4*>(pos), acc, b->d_counter.as<int>(), default_bpc(mode),
+               st);
+  return 0;
+}
+
+// Captures the same launch (counter reset + persistent kernel) into a HIP
+// graph with its own work counter; replay with pg5t_batch_graph_launch.
+// Inputs are read at replay time from the same device addresses.
+PGAMD_API void* pg5t_batch_graph(int devid, int ngroups, const int* ioff, const int* icnt,
+                                 const int* ilist, const int* joff, const int* jcnt,
+                                 const int* jlist, const int* order, const void* pos,
+                                 double* acc, int mode) {
+  Board* b = board(devid, "pg5t_batch_graph");
+  if (!b) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!batch_check("pg5t_batch_graph", mode, ngroups)) return nullptr;
+  ensure_table();
+  // uploads happen before capture
+  table_dev(b->stream);
+  if (mode == 1) mlp_dev(b->stream);
+  BatchGraph* bg = new BatchGraph();
+  bg->counter.reserve(256);
+  hipStream_t cs;
+  PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
+               static_cast<const float4*>(pos), acc, bg->counter.as<int>(), default_bpc(mode), cs);
+  PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &bg->graph));
+  PGAMD_HIP_CHECK(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
+  PGAMD_HIP_CHECK(hipStreamDestroy(cs));
+  return bg;
+}
+
+PGAMD_API int pg5t_batch_graph_launch(void* h, void* stream) {
+  if (!h) {
+    api_error(3, "pg5t_batch_graph_launch: null graph");
+    return -1;
+  }
+  BatchGraph* bg = static_cast<BatchGraph*>(h);
+  hipStream_t st = stream ? (hipStream_t)stream : board(0, "pg5t_batch_graph_launch")->stream;
+  PGAMD_HIP_CHECK(hipGraphLaunch(bg->exec, st));
+  return 0;
+}
+
+PGAMD_API int pg5t_batch_graph_destroy(void* h) {
+  if (!h) return 0;
+  BatchGraph* bg = static_cast<BatchGraph*>(h);
+  PGAMD_HIP_CHECK(hipDeviceSynchronize());
+  PGAMD_HIP_CHECK(hipGraphExecDestroy(bg->exec));
+  PGAMD_HIP_CHECK(hipGraphDestroy(bg->graph));
+  bg->counter.release();
+  delete bg;
+  return 0;
+}
+
+// Autotuner: times the batch launch for persistent-grid sizes of
+// {1, 2, 3, 4, 6, 8, 12, 16} blocks per CU on these inputs (one warm-up, then
+// the mean of `reps`), keeps the fastest for this mode and returns it through
+// *best; ms[8] (nullable) receives the timings.
+PGAMD_API int pg5t_batch_autotune(int devid, int ngroups, const int* ioff, const int* icnt,
+                                  const int* ilist, const int* joff, const int* jcnt,
+                                  const int* jlist, const int* order, const void* pos,
+                                  double* acc, int mode, int reps, int* best, float* ms) {
+  Board* b = board(devid, "pg5t_batch_autotune");
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!batch_check("pg5t_batch_autotune", mode, ngroups)) return -1;
+  ensure_table();
+  const int cand[8] = {1, 2, 3, 4, 6, 8, 12, 16};
+  hipStream_t st = b->stream;
+  b->d_counter.reserve(256);
+  hipEvent_t e0, e1;
+  PGAMD_HIP_CHECK(hipEventCreate(&e0));
+  PGAMD_HIP_CHECK(hipEventCreate(&e1));
+  float bestt = 1e30f;
+  int bestc = default_bpc(mode);
+  reps = std::max(1, reps);
+  for (int k = 0; k < 8; ++k) {
+    auto go = [&] {
+      launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
+                   static_cast<const float4*>(pos), acc, b->d_counter.as<int>(), cand[k], st);
+    };
+    go();
+    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) go();
+    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    t /= reps;
+    if (ms) ms[k] = t;
+    if (t < bestt) {
+      bestt = t;
+      bestc = cand[k];
+    }
+  }
+  PGAMD_HIP_CHECK(hipEventDestroy(e0));
+  PGAMD_HIP_CHECK(hipEventDestroy(e1));
+  g_batch_bpc[mode] = bestc;
+  if (best) *best = bestc;
+  return 0;
+}
+
+PGAMD_API int pg5t_batch_set_blocks_per_cu(int mode, int bpc) {
+  if ((mode != 0 && mode != 1) || bpc < 0 || bpc > 64) {
+    api_error(3, "pg5t_batch_set_blocks_per_cu: mode=%d bpc=%d", mode, bpc);
+    return -1;
+  }
+  g_batch_bpc[mode] = bpc;
+  return 0;
 }

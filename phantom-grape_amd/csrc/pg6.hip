@@ -16,6 +16,7 @@
 #include "pgamd_api.h"
 #include "pg6_kernels.h"
 #include "pg6_nblist.h"
+#include "cluster_batch.h"
 
 #include <algorithm>
 #include <vector>
@@ -801,5 +802,166 @@ PGAMD_API int pg6_dev_timing_read(double* ms, int* count) {
   if (count) *count = g_timer.used;
   g_timer.on = false;
   g_timer.used = 0;
+  return 0;
+}
+
+// ------------------------------------------------------ batched many clusters
+
+namespace {
+int g_cl_bpc = 0;  // persistent blocks per CU (0 = default 4)
+DevBuf g_cl_counter;
+
+struct ClArgs {
+  int nitems;
+  const int2* items;
+  const int* coff;
+  const double *pos, *vel, *mass;
+  const float* eps2;
+  double *acc, *jrk, *pot;
+};
+
+void launch_cl(const ClArgs& a, int* counter, int bpc, hipStream_t st) {
+  hipLaunchKernelGGL(pgc::zero_counter, dim3(1), dim3(64), 0, st, counter);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  if (a.nitems <= 0) return;
+  const int blocks = std::max(1, std::min(a.nitems, device_cu_count() * bpc));
+  hipLaunchKernelGGL(pgc::cluster_force, dim3(blocks), dim3(pgc::kThreads), 0, st, a.nitems,
+                     a.items, a.coff, a.pos, a.vel, a.mass, a.eps2, a.acc, a.jrk, a.pot, counter);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
+int cl_bpc() { return g_cl_bpc > 0 ? g_cl_bpc : 4; }
+
+struct ClGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf counter;
+};
+}  // namespace
+
+// Work items for pg6_cluster_force from host cluster offsets coff[ncl+1]:
+// one item (cluster, first i) per chunk of up to 256 i-particles, ordered by
+// decreasing cost (chunk size x cluster size) so the largest start first.
+// Writes 2*count ints to items (if non-null, at most maxitems items);
+// returns the number of items.
+PGAMD_API int pg6_cluster_items(int ncl, const int* coff, int* items, int maxitems) {
+  std::vector<std::pair<long long, std::pair<int, int>>> v;
+  for (int c = 0; c < ncl; ++c) {
+    const int n = coff[c + 1] - coff[c];
+    for (int i0 = 0; i0 < n; i0 += pgc::kThreads)
+      v.push_back({-(long long)std::min(pgc::kThreads, n - i0) * n, {c, i0}});
+  }
+  std::stable_sort(v.begin(), v.end(),
+                   [](const auto& a, const auto& b) { return a.first < b.first; });
+  if (items) {
+    if ((int)v.size() > maxitems) {
+      api_error(3, "pg6_cluster_items: %d items exceed maxitems=%d", (int)v.size(), maxitems);
+      return -1;
+    }
+    for (size_t k = 0; k < v.size(); ++k) {
+      items[2 * k] = v[k].second.first;
+      items[2 * k + 1] = v[k].second.second;
+    }
+  }
+  return (int)v.size();
+}
+
+// Acc, jerk and pot of every particle of many independent clusters in one
+// persistent launch (cluster_batch.h).  Device arrays: coff[ncl+1],
+// items[2*nitems] from pg6_cluster_items, pos/vel [n][3], mass[n] (FP64),
+// eps2[ncl] (FP32); outputs acc/jrk [n][3], pot[n].  Asynchronous on stream.
+PGAMD_API int pg6_cluster_force(int nitems, const int* items, const int* coff, const double* pos,
+                                const double* vel, const double* mass, const float* eps2,
+                                double* acc, double* jrk, double* pot, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_cluster_force")) return -1;
+  if (nitems < 0) {
+    api_error(3, "pg6_cluster_force: nitems=%d", nitems);
+    return -1;
+  }
+  g_cl_counter.reserve(256);
+  launch_cl(ClArgs{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
+                   jrk, pot},
+            g_cl_counter.as<int>(), cl_bpc(), stream ? (hipStream_t)stream : g_board.stream);
+  return 0;
+}
+
+PGAMD_API void* pg6_cluster_graph(int nitems, const int* items, const int* coff,
+                                  const double* pos, const double* vel, const double* mass,
+                                  const float* eps2, double* acc, double* jrk, double* pot) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_cluster_graph")) return nullptr;
+  ClGraph* g = new ClGraph();
+  g->counter.reserve(256);
+  hipStream_t cs;
+  PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  launch_cl(ClArgs{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
+                   jrk, pot},
+            g->counter.as<int>(), cl_bpc(), cs);
+  PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
+  PGAMD_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  PGAMD_HIP_CHECK(hipStreamDestroy(cs));
+  return g;
+}
+
+PGAMD_API int pg6_cluster_graph_launch(void* h, void* stream) {
+  if (!h) {
+    api_error(3, "pg6_cluster_graph_launch: null graph");
+    return -1;
+  }
+  PGAMD_HIP_CHECK(hipGraphLaunch(static_cast<ClGraph*>(h)->exec,
+                                 stream ? (hipStream_t)stream : g_board.stream));
+  return 0;
+}
+
+PGAMD_API int pg6_cluster_graph_destroy(void* h) {
+  if (!h) return 0;
+  ClGraph* g = static_cast<ClGraph*>(h);
+  PGAMD_HIP_CHECK(hipDeviceSynchronize());
+  PGAMD_HIP_CHECK(hipGraphExecDestroy(g->exec));
+  PGAMD_HIP_CHECK(hipGraphDestroy(g->graph));
+  g->counter.release();
+  delete g;
+  return 0;
+}
+
+// Autotuner over {1, 2, 3, 4, 6, 8, 12, 16} persistent blocks per CU.
+PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff,
+                                   const double* pos, const double* vel, const double* mass,
+                                   const float* eps2, double* acc, double* jrk, double* pot,
+                                   int reps, int* best, float* ms) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_cluster_autotune")) return -1;
+  const ClArgs a{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
+                 jrk, pot};
+  g_cl_counter.reserve(256);
+  const int cand[8] = {1, 2, 3, 4, 6, 8, 12, 16};
+  hipStream_t st = g_board.stream;
+  hipEvent_t e0, e1;
+  PGAMD_HIP_CHECK(hipEventCreate(&e0));
+  PGAMD_HIP_CHECK(hipEventCreate(&e1));
+  float bestt = 1e30f;
+  int bestc = cl_bpc();
+  reps = std::max(1, reps);
+  for (int k = 0; k < 8; ++k) {
+    launch_cl(a, g_cl_counter.as<int>(), cand[k], st);
+    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch_cl(a, g_cl_counter.as<int>(), cand[k], st);
+    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    t /= reps;
+    if (ms) ms[k] = t;
+    if (t < bestt) {
+      bestt = t;
+      bestc = cand[k];
+    }
+  }
+  PGAMD_HIP_CHECK(hipEventDestroy(e0));
+  PGAMD_HIP_CHECK(hipEventDestroy(e1));
+  g_cl_bpc = bestc;
+  if (best) *best = bestc;
   return 0;
 }

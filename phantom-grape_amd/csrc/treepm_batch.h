@@ -1,0 +1,149 @@
+// Batched TreePM short-range force: every tree i-group with its j-list in one
+// persistent launch.  No counterpart in the reference, whose caller loops over
+// groups with g5_set_xmjMC + g5_calculate_force_on_xMC inside an OpenMP loop
+// (the pattern of PG5/newton_quad/source.cpp:813-860).
+//
+// Inputs (device): pos[np] float4 {x*xs, y*xs, z*xs, m} (positions scaled as
+// in PG5/table/libpg5/phantom_g5.c:112-128; periodic images are ghost copies
+// appended by the caller), per group g an i-list ilist[ioff[g] .. +icnt[g])
+// and a j-list jlist[joff[g] .. +jcnt[g]) (groups may share list ranges), an
+// optional processing order (groups by decreasing cost), and a work counter
+// that must be 0 at launch.
+// Each block repeatedly takes the next group from the counter; a group's
+// i-particles (up to 256 at a time) are spread over the block's lanes and the
+// j-list is staged in LDS tiles of 256 (gathered through jlist).  The cut-off
+// shape comes from the evaluator: the LDS table (bit-identical per pair with
+// the drop-in kernel) or the learned MLP.  Results: acc[i][3] FP64 (written,
+// not accumulated) for every listed i.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mlp_kernels.h"
+#include "table_kernels.h"
+
+namespace pgb {
+
+constexpr int kThreads = 256;
+constexpr int kJTile = 256;
+
+// Table evaluator: s = 2 + |dx_s|^2, f from the LDS table.
+struct TableCut {
+  const float2* gtab;
+  int ntab, frc_bit;
+};
+
+// MLP evaluator, scaled units: r_s = |dx_s|, v = sqrt(r_s / rc_s),
+// f = xs^3 Y(v) / (r_s^3 + r0_s^3) for 0 < r_s < rc_s.
+struct MlpCut {
+  const float* w;
+  float inv_rc_s;  // 1 / (r_c xs)
+  float rc2_s;     // (r_c xs)^2
+  float r03_s;     // (r_0 xs)^3
+  float xs3;       // xs^3
+};
+
+__device__ inline int take_item(int* counter, int* s_item) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
+  __syncthreads();
+  return *s_item;
+}
+
+template <bool MLP>
+__global__ __launch_bounds__(kThreads) void treepm_batch(
+    int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
+    const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ joff,
+    const int* __restrict__ jcnt, const int* __restrict__ jlist,
+    const float4* __restrict__ pos, TableCut tc, MlpCut mc,
+    double ascale, double* __restrict__ acc, int* counter) {
+  extern __shared__ float2 s_tab[];  // table entries (table evaluator only)
+  __shared__ float4 s_j[kJTile];
+  __shared__ double s_red[3][kThreads];
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  int shift = 0, base = 0;
+  float scut = 0.f;
+  if (!MLP) {
+    for (int k = tid; k < tc.ntab; k += kThreads) s_tab[k] = tc.gtab[k];
+    shift = 23 - tc.frc_bit;
+    base = 1 << (30 - shift);
+    scut = __uint_as_float((1u << 30) | ((unsigned)(tc.ntab - 1) << shift));
+  }
+  for (;;) {
+    const int it = take_item(counter, &s_item);
+    if (it >= ngroups) break;
+    const int grp = order ? order[it] : it;
+    const int i0 = ioff[grp], ni = icnt[grp];
+    const int j0 = joff[grp], nj = jcnt[grp];
+    for (int ib = 0; ib < ni; ib += kThreads) {
+      const int nib = min(kThreads, ni - ib);
+      const int bi = nib <= 64 ? 64 : (nib <= 128 ? 128 : 256);
+      const int G = kThreads / bi;
+      const int il = tid % bi;
+      const int g = tid / bi;
+      const bool iv = il < nib;
+      const int ii = ilist[i0 + ib + (iv ? il : 0)];
+      const float4 ip = pos[ii];
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int jt = 0; jt < nj; jt += kJTile) {
+        const int nt = min(kJTile, nj - jt);
+        __syncthreads();
+        if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
+        __syncthreads();
+        float fax = 0.f, fay = 0.f, faz = 0.f;
+#pragma unroll 2
+        for (int jj = g; jj < nt; jj += G) {
+          const float4 q = s_j[jj];
+          const float dx = q.x - ip.x;
+          const float dy = q.y - ip.y;
+          const float dz = q.z - ip.z;
+          float f;
+          if (MLP) {
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float u = r * mc.inv_rc_s;
+            const float v = u * __builtin_amdgcn_rsqf(u);
+            const float y = mlp::eval(mlp::fresh(mlp::as_const(mc.w)), v);
+            const float fv = mc.xs3 * y * __builtin_amdgcn_rcpf(fmaf(r2, r, mc.r03_s));
+            f = (r2 > 0.f && r2 < mc.rc2_s) ? fv : 0.f;
+          } else {
+            const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
+            f = pgt::table_f(s_tab, s, scut, shift, base);
+          }
+          fax = fmaf(f, q.w * dx, fax);
+          fay = fmaf(f, q.w * dy, fay);
+          faz = fmaf(f, q.w * dz, faz);
+        }
+        ax += (double)fax;
+        ay += (double)fay;
+        az += (double)faz;
+      }
+      if (G > 1) {
+        __syncthreads();
+        s_red[0][tid] = ax;
+        s_red[1][tid] = ay;
+        s_red[2][tid] = az;
+        __syncthreads();
+        if (g == 0)
+          for (int q = 1; q < G; ++q) {
+            ax += s_red[0][q * bi + il];
+            ay += s_red[1][q * bi + il];
+            az += s_red[2][q * bi + il];
+          }
+      }
+      if (g == 0 && iv) {
+        acc[3 * (size_t)ii + 0] = ax * ascale;
+        acc[3 * (size_t)ii + 1] = ay * ascale;
+        acc[3 * (size_t)ii + 2] = az * ascale;
+      }
+    }
+  }
+}
+
+// Clears the work counter (a kernel, not a memset, so it can sit in a graph).
+__global__ void zero_counter(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+}  // namespace pgb

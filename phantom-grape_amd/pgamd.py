@@ -131,6 +131,12 @@ class G5:
         _sig(L, "pg5_dev_quad", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
         _sig(L, "pg5_dev_stream", _vp, C.c_int)
         _sig(L, "pg5_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
+        bh = [C.c_int] * 1 + [_vp] * 10 + [_vp, _vp, C.c_double, _vp, _vp]
+        _sig(L, "pg5_bh_batch", C.c_int, C.c_int, *bh, _vp)
+        _sig(L, "pg5_bh_graph", _vp, C.c_int, *bh)
+        _sig(L, "pg5_bh_graph_launch", C.c_int, _vp, _vp)
+        _sig(L, "pg5_bh_graph_destroy", C.c_int, _vp)
+        _sig(L, "pg5_bh_autotune", C.c_int, C.c_int, *bh, C.c_int, _ip, _fp)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
         L.g5_open()
@@ -219,6 +225,13 @@ class G6:
         _sig(L, "pg6_dev_read_predicted", C.c_int, C.c_int, _dp, _fp, _fp)
         _sig(L, "pg6_dev_timing_start", C.c_int, C.c_int)
         _sig(L, "pg6_dev_timing_read", C.c_int, _dp, _ip)
+        _sig(L, "pg6_cluster_items", C.c_int, C.c_int, _ip, _ip, C.c_int)
+        cl = [C.c_int] + [_vp] * 9
+        _sig(L, "pg6_cluster_force", C.c_int, *cl, _vp)
+        _sig(L, "pg6_cluster_graph", _vp, *cl)
+        _sig(L, "pg6_cluster_graph_launch", C.c_int, _vp, _vp)
+        _sig(L, "pg6_cluster_graph_destroy", C.c_int, _vp)
+        _sig(L, "pg6_cluster_autotune", C.c_int, *cl, C.c_int, _ip, _fp)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
         L.g6_open(0)
@@ -289,8 +302,140 @@ class G6:
         self.lib.pg6_dev_plan(ni, nj, iptr(o))
         return tuple(int(v) for v in o)
 
+    def cluster_items(self, coff):
+        coff = i32(coff)
+        n = self.lib.pg6_cluster_items(len(coff) - 1, iptr(coff), None, 0)
+        items = np.zeros(2 * n, np.int32)
+        self.lib.pg6_cluster_items(len(coff) - 1, iptr(coff), iptr(items), n)
+        return items
+
     def neighbour_list(self, ipipe, maxlen=1024):
         n = np.zeros(1, np.int32)
         buf = np.zeros(max(maxlen, 1), np.int32)
         rc = self.lib.g6_get_neighbour_list(0, ipipe, maxlen, iptr(n), iptr(buf))
         return rc, buf[: n[0]].copy()
+
+
+# --------------------------------------------------------- table (TreePM)
+class G5Table:
+    """Tabulated-cutoff GRAPE-5 board (libpg5_table.so) with the learned
+    cut-off and the batched TreePM entry points."""
+
+    def __init__(self):
+        self.lib = L = load("pg5_table")
+        _sig(L, "g5_open", None)
+        _sig(L, "pg5_gen_s2_force_table", None, C.c_double, C.c_double)
+        _sig(L, "pg5_set_table_frc_bit", C.c_int, C.c_int)
+        _sig(L, "pg5_get_table", C.c_int, _fp, _dp, _fp)
+        _sig(L, "g5_set_xmj", None, C.c_int, C.c_int, _dp, _dp)
+        _sig(L, "g5_set_xmjMC", None, C.c_int, C.c_int, C.c_int, _dp, _dp)
+        _sig(L, "g5_set_n", None, C.c_int)
+        _sig(L, "g5_set_nMC", None, C.c_int, C.c_int)
+        _sig(L, "g5_calculate_force_on_x", None, _dp, _dp, _dp, C.c_int)
+        _sig(L, "g5_calculate_force_on_xMC", None, C.c_int, _dp, _dp, _dp, C.c_int)
+        _sig(L, "pg5_set_cutoff_mode", C.c_int, C.c_int)
+        _sig(L, "pg5_get_cutoff_mode", C.c_int)
+        _sig(L, "pg5_mlp_set_params", C.c_int, _fp, C.c_double, C.c_double)
+        _sig(L, "pg5_mlp_get_params", C.c_int, _fp, _dp, _dp)
+        _sig(L, "pg5_mlp_grad", C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp, _dp)
+        _sig(L, "pg5_mlp_train", C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint, _dp)
+        _sig(L, "pg5_mlp_eval_dev", C.c_int, C.c_int, _vp, _vp)
+        _sig(L, "pg5t_dev_eval", C.c_int, C.c_int, _vp, _vp)
+        _sig(L, "pg5t_dev_force", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp)
+        bt = [C.c_int] + [_vp] * 9 + [C.c_int]
+        _sig(L, "pg5t_batch_force", C.c_int, C.c_int, *bt, _vp)
+        _sig(L, "pg5t_batch_graph", _vp, C.c_int, *bt)
+        _sig(L, "pg5t_batch_graph_launch", C.c_int, _vp, _vp)
+        _sig(L, "pg5t_batch_graph_destroy", C.c_int, _vp)
+        _sig(L, "pg5t_batch_autotune", C.c_int, C.c_int, *bt, C.c_int, _ip, _fp)
+        _sig(L, "pg5t_batch_set_blocks_per_cu", C.c_int, C.c_int, C.c_int)
+        _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
+        _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
+        L.g5_open()
+
+    def table(self):
+        n = self.lib.pg5_get_table(None, None, None)
+        t = np.zeros(2 * n, np.float32)
+        xs = np.zeros(1)
+        self.lib.pg5_get_table(fptr(t), dptr(xs), None)
+        return t, float(xs[0])
+
+    def force(self, xi, xj, mj, devid=0):
+        xi, xj, mj = f64(xi), f64(xj), f64(mj)
+        self.lib.g5_set_xmjMC(devid, 0, len(mj), dptr(xj), dptr(mj))
+        self.lib.g5_set_nMC(devid, len(mj))
+        a = np.zeros((len(xi), 3))
+        p = np.zeros(len(xi))
+        self.lib.g5_calculate_force_on_xMC(devid, dptr(xi), dptr(a), dptr(p), len(xi))
+        return a
+
+    def mlp_params(self):
+        p = np.zeros(321, np.float32)
+        rc = np.zeros(1)
+        r0 = np.zeros(1)
+        ok = self.lib.pg5_mlp_get_params(fptr(p), dptr(rc), dptr(r0))
+        return (p, float(rc[0]), float(r0[0])) if ok else None
+
+    def mlp_train(self, steps=4000, batch=4096, lr=3e-3, seed=1):
+        loss = np.zeros(1)
+        rc = self.lib.pg5_mlp_train(steps, batch, lr, seed, dptr(loss))
+        assert rc == 0
+        return float(loss[0])
+
+    def mlp_grad(self, v, t, wt, p):
+        v, t, wt, p = (np.ascontiguousarray(a, np.float32) for a in (v, t, wt, p))
+        g = np.zeros(321, np.float32)
+        loss = np.zeros(1)
+        assert self.lib.pg5_mlp_grad(len(v), fptr(v), fptr(t), fptr(wt), fptr(p), fptr(g),
+                                     dptr(loss)) == 0
+        return g, float(loss[0])
+
+
+# ------------------------------------------------------- interaction lists
+_TREE_FIELDS = ("pos", "src", "ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "coff", "ccnt",
+                "clist", "order", "cells")
+
+
+def _tree_result(L, h):
+    info = np.zeros(10, np.int64)
+    L.pgt_info(h, info.ctypes.data_as(C.c_void_p))
+    np_, ng, ni, nj, nc, ncell = (int(v) for v in info[:6])
+    sizes = dict(pos=(np_, 3), src=np_, ioff=ng, icnt=ng, ilist=ni, joff=ng, jcnt=ng, jlist=nj,
+                 coff=ng, ccnt=ng, clist=nc, order=ng, cells=(ncell, 10))
+    out = {}
+    for k, name in enumerate(_TREE_FIELDS):
+        dt = np.float64 if name in ("pos", "cells") else np.int32
+        a = np.zeros(sizes[name], dt)
+        if a.size:
+            L.pgt_get(h, k, a.ctypes.data_as(C.c_void_p))
+        out[name] = a
+    out["work_pp"] = int(info[6])
+    out["work_pc"] = int(info[7])
+    L.pgt_free(h)
+    return out
+
+
+def _tree_lib():
+    L = load("pgtree")
+    _sig(L, "pgt_treepm_build", _vp, C.c_int, _dp, C.c_double, C.c_int)
+    _sig(L, "pgt_bh_build", _vp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int)
+    _sig(L, "pgt_info", C.c_int, _vp, _vp)
+    _sig(L, "pgt_get", C.c_int, _vp, C.c_int, _vp)
+    _sig(L, "pgt_free", None, _vp)
+    return L
+
+
+def treepm_lists(x, rcut, ncrit=256):
+    """TreePM i-groups and j-lists (libpgtree.so, host): periodic unit box,
+    ghosts appended after the n originals (``src`` maps back)."""
+    L = _tree_lib()
+    x = f64(x)
+    return _tree_result(L, L.pgt_treepm_build(len(x), dptr(x), float(rcut), int(ncrit)))
+
+
+def bh_lists(x, m, theta=0.5, ncrit=512, nleaf=8):
+    """Barnes-Hut i-groups, particle lists and quadrupole cell lists."""
+    L = _tree_lib()
+    x, m = f64(x), f64(m)
+    return _tree_result(L, L.pgt_bh_build(len(x), dptr(x), dptr(m), float(theta), int(ncrit),
+                                          int(nleaf)))

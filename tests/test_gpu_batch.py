@@ -1,0 +1,281 @@
+"""Batched GPU paths (one persistent launch over many groups / clusters),
+their hipGraph replays and autotuners, and the learned cut-off: parity with
+the per-group drop-in calls, the CPU oracle and the closed forms."""
+import numpy as np
+import pytest
+
+import ics
+import mlp_oracle as M
+import oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+PP, PM = 0.1 / 128, 3.0 / 128
+
+
+def dev(a, dtype=None):
+    import torch
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+    return torch.from_numpy(a).cuda()
+
+
+def ptr(t):
+    return t.data_ptr()
+
+
+@pytest.fixture(scope="module")
+def tab():
+    import pgamd
+    t = pgamd.G5Table()
+    t.lib.pg5_gen_s2_force_table(PP, PM)
+    return t
+
+
+@pytest.fixture(scope="module")
+def treepm_case(tab):
+    import pgamd
+    m, x = ics.clumpy_box(1 << 15, 41)
+    L = pgamd.treepm_lists(x, PM, 256)
+    _, xs = tab.table()
+    P = L["pos"]
+    mm = m[L["src"]]
+    pos4 = np.zeros((len(P), 4), np.float32)
+    pos4[:, :3] = (P.astype(np.float32) * np.float32(xs))
+    pos4[:, 3] = mm
+    d = {k: dev(L[k]) for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
+    d["pos"] = dev(pos4)
+    return dict(L=L, m=mm, pos4=pos4, xs=xs, d=d, n=len(x))
+
+
+def run_treepm(tab, c, mode, acc=None):
+    import torch
+    d = c["d"]
+    if acc is None:
+        acc = torch.zeros((len(c["pos4"]), 3), dtype=torch.float64, device="cuda")
+    ng = len(c["L"]["ioff"])
+    rc = tab.lib.pg5t_batch_force(0, ng, ptr(d["ioff"]), ptr(d["icnt"]), ptr(d["ilist"]),
+                                  ptr(d["joff"]), ptr(d["jcnt"]), ptr(d["jlist"]), ptr(d["order"]),
+                                  ptr(d["pos"]), ptr(acc), mode, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return acc.cpu().numpy()
+
+
+def test_treepm_batch_table_matches_dropin_and_closed_form(tab, treepm_case):
+    import torch
+    c = treepm_case
+    L = c["L"]
+    a = run_treepm(tab, c, 0)
+    # the drop-in kernel on the same scaled positions, group by group
+    rng = np.random.default_rng(0)
+    for g in rng.choice(len(L["ioff"]), 6, replace=False):
+        il = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        jl = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        xi = dev(c["pos4"][il] * np.array([1, 1, 1, 0], np.float32))
+        xj = dev(c["pos4"][jl])
+        out = torch.zeros((len(il), 3), dtype=torch.float64, device="cuda")
+        assert tab.lib.pg5t_dev_force(0, len(il), ptr(xi), len(jl), ptr(xj), ptr(out), None) == 0
+        torch.cuda.synchronize()
+        ref = out.cpu().numpy()
+        scale = np.linalg.norm(ref, axis=1).max()
+        assert rel_err(a[il], ref, floor=1e-3 * scale).max() < 1e-6
+    # against the closed-form S2 force (FP64) on a sample of i
+    sel = rng.choice(c["n"], 300, replace=False)
+    f = O.table_force_f64(L["pos"][sel], L["pos"], c["m"], PP, PM)
+    scale = np.linalg.norm(f, axis=1).max()
+    e = rel_err(a[sel], f, floor=1e-3 * scale)
+    assert np.median(e) < 3e-3
+
+
+def test_treepm_graph_replay_and_autotune(tab, treepm_case):
+    import torch
+    c = treepm_case
+    d = c["d"]
+    ng = len(c["L"]["ioff"])
+    eager = run_treepm(tab, c, 0)
+    acc = torch.zeros((len(c["pos4"]), 3), dtype=torch.float64, device="cuda")
+    args = (0, ng, ptr(d["ioff"]), ptr(d["icnt"]), ptr(d["ilist"]), ptr(d["joff"]),
+            ptr(d["jcnt"]), ptr(d["jlist"]), ptr(d["order"]), ptr(d["pos"]), ptr(acc), 0)
+    h = tab.lib.pg5t_batch_graph(*args)
+    assert h
+    try:
+        for rep in range(3):
+            acc.zero_()
+            assert tab.lib.pg5t_batch_graph_launch(h, None) == 0
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(acc.cpu().numpy(), eager)
+            run_treepm(tab, c, 0)  # eager work between replays
+    finally:
+        tab.lib.pg5t_batch_graph_destroy(h)
+    best = np.zeros(1, np.int32)
+    ms = np.zeros(8, np.float32)
+    import pgamd
+    assert tab.lib.pg5t_batch_autotune(*args, 2, pgamd.iptr(best), pgamd.fptr(ms)) == 0
+    assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16) and np.all(ms > 0)
+    np.testing.assert_array_equal(run_treepm(tab, c, 0), eager)
+
+
+def test_mlp_gradient_kernel_matches_oracle(tab):
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.5, M.NP).astype(np.float32)
+    v = rng.uniform(0, 1, 3000).astype(np.float32)
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    wt = rng.uniform(0.5, 2, 3000).astype(np.float32)
+    g, loss = tab.mlp_grad(v, t, wt, p)
+    lo, go = M.loss_and_grad(p, v, t, wt)
+    assert abs(loss - lo) < 1e-5 * lo
+    assert np.max(np.abs(g - go)) < 1e-4 * np.max(np.abs(go))
+
+
+def test_mlp_training_and_learned_force(tab, treepm_case):
+    loss = tab.mlp_train(steps=6000, batch=4096, lr=3e-3, seed=7)
+    p, rc, r0 = tab.mlp_params()
+    assert abs(rc - PM) < 1e-12 and abs(r0 - PP) < 1e-12
+    # the trained network reproduces the cut-off shape: compare the GPU
+    # evaluation with the closed form on radii between r0 and 0.9 rc
+    import torch
+    r = np.geomspace(PP, 0.9 * PM, 2000).astype(np.float32)
+    dr = dev(r)
+    out = torch.empty_like(dr)
+    assert tab.lib.pg5_mlp_eval_dev(len(r), ptr(dr), ptr(out)) == 0
+    f = out.cpu().numpy().astype(np.float64)
+    fo = O.s2_force(PP, PM, r.astype(np.float64))
+    # the oracle forward pass agrees with the GPU's
+    v, _ = M.target(lambda rr: O.s2_force(PP, PM, rr), r.astype(np.float64), rc, r0)
+    yo = M.forward(p, v) / (r.astype(np.float64) ** 3 + r0 ** 3)
+    assert np.max(np.abs(f - yo) / np.abs(yo)) < 1e-4
+    e = np.abs(f - fo) / fo
+    assert np.median(e) < 5e-3, (loss, np.median(e))
+    # forces through the batch kernel with the learned cut-off
+    c = treepm_case
+    a = run_treepm(tab, c, 1)
+    L = c["L"]
+    sel = np.random.default_rng(1).choice(c["n"], 300, replace=False)
+    fa = O.table_force_f64(L["pos"][sel], L["pos"], c["m"], PP, PM)
+    scale = np.linalg.norm(fa, axis=1).max()
+    assert np.median(rel_err(a[sel], fa, floor=1e-3 * scale)) < 1e-2
+
+
+@pytest.fixture(scope="module")
+def bh_case():
+    import pgamd
+    g5 = pgamd.G5()
+    mh, xh = ics.hernquist(1 << 15, 51)
+    L = pgamd.bh_lists(xh, mh, 0.5, 512, 8)
+    P = L["pos"]
+    mm = mh[L["src"]]
+    pos4 = np.zeros((len(P), 4), np.float32)
+    pos4[:, :3] = P
+    pos4[:, 3] = mm
+    cells = np.zeros((len(L["cells"]), 12), np.float32)
+    cells[:, :10] = L["cells"]
+    d = {k: dev(L[k]) for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "coff",
+                                "ccnt", "clist", "order")}
+    d["pos"] = dev(pos4)
+    d["cells"] = dev(cells)
+    return dict(g5=g5, L=L, m=mm, pos4=pos4, cells=cells, d=d)
+
+
+def run_bh(c, eps):
+    import torch
+    d = c["d"]
+    n = len(c["pos4"])
+    acc = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    pot = torch.zeros(n, dtype=torch.float64, device="cuda")
+    args = (0, len(c["L"]["ioff"]), ptr(d["ioff"]), ptr(d["icnt"]), ptr(d["ilist"]),
+            ptr(d["joff"]), ptr(d["jcnt"]), ptr(d["jlist"]), ptr(d["coff"]), ptr(d["ccnt"]),
+            ptr(d["clist"]), ptr(d["order"]), ptr(d["pos"]), ptr(d["cells"]), eps * eps,
+            ptr(acc), ptr(pot))
+    return args, acc, pot
+
+
+def test_bh_batch_matches_oracle_and_graph(bh_case):
+    import torch
+    c = bh_case
+    L = c["L"]
+    eps = 1.0 / 128
+    args, acc, pot = run_bh(c, eps)
+    g5 = c["g5"]
+    assert g5.lib.pg5_bh_batch(*args, None) == 0
+    torch.cuda.synchronize()
+    a = acc.cpu().numpy()
+    p = pot.cpu().numpy()
+    f32 = lambda z: np.asarray(z, np.float32).astype(np.float64)  # noqa: E731
+    P4 = c["pos4"].astype(np.float64)
+    Cc = c["cells"].astype(np.float64)
+    for g in np.random.default_rng(2).choice(len(L["ioff"]), 5, replace=False):
+        il = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        pl = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        cl = L["clist"][L["coff"][g]:L["coff"][g] + L["ccnt"][g]]
+        ap, pp = O.pg5_force(P4[il, :3], f32(eps * eps), P4[pl, :3], P4[pl, 3])
+        aq, pq = O.quad_force(P4[il, :3], f32(eps * eps), Cc[cl, :3], Cc[cl, 3], Cc[cl, 4:10])
+        assert np.percentile(rel_err(a[il], ap + aq), 99) < 1e-5
+        assert rel_err(p[il], pp + pq).max() < 1e-5
+        # and the tree force is close to the direct sum (theta = 0.5 + quadrupole)
+        ad, _ = O.pg5_force(L["pos"][il], eps * eps, L["pos"], c["m"])
+        assert np.median(rel_err(a[il], ad)) < 1e-3
+    h = g5.lib.pg5_bh_graph(*args)
+    try:
+        for rep in range(3):
+            acc.zero_()
+            assert g5.lib.pg5_bh_graph_launch(h, None) == 0
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(acc.cpu().numpy(), a)
+            assert g5.lib.pg5_bh_batch(*args, None) == 0
+    finally:
+        g5.lib.pg5_bh_graph_destroy(h)
+    import pgamd
+    best = np.zeros(1, np.int32)
+    assert g5.lib.pg5_bh_autotune(*args, 2, pgamd.iptr(best), None) == 0
+    assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16)
+
+
+def test_clusters_match_oracle_and_graph():
+    import torch
+
+    import pgamd
+    g6 = pgamd.G6()
+    sizes = ics.cluster_sizes(48, 61, 64, 1024)
+    xs, vs, ms, e2 = [], [], [], []
+    for c, n in enumerate(sizes):
+        m, x, v = ics.plummer(int(n), 1000 + c)
+        xs.append(x + 10.0 * c)  # far apart (they never interact anyway)
+        vs.append(v)
+        ms.append(m)
+        e2.append((4.0 / n) ** 2)
+    coff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    X, V, Mm = np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
+    items = g6.cluster_items(coff)
+    n = len(Mm)
+    d = dict(items=dev(items), coff=dev(coff), pos=dev(X), vel=dev(V), mass=dev(Mm),
+             eps2=dev(np.array(e2, np.float32)))
+    acc = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    jrk = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    pot = torch.zeros(n, dtype=torch.float64, device="cuda")
+    args = (len(items) // 2, ptr(d["items"]), ptr(d["coff"]), ptr(d["pos"]), ptr(d["vel"]),
+            ptr(d["mass"]), ptr(d["eps2"]), ptr(acc), ptr(jrk), ptr(pot))
+    assert g6.lib.pg6_cluster_force(*args, None) == 0
+    torch.cuda.synchronize()
+    a, j, p = acc.cpu().numpy(), jrk.cpu().numpy(), pot.cpu().numpy()
+    for c in range(0, len(sizes), 5):
+        s = slice(coff[c], coff[c + 1])
+        idx = np.arange(coff[c + 1] - coff[c], dtype=np.int32)
+        vf = V[s].astype(np.float32).astype(np.float64)
+        o = O.pg6_force(X[s], vf, idx, float(np.float32(e2[c])), X[s], vf, Mm[s], idx)
+        assert np.percentile(rel_err(a[s], o["acc"]), 99) < 1e-6
+        assert np.percentile(rel_err(j[s], o["jrk"]), 99) < 1e-5
+        assert rel_err(p[s], o["pot"]).max() < 1e-6
+    h = g6.lib.pg6_cluster_graph(*args)
+    try:
+        for rep in range(3):
+            acc.zero_()
+            assert g6.lib.pg6_cluster_graph_launch(h, None) == 0
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(acc.cpu().numpy(), a)
+            assert g6.lib.pg6_cluster_force(*args, None) == 0
+    finally:
+        g6.lib.pg6_cluster_graph_destroy(h)
+    best = np.zeros(1, np.int32)
+    assert g6.lib.pg6_cluster_autotune(*args, 2, pgamd.iptr(best), None) == 0
+    assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16)

@@ -292,3 +292,32 @@ def test_quad_oracle_matches_reference_fixture():
     # reference: raw 12-bit rsqrtps, FP32 sums; the g5c potential is negative
     assert np.median(rel_err(g["acc"], a)) < 3e-4
     np.testing.assert_allclose(g["pot"], -p, rtol=3e-3)
+
+
+# --------------------------------------------------------- learned cut-off
+def test_mlp_oracle_gradient_matches_finite_differences():
+    import mlp_oracle as M
+    rng = np.random.default_rng(5)
+    p = rng.normal(0, 0.7, M.NP)
+    v = rng.uniform(0, 1, 64)
+    t = rng.normal(0, 1, 64)
+    wt = rng.uniform(0.5, 2, 64)
+    _, g = M.loss_and_grad(p, v, t, wt)
+    h = 1e-6
+    for k in list(range(0, 32, 5)) + [40, 100, 287, 290, 305, 320]:
+        e = np.zeros(M.NP)
+        e[k] = h
+        fd = (M.loss_and_grad(p + e, v, t, wt)[0] - M.loss_and_grad(p - e, v, t, wt)[0]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(g[k])), (k, fd, g[k])
+    # forward agrees with the loss' own forward
+    y = M.forward(p, v)
+    assert abs(M.loss_and_grad(p, v, y, wt)[0]) < 1e-28
+
+
+def test_mlp_target_is_bounded_and_smooth():
+    import mlp_oracle as M
+    pp, pm = 0.1 / 128, 3.0 / 128
+    r = np.linspace(0, pm, 20001)[:-1]
+    v, y = M.target(lambda rr: O.s2_force(pp, pm, rr), r, pm, pp)
+    assert y.max() < 13 and y.min() > -1e-6
+    assert np.max(np.abs(np.diff(y))) < 0.05  # no jumps on a fine grid
