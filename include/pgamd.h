@@ -39,6 +39,30 @@ void *pg5_dev_stream(int devid);
 /* bi in {0 = automatic, 64, 128, 256} i-particles per block; blocks_per_cu and
  * min_jper (> 0) set the j-range split. */
 int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
+/* Batched tree-list force (the work of the per-group g5_calculate_force_on_xMC
+ * + g5c_calculate_force_on_xMC pair of PG5/newton_quad/source.cpp:845-867 for
+ * every group) in one persistent launch.  Device arrays: pos[np] float4
+ * {x,y,z,m}; cells[nc] (12 floats as pg5_dev_quad); per group g the i-list
+ * ilist[ioff[g]..+icnt[g]), particle list plist[poff[g]..+pcnt[g]), cell list
+ * clist[coff[g]..+ccnt[g]); order (nullable) = processing order.  Writes
+ * acc[i][3] and pot[i] = + sum m/r + sum (phi_p + phi_q) for the listed i. */
+int pg5_bh_batch(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
+                 const int *poff, const int *pcnt, const int *plist, const int *coff,
+                 const int *ccnt, const int *clist, const int *order, const void *pos,
+                 const void *cells, double eps2, double *acc, double *pot, void *stream);
+/* The same launch captured in a HIP graph; replay / destroy. */
+void *pg5_bh_graph(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
+                   const int *poff, const int *pcnt, const int *plist, const int *coff,
+                   const int *ccnt, const int *clist, const int *order, const void *pos,
+                   const void *cells, double eps2, double *acc, double *pot);
+int pg5_bh_graph_launch(void *graph, void *stream);
+int pg5_bh_graph_destroy(void *graph);
+/* Autotuner over persistent blocks per CU {1,2,3,4,6,8,12,16}. */
+int pg5_bh_autotune(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
+                    const int *poff, const int *pcnt, const int *plist, const int *coff,
+                    const int *ccnt, const int *clist, const int *order, const void *pos,
+                    const void *cells, double eps2, double *acc, double *pot, int reps,
+                    int *best, float *ms);
 
 /* ------------------------------------------------------------------ libpg5_table.so
  * pgamd-library: pg5_table */
@@ -56,6 +80,36 @@ int pg5t_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
 int pg5t_dev_eval(int n, const float *s, float *out);
 int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 void *pg5t_dev_stream(int devid);
+/* Learned cut-off: a 1-16-16-1 tanh FP32 network, trained on the GPU on
+ * samples of the current cut-off shape, replacing the table lookup of
+ * gravity_kernel_avx2.c:74-100 (layout and target in mlp_kernels.h).
+ * mode 0 = table (default), 1 = network, for the drop-in calls. */
+int pg5_set_cutoff_mode(int mode);
+int pg5_get_cutoff_mode(void);
+int pg5_mlp_set_params(const float *p321, double rc, double r0);
+int pg5_mlp_get_params(float *p321, double *rc, double *r0);
+int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, double *loss);
+int pg5_mlp_grad(int n, const float *v, const float *t, const float *wt, const float *p321,
+                 float *grad321, double *loss);
+int pg5_mlp_eval_dev(int n, const float *r, float *f);
+/* Batched TreePM short-range force of every group in one persistent launch
+ * (the per-group g5_set_xmjMC + g5_calculate_force_on_xMC loop of a TreePM
+ * caller).  Device arrays: pos[np] float4 {x*xs,y*xs,z*xs,m}; per group g
+ * ilist[ioff[g]..+icnt[g]), jlist[joff[g]..+jcnt[g]); order nullable;
+ * acc[i][3] FP64 written for the listed i; mode 0 table, 1 network. */
+int pg5t_batch_force(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
+                     const int *joff, const int *jcnt, const int *jlist, const int *order,
+                     const void *pos, double *acc, int mode, void *stream);
+void *pg5t_batch_graph(int devid, int ngroups, const int *ioff, const int *icnt,
+                       const int *ilist, const int *joff, const int *jcnt, const int *jlist,
+                       const int *order, const void *pos, double *acc, int mode);
+int pg5t_batch_graph_launch(void *graph, void *stream);
+int pg5t_batch_graph_destroy(void *graph);
+int pg5t_batch_autotune(int devid, int ngroups, const int *ioff, const int *icnt,
+                        const int *ilist, const int *joff, const int *jcnt, const int *jlist,
+                        const int *order, const void *pos, double *acc, int mode, int reps,
+                        int *best, float *ms);
+int pg5t_batch_set_blocks_per_cu(int mode, int bpc);
 
 /* ------------------------------------------------------------------ libpg6.so
  * pgamd-library: pg6 */
@@ -86,6 +140,24 @@ int pg6_dev_read_predicted(int nj, double *pos, float *vel, float *mass);
  * kernel time (ms) and the number of launches timed, and stops timing. */
 int pg6_dev_timing_start(int max_launches);
 int pg6_dev_timing_read(double *ms, int *count);
+/* Many independent clusters (one GRAPE-6 session per cluster, each
+ * g6calc_firsthalf/lasthalf over its own members) in one persistent launch.
+ * pg6_cluster_items builds the work items (cluster, first i) from host
+ * offsets coff[ncl+1], largest first; returns their number. */
+int pg6_cluster_items(int ncl, const int *coff, int *items, int maxitems);
+/* Device arrays: items[2*nitems], coff[ncl+1], pos/vel [n][3], mass[n] FP64,
+ * eps2[ncl] FP32; outputs acc/jrk [n][3], pot[n] FP64. */
+int pg6_cluster_force(int nitems, const int *items, const int *coff, const double *pos,
+                      const double *vel, const double *mass, const float *eps2, double *acc,
+                      double *jrk, double *pot, void *stream);
+void *pg6_cluster_graph(int nitems, const int *items, const int *coff, const double *pos,
+                        const double *vel, const double *mass, const float *eps2, double *acc,
+                        double *jrk, double *pot);
+int pg6_cluster_graph_launch(void *graph, void *stream);
+int pg6_cluster_graph_destroy(void *graph);
+int pg6_cluster_autotune(int nitems, const int *items, const int *coff, const double *pos,
+                         const double *vel, const double *mass, const float *eps2, double *acc,
+                         double *jrk, double *pot, int reps, int *best, float *ms);
 
 #if defined(__cplusplus)
 }

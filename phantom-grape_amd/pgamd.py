@@ -26,7 +26,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
-LIBS = ("pg5", "pg5_sym", "pg5_table", "pg6")
+LIBS = ("pg5", "pg5_sym", "pg5_table", "pg6", "pgtree")
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
