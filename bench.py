@@ -83,6 +83,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel", type=int, default=-1,
+                    help="PG6 force kernel: 0 LDS j-tiles, 1 SGPR broadcast (default: library's)")
+    ap.add_argument("--configs", default="c1,c3,c4,c5",
+                    help="extra configurations to measure (comma list, '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -99,11 +103,18 @@ def main():
     log(f"device {torch.cuda.get_device_name(0)}")
     g6 = pgamd.G6()
     L = g6.lib
+    if args.kernel >= 0:
+        assert L.pg6_dev_set_kernel(args.kernel) == 0
     n = N
     m, x, v = ics.plummer(n, SEED)
     idx = np.arange(n, dtype=np.int32)
-    stream = torch.cuda.current_stream()
+    # A dedicated stream at the default priority: torch's default stream has
+    # the null handle, which the library reads as "the board's own stream".
+    # Every library call and every timing event below uses this stream.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
+    assert sp, "expected a non-null HIP stream handle"
 
     def dev_arrays(ni):
         return dict(acc=torch.empty((ni, 3), dtype=torch.float64, device=dev),
@@ -241,6 +252,13 @@ def main():
     if not args.no_cpu:
         result.update(cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk,
                               gpu_pot, value))
+    extra = {}
+    for c in [c for c in args.configs.split(",") if c]:
+        t0 = time.time()
+        extra[c] = EXTRA[c](args, stream, not args.no_cpu)
+        log(f"{c}: {json.dumps(extra[c])[:400]} ({time.time() - t0:.0f} s)")
+    if extra:
+        result["configs"] = extra
     print(json.dumps(result), flush=True)
 
 
@@ -296,6 +314,252 @@ def cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk, gpu_pot,
         "reference_accuracy": {"acc": stats(r["acc"], o["acc"]), "jerk": stats(r["jrk"], o["jrk"])},
     }
     return out
+
+
+# ------------------------------------------------------------ other configs
+def _errs(a, b):
+    import numpy as np
+    e = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+    return {"median": float(f"{np.median(e):.3e}"), "p99": float(f"{np.percentile(e, 99):.3e}"),
+            "normwise": float(f"{np.abs(a - b).sum() / np.abs(b).sum():.3e}")}
+
+
+def _time(fn, stream, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def bench_c1(args, stream, cpu):
+    """PG5 direct sum: Plummer N = 16384, eps = 1/64, one N x N force+pot."""
+    import numpy as np
+    import torch
+
+    import ics
+    import pgamd
+    n, eps = 16384, 1.0 / 64
+    m, x, _ = ics.plummer(n, 16384)
+    g5 = pgamd.G5()
+    xi = np.zeros((n, 4), np.float32)
+    xi[:, :3] = x
+    xi[:, 3] = eps * eps
+    xj = np.zeros((n, 4), np.float32)
+    xj[:, :3] = x
+    xj[:, 3] = m
+    d_xi = torch.from_numpy(xi).cuda()
+    d_xj = torch.from_numpy(xj).cuda()
+    acc = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    pot = torch.empty(n, dtype=torch.float64, device="cuda")
+    sp = stream.cuda_stream
+
+    def run():
+        assert g5.lib.pg5_dev_force(0, n, d_xi.data_ptr(), n, d_xj.data_ptr(), None,
+                                    acc.data_ptr(), pot.data_ptr(), sp) == 0
+    ms = _time(run, stream, 20)
+    inter = float(n) * n
+    out = {"workload": "pg5_plummer_n16384_eps1/64", "interactions": inter,
+           "ms": round(ms, 4), "ginteractions_s": round(inter / ms / 1e6, 1),
+           "pct_fp32_peak": round(100 * 38 * inter / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
+           "flops_per_interaction": 38}
+    O = _oracle()
+    sel = np.sort(np.random.default_rng(3).choice(n, 1024, replace=False))
+    oa, _ = O.pg5_force(x[sel], eps * eps, x, m)
+    out["accuracy_vs_fp64"] = _errs(acc.cpu().numpy()[sel], oa)
+    if cpu and O.ref_available("pg5_omp"):
+        R = O.RefPG5(omp=True)
+        ts = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            ra, _ = R.force(x[sel], x, m, eps)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts[1:]))
+        out["cpu_baseline"] = {"ginteractions_s": round(len(sel) * n / t / 1e9, 3),
+                               "kind": "reference", "sample": f"{len(sel)} i x {n} j",
+                               "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                               "accuracy_vs_fp64": _errs(ra, oa)}
+    return out
+
+
+def bench_c3(args, stream, cpu):
+    """TreePM short range: N = 2^20 clumpy periodic box, mesh 128^3,
+    r_cut = 3/128, eps_PP = 0.1/128; 1024-entry table vs learned MLP cut-off,
+    all groups in one graph-captured persistent launch."""
+    import numpy as np
+    import torch
+
+    import ics
+    import pgamd
+    pp, pm = 0.1 / 128, 3.0 / 128
+    m, x = ics.clumpy_box(1 << 20, 20)
+    t0 = time.time()
+    L = pgamd.treepm_lists(x, pm, 256)
+    tb = time.time() - t0
+    T = pgamd.G5Table()
+    T.lib.pg5_set_table_frc_bit(6)  # 1024 entries
+    T.lib.pg5_gen_s2_force_table(pp, pm)
+    _, xs = T.table()
+    t0 = time.time()
+    loss = T.mlp_train(steps=6000, batch=4096, lr=3e-3, seed=7)
+    ttrain = time.time() - t0
+    P = L["pos"]
+    mm = m[L["src"]]
+    pos4 = np.zeros((len(P), 4), np.float32)
+    pos4[:, :3] = P.astype(np.float32) * np.float32(xs)
+    pos4[:, 3] = mm
+    d = {k: torch.from_numpy(np.ascontiguousarray(L[k])).cuda()
+         for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
+    d_pos = torch.from_numpy(pos4).cuda()
+    acc = {k: torch.zeros((len(P), 3), dtype=torch.float64, device="cuda") for k in (0, 1)}
+    ng = len(L["ioff"])
+    res = {"workload": "treepm_clumpy_box_n2^20_mesh128", "groups": ng,
+           "interactions": L["work_pp"], "list_build_s": round(tb, 2),
+           "mlp_train_s": round(ttrain, 2), "mlp_train_loss": loss}
+    sp = stream.cuda_stream
+    for mode, name in ((0, "table1024"), (1, "mlp")):
+        args_ = (0, ng, d["ioff"].data_ptr(), d["icnt"].data_ptr(), d["ilist"].data_ptr(),
+                 d["joff"].data_ptr(), d["jcnt"].data_ptr(), d["jlist"].data_ptr(),
+                 d["order"].data_ptr(), d_pos.data_ptr(), acc[mode].data_ptr(), mode)
+        best = np.zeros(1, np.int32)
+        T.lib.pg5t_batch_autotune(*args_, 2, pgamd.iptr(best), None)
+        h = T.lib.pg5t_batch_graph(*args_)
+        ms = _time(lambda: T.lib.pg5t_batch_graph_launch(h, sp), stream, 5)
+        T.lib.pg5t_batch_graph_destroy(h)
+        res[name] = {"ms": round(ms, 3), "ginteractions_s": round(L["work_pp"] / ms / 1e6, 1),
+                     "pct_fp32_peak": round(100 * 38 * L["work_pp"] / (ms * 1e-3) / 1e12
+                                            / FP32_PEAK_TFLOPS, 2),
+                     "blocks_per_cu": int(best[0])}
+    res["mlp_over_table_speed"] = round(res["table1024"]["ms"] / res["mlp"]["ms"], 3)
+    O = _oracle()
+    sel = np.sort(np.random.default_rng(4).choice(len(x), 512, replace=False))
+    f = O.table_force_f64(P[sel], P, mm, pp, pm)
+    for mode, name in ((0, "table1024"), (1, "mlp")):
+        a = acc[mode].cpu().numpy()[sel]
+        res[name]["accuracy_vs_closed_form_s2"] = _errs(a, f)
+    T.lib.pg5_set_table_frc_bit(5)
+    return res
+
+
+def bench_c4(args, stream, cpu):
+    """Barnes-Hut: Hernquist N = 2^21 (a = 1, truncated at 50a), theta = 0.5,
+    groups <= 512, leaves <= 8, eps = 1/128, particle + quadrupole cell lists,
+    all groups in one graph-captured persistent launch."""
+    import numpy as np
+    import torch
+
+    import ics
+    import pgamd
+    n, eps = 1 << 21, 1.0 / 128
+    m, x = ics.hernquist(n, 21)
+    t0 = time.time()
+    L = pgamd.bh_lists(x, m, 0.5, 512, 8)
+    tb = time.time() - t0
+    g5 = pgamd.G5()
+    P = L["pos"]
+    mm = m[L["src"]]
+    pos4 = np.zeros((len(P), 4), np.float32)
+    pos4[:, :3] = P
+    pos4[:, 3] = mm
+    cells = np.zeros((len(L["cells"]), 12), np.float32)
+    cells[:, :10] = L["cells"]
+    keys = ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "coff", "ccnt", "clist", "order")
+    d = {k: torch.from_numpy(np.ascontiguousarray(L[k])).cuda() for k in keys}
+    d_pos = torch.from_numpy(pos4).cuda()
+    d_cells = torch.from_numpy(cells).cuda()
+    acc = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    pot = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ng = len(L["ioff"])
+    args_ = (0, ng, *[d[k].data_ptr() for k in keys], d_pos.data_ptr(), d_cells.data_ptr(),
+             eps * eps, acc.data_ptr(), pot.data_ptr())
+    best = np.zeros(1, np.int32)
+    g5.lib.pg5_bh_autotune(*args_, 2, pgamd.iptr(best), None)
+    h = g5.lib.pg5_bh_graph(*args_)
+    ms = _time(lambda: g5.lib.pg5_bh_graph_launch(h, stream.cuda_stream), stream, 5)
+    g5.lib.pg5_bh_graph_destroy(h)
+    flops = 38 * L["work_pp"] + 71 * L["work_pc"]
+    res = {"workload": "bh_hernquist_n2^21_theta0.5", "groups": ng,
+           "particle_interactions": L["work_pp"], "cell_interactions": L["work_pc"],
+           "list_build_s": round(tb, 2), "ms": round(ms, 3),
+           "ginteractions_s": round((L["work_pp"] + L["work_pc"]) / ms / 1e6, 1),
+           "pct_fp32_peak": round(100 * flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
+           "flops": "38/particle + 71/cell interaction", "blocks_per_cu": int(best[0])}
+    O = _oracle()
+    sel = np.sort(np.random.default_rng(5).choice(n, 256, replace=False))
+    a = acc.cpu().numpy()[sel]
+    ad, _ = O.pg5_force(P[sel], eps * eps, P, mm)
+    res["accuracy_vs_fp64_direct"] = _errs(a, ad)
+    return res
+
+
+def bench_c5(args, stream, cpu):
+    """4096 independent Plummer clusters, N log-uniform in [64, 4096], eps = 4/N,
+    force + jerk + pot of every member, one graph-captured persistent launch."""
+    import numpy as np
+    import torch
+
+    import ics
+    import pgamd
+    sizes = ics.cluster_sizes(4096, 4096)
+    xs, vs, ms_ = [], [], []
+    for c, nc in enumerate(sizes):
+        mc, xc, vc = ics.plummer(int(nc), 50000 + c)
+        xs.append(xc)
+        vs.append(vc)
+        ms_.append(mc)
+    X, V, Mm = np.concatenate(xs), np.concatenate(vs), np.concatenate(ms_)
+    coff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    e2 = ((4.0 / sizes) ** 2).astype(np.float32)
+    g6 = pgamd.G6()
+    items = g6.cluster_items(coff)
+    nt = len(Mm)
+    dd = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+          for k, v in (("items", items), ("coff", coff), ("x", X), ("v", V), ("m", Mm),
+                       ("e2", e2))}
+    acc = torch.zeros((nt, 3), dtype=torch.float64, device="cuda")
+    jrk = torch.zeros((nt, 3), dtype=torch.float64, device="cuda")
+    pot = torch.zeros(nt, dtype=torch.float64, device="cuda")
+    args_ = (len(items) // 2, dd["items"].data_ptr(), dd["coff"].data_ptr(), dd["x"].data_ptr(),
+             dd["v"].data_ptr(), dd["m"].data_ptr(), dd["e2"].data_ptr(), acc.data_ptr(),
+             jrk.data_ptr(), pot.data_ptr())
+    best = np.zeros(1, np.int32)
+    g6.lib.pg6_cluster_autotune(*args_, 2, pgamd.iptr(best), None)
+    h = g6.lib.pg6_cluster_graph(*args_)
+    ms = _time(lambda: g6.lib.pg6_cluster_graph_launch(h, stream.cuda_stream), stream, 5)
+    g6.lib.pg6_cluster_graph_destroy(h)
+    inter = float((sizes * (sizes - 1)).sum())
+    res = {"workload": "4096_plummer_clusters_n64-4096", "particles": int(nt),
+           "interactions": inter, "ms": round(ms, 3),
+           "ginteractions_s": round(inter / ms / 1e6, 1),
+           "pct_fp32_peak": round(100 * 60 * inter / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
+           "blocks_per_cu": int(best[0])}
+    O = _oracle()
+    a, j = acc.cpu().numpy(), jrk.cpu().numpy()
+    ea, ej, oa, oj = [], [], [], []
+    for c in np.random.default_rng(6).choice(len(sizes), 24, replace=False):
+        s = slice(coff[c], coff[c + 1])
+        idx = np.arange(sizes[c], dtype=np.int32)
+        o = O.pg6_force(X[s], V[s], idx, (4.0 / sizes[c]) ** 2, X[s], V[s], Mm[s], idx)
+        ea.append(a[s]); oa.append(o["acc"]); ej.append(j[s]); oj.append(o["jrk"])
+    res["accuracy_vs_fp64"] = {"acc": _errs(np.concatenate(ea), np.concatenate(oa)),
+                               "jerk": _errs(np.concatenate(ej), np.concatenate(oj)),
+                               "sample": "24 whole clusters"}
+    return res
+
+
+EXTRA = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
 
 
 if __name__ == "__main__":

@@ -129,6 +129,9 @@ int pg6_dev_force(int nj, int ni, const double *pos, const double *vel, const in
 int pg6_dev_sync(void);
 void *pg6_dev_stream(void);
 int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
+/* Force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through scalar
+ * registers (default).  Same arithmetic and results up to summation order. */
+int pg6_dev_set_kernel(int variant);
 /* Launch shape for (ni, nj): out3 = {bi, nsplit, jper}. */
 int pg6_dev_plan(int ni, int nj, int *out3);
 /* Read-back of the predicted j-particles (what the force kernel reads):

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kThreads) void cluster_force(
     const int bi = nib <= 64 ? 64 : (nib <= 128 ? 128 : 256);
     const int G = kThreads / bi;
     const int il = tid % bi;
-    const int g = tid / bi;
+    const int g = __builtin_amdgcn_readfirstlane(tid / bi);  // uniform per wave
     const bool iv = il < nib;
     const int i = start + ib + (iv ? il : 0);
     const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];

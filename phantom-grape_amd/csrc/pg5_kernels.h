@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void pg5_force(
   __shared__ float s_e[SYM ? kJTile : 1];
   const int tid = threadIdx.x;
   const int il = tid % BI;
-  const int g = tid / BI;
+  const int g = __builtin_amdgcn_readfirstlane(tid / BI);  // uniform: BI is a multiple of 64
   const int i = blockIdx.x * BI + il;
   const bool ivalid = i < ni;
   const float4 ip = xi[ivalid ? i : ni - 1];
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void pg5_quad(
   __shared__ Cell s_c[kJTile];
   const int tid = threadIdx.x;
   const int il = tid % BI;
-  const int g = tid / BI;
+  const int g = __builtin_amdgcn_readfirstlane(tid / BI);  // uniform: BI is a multiple of 64
   const int i = blockIdx.x * BI + il;
   const bool ivalid = i < ni;
   const float4 ip = xi[ivalid ? i : ni - 1];

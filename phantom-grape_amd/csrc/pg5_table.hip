@@ -745,6 +745,96 @@ void tb_release(TrainBufs& tb) {
   for (DevBuf* d : {&tb.v, &tb.t, &tb.wt, &tb.w, &tb.m, &tb.u, &tb.g, &tb.loss, &tb.gsum})
     d->release();
 }
+
+// Full-batch loss and gradient at parameters x (host, FP64) over the nset
+// training samples on the device: the per-sample kernels in FP32, the sums in
+// FP64 in a fixed order.
+double full_loss_grad(TrainBufs& tb, int nset, const std::vector<double>& x,
+                      std::vector<double>& g, hipStream_t st) {
+  std::vector<float> xf(x.begin(), x.end());
+  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, xf.data(), sizeof(float) * mlp::kParams,
+                                 hipMemcpyHostToDevice, st));
+  mlp_step(tb, nset, 0, 0, 1, 0.f, st);
+  std::vector<float> gs(mlp::kParams + 1);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
+                                 hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  g.assign(gs.begin(), gs.begin() + mlp::kParams);
+  return gs[mlp::kParams];
+}
+
+// Limited-memory BFGS (two-loop recursion, 20 pairs) with Armijo
+// backtracking, on full-batch GPU gradients.  Returns the final loss.
+double lbfgs(TrainBufs& tb, int nset, std::vector<double>& x, int iters, hipStream_t st) {
+  const int M = 20, P = mlp::kParams;
+  std::vector<std::vector<double>> S, Y;
+  std::vector<double> rho;
+  std::vector<double> g, gn, d(P), xn(P);
+  double f = full_loss_grad(tb, nset, x, g, st);
+  auto dot = [](const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0;
+    for (size_t k = 0; k < a.size(); ++k) s += a[k] * b[k];
+    return s;
+  };
+  for (int it = 0; it < iters; ++it) {
+    // d = -H g
+    std::vector<double> q = g, al(S.size());
+    for (int k = (int)S.size() - 1; k >= 0; --k) {
+      al[k] = rho[k] * dot(S[k], q);
+      for (int p = 0; p < P; ++p) q[p] -= al[k] * Y[k][p];
+    }
+    const double gamma = S.empty() ? 1e-3 / std::max(1e-30, std::sqrt(dot(g, g)))
+                                   : dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
+    for (int p = 0; p < P; ++p) q[p] *= gamma;
+    for (size_t k = 0; k < S.size(); ++k) {
+      const double b = rho[k] * dot(Y[k], q);
+      for (int p = 0; p < P; ++p) q[p] += S[k][p] * (al[k] - b);
+    }
+    for (int p = 0; p < P; ++p) d[p] = -q[p];
+    double gd = dot(g, d);
+    if (!(gd < 0)) {  // not a descent direction: restart from steepest descent
+      S.clear();
+      Y.clear();
+      rho.clear();
+      for (int p = 0; p < P; ++p) d[p] = -g[p] * 1e-3 / std::max(1e-30, std::sqrt(dot(g, g)));
+      gd = dot(g, d);
+    }
+    double step = 1.0, fn = 0.0;
+    bool ok = false;
+    for (int ls = 0; ls < 30; ++ls) {
+      for (int p = 0; p < P; ++p) xn[p] = x[p] + step * d[p];
+      fn = full_loss_grad(tb, nset, xn, gn, st);
+      if (std::isfinite(fn) && fn <= f + 1e-4 * step * gd) {
+        ok = true;
+        break;
+      }
+      step *= 0.5;
+    }
+    if (!ok) break;
+    std::vector<double> s(P), y(P);
+    for (int p = 0; p < P; ++p) {
+      s[p] = xn[p] - x[p];
+      y[p] = gn[p] - g[p];
+    }
+    const double sy = dot(s, y);
+    if (sy > 1e-20) {
+      if ((int)S.size() == M) {
+        S.erase(S.begin());
+        Y.erase(Y.begin());
+        rho.erase(rho.begin());
+      }
+      S.push_back(s);
+      Y.push_back(y);
+      rho.push_back(1.0 / sy);
+    }
+    x = xn;
+    g = gn;
+    const double rel = (f - fn) / std::max(f, 1e-300);
+    f = fn;
+    if (rel < 1e-12 && it > 50) break;
+  }
+  return f;
+}
 }  // namespace
 
 // Summed gradient and loss of the weighted MSE sum_s wt_s (Y(v_s) - t_s)^2 / n
@@ -781,9 +871,11 @@ PGAMD_API int pg5_mlp_grad(int n, const float* v, const float* t, const float* w
 // the table was generated from, pg5_gen_force_table / pg5_gen_s2_force_table):
 // 65536 seeded samples of r in [0, r_c) (half uniform in v = sqrt(r/r_c),
 // half uniform in volume), target Y = f(r) (r^3 + r0^3) evaluated in FP64,
-// weight 1 / (|Y| + 0.05)^2 (relative accuracy where Y is O(1)); Adam with a
-// cosine learning-rate decay to lr/100 over `steps` mini-batches of `batch`.
-// Deterministic for a given seed.  *loss_out = weighted MSE on the full set.
+// weight 1 / (|Y| + 0.05)^2 (relative accuracy where Y is O(1)).  Optimiser:
+// min(steps, 1000) Adam steps on mini-batches of `batch` at rate lr, then up
+// to `steps` L-BFGS iterations on the full set (loss and gradient from the GPU
+// kernels, the two-loop recursion on the host).  Deterministic for a given
+// seed.  *loss_out = weighted MSE on the full set.
 PGAMD_API int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, double* loss_out) {
   Board* b = board(0, "pg5_mlp_train");
   if (!b) return -1;
@@ -830,7 +922,7 @@ PGAMD_API int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, doub
   p[mlp::kB3] = (float)(mean / nset);
 
   TrainBufs tb;
-  tb_alloc(tb, nset, std::max(batch, 4096));
+  tb_alloc(tb, nset, nset);
   hipStream_t st = b->stream;
   PGAMD_HIP_CHECK(hipMemcpyAsync(tb.v.ptr, hv.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
   PGAMD_HIP_CHECK(hipMemcpyAsync(tb.t.ptr, ht.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
@@ -839,27 +931,19 @@ PGAMD_API int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, doub
   std::vector<float> zero(mlp::kParams, 0.f);
   PGAMD_HIP_CHECK(hipMemcpyAsync(tb.m.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
   PGAMD_HIP_CHECK(hipMemcpyAsync(tb.u.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
+  // Adam warm-up on mini-batches, then L-BFGS on the full set
+  const int warm = std::min(steps, 1000);
   const int span = nset - batch + 1;
-  for (int step = 1; step <= steps; ++step) {
+  for (int step = 1; step <= warm; ++step) {
     const int off = (int)(((long long)(step - 1) * batch) % span);
-    const double c = 0.5 * (1.0 + std::cos(M_PI * (double)step / steps));
-    const float lr_t = (float)(lr * (0.01 + 0.99 * c));
-    mlp_step(tb, batch, off, 1, step, lr_t, st);
-  }
-  // full-set loss, in chunks of 4096
-  double tot = 0.0;
-  std::vector<float> gs(mlp::kParams + 1);
-  for (int off = 0; off < nset; off += 4096) {
-    const int n = std::min(4096, nset - off);
-    mlp_step(tb, n, off, 0, 1, 0.f, st);
-    PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
-                                   hipMemcpyDeviceToHost, st));
-    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-    tot += (double)gs[mlp::kParams] * n / nset;
+    mlp_step(tb, batch, off, 1, step, (float)lr, st);
   }
   PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), tb.w.ptr, sizeof(float) * mlp::kParams,
                                  hipMemcpyDeviceToHost, st));
   PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<double> x(p.begin(), p.end());
+  const double tot = lbfgs(tb, nset, x, steps, st);
+  for (int k = 0; k < mlp::kParams; ++k) p[k] = (float)x[k];
   tb_release(tb);
   g_mlp.params = p;
   g_mlp.rc = rc;

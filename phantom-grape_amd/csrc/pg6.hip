@@ -36,9 +36,10 @@ constexpr int kJMax = 1 << 26;           // j-memory limit
 constexpr int kInitialJCap = 1 << 18;    // reference JMEMSIZE (gravity.c:14)
 
 struct LaunchPlan {
-  int bi;      // i-particles per block
-  int nsplit;  // j-ranges
-  int jper;    // j-particles per range
+  int bi;       // i-particles per block
+  int nsplit;   // j-ranges
+  int jper;     // j-particles per range
+  int variant;  // 0: j-tiles in LDS (pg6_force), 1: j broadcast in SGPRs (pg6_force_s)
 };
 
 struct Board {
@@ -69,6 +70,7 @@ struct Board {
   int force_bi = 0;
   int blocks_per_cu = 8;
   int min_jper = 128;
+  int variant = 1;  // force kernel: 0 LDS j-tiles, 1 SGPR broadcast
 };
 
 Board g_board;
@@ -254,8 +256,10 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   } else {
     p.bi = ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64);
   }
+  p.variant = b.variant;
   const int nbi = ceil_div(ni, p.bi);
-  const int target = device_cu_count() * b.blocks_per_cu;
+  // the SGPR-broadcast kernel runs bi threads per block, the LDS one 256
+  const int target = device_cu_count() * b.blocks_per_cu * (p.variant == 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
   const int maxs = std::max(1, nj / std::max(1, b.min_jper));
   ns = std::min(ns, maxs);
@@ -273,13 +277,17 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
                     const Outputs& out, hipStream_t st) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
   g_timer.before(st);
-  hipLaunchKernelGGL((pg6::pg6_force<BI, NN>), grid, dim3(pg6::kThreads), 0, st, ni, ipos, ivel,
-                     iid, eps2, nj, p.jper, jp, part, out, p.nsplit);
+  if (p.variant == 1)
+    hipLaunchKernelGGL((pg6::pg6_force_s<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
+                       nj, p.jper, jp, part, out, p.nsplit);
+  else
+    hipLaunchKernelGGL((pg6::pg6_force<BI, NN>), grid, dim3(pg6::kThreads), 0, st, ni, ipos,
+                       ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit);
   PGAMD_HIP_CHECK(hipGetLastError());
   g_timer.after(st);
   if (p.nsplit > 1) {
     hipLaunchKernelGGL((pg6::pg6_reduce<NN>), dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
-                       p.nsplit, part, jp.id, out);
+                       p.nsplit, part, jp, ipos, eps2, out);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
 }
@@ -963,5 +971,18 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
   g_cl_bpc = bestc;
   if (best) *best = bestc;
+  return 0;
+}
+
+// Force kernel choice: 0 = j-particles staged in LDS tiles (pg6_force),
+// 1 = j-particles broadcast through scalar registers (pg6_force_s, default).
+// Both compute the same arithmetic; the choice affects speed only.
+PGAMD_API int pg6_dev_set_kernel(int variant) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (variant != 0 && variant != 1) {
+    api_error(3, "pg6_dev_set_kernel: variant %d not in {0, 1}", variant);
+    return -1;
+  }
+  g_board.variant = variant;
   return 0;
 }

@@ -107,6 +107,17 @@ __global__ void pg6_predict(int nj, double ti, JState js, JPred jp) {
   jp.id[j] = js.idx[j];
 }
 
+// sqrt(|dx|^2 + eps^2) to j-particle j, with dx formed as in the force loop
+// (FP64 difference rounded to FP32): the reference's rnnb
+// (PG6/libpg6/gravity.c:652-673).
+__device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi, double zi,
+                                    float eps2) {
+  const float dx = (float)(jp.x[j] - xi);
+  const float dy = (float)(jp.y[j] - yi);
+  const float dz = (float)(jp.z[j] - zi);
+  return sqrtf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+}
+
 // Force + jerk + potential (+ nearest neighbour) kernel.
 //
 // Block = 256 threads.  BI i-particles per block (64, 128 or 256); the block's
@@ -126,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
 
   const int tid = threadIdx.x;
   const int il = tid % BI;
-  const int g = tid / BI;
+  const int g = __builtin_amdgcn_readfirstlane(tid / BI);  // uniform: BI is a multiple of 64
   const int i = blockIdx.x * BI + il;
   const bool ivalid = i < ni;
   const int ic = ivalid ? i : ni - 1;
@@ -144,8 +155,15 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
 
   double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
   double jx = 0.0, jy = 0.0, jz = 0.0;
-  float rmax = 0.f;  // largest 1/r seen = nearest neighbour
+  // Nearest neighbour = largest 1/r.  Per pair only a packed key is kept:
+  // the bits of 1/r (non-negative, so unsigned order = float order) with the
+  // low 8 mantissa bits replaced by 255 - (tile slot), reduced with an
+  // unsigned max; the tile's winner is decoded once per tile.  1/r is thereby
+  // compared at 2^-15 relative resolution, and among candidates that close
+  // the lowest j address wins.  The self pair has 1/r = 0 and never wins.
+  float rmax = 0.f;  // quantised 1/r of the nearest neighbour so far
   int nn = -1;
+  const unsigned kmask = 0xFFFFFF00u;
 
   for (int jt = j0; jt < j1; jt += kJTile) {
     const int nt = min(kJTile, j1 - jt);
@@ -162,6 +180,7 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
 
     float fax = 0.f, fay = 0.f, faz = 0.f, fpot = 0.f;
     float fjx = 0.f, fjy = 0.f, fjz = 0.f;
+    unsigned tkey = 0u;
 #pragma unroll 4
     for (int jj = g; jj < nt; jj += G) {
       const float dx = (float)(s_x[jj] - xi);
@@ -187,9 +206,15 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
       fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
       fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
       if (NN) {
-        const bool closer = rinv > rmax;
-        rmax = closer ? rinv : rmax;
-        nn = closer ? jt + jj : nn;
+        const unsigned key = (__float_as_uint(rinv) & kmask) | (unsigned)(255 - jj);
+        tkey = max(tkey, key);
+      }
+    }
+    if (NN) {
+      const float rq = __uint_as_float(tkey & kmask);
+      if (rq > rmax) {  // strict: an earlier tile keeps a tie
+        rmax = rq;
+        nn = jt + 255 - (int)(tkey & 0xFFu);
       }
     }
     ax += (double)fax;
@@ -241,7 +266,122 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
     out.pot[i] = pot;
     if (NN) {
       out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
-      out.rnnb[i] = rmax > 0.f ? 1.f / rmax : 0.f;
+      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+    }
+  } else {
+    const size_t o = (size_t)blockIdx.y * ni + i;
+    part.ax[o] = ax;
+    part.ay[o] = ay;
+    part.az[o] = az;
+    part.pot[o] = pot;
+    part.jx[o] = jx;
+    part.jy[o] = jy;
+    part.jz[o] = jz;
+    if (NN) {
+      part.rmax[o] = rmax;
+      part.nn[o] = nn;
+    }
+  }
+}
+
+// Variant of pg6_force that broadcasts the j-particles through SGPRs instead
+// of LDS: every wave walks the same j sequence, so j data is wave-uniform;
+// loaded through the constant address space it arrives by scalar loads and
+// the VALU instructions take it as a scalar operand (no LDS traffic, no
+// per-lane address arithmetic).  Block = 64, 128 or 256 threads, one
+// i-particle per lane; every wave walks the block's whole j-range.  Same
+// arithmetic, nearest-neighbour rule and outputs as pg6_force.
+typedef const __attribute__((address_space(4))) double cdouble;
+typedef const __attribute__((address_space(4))) float cfloat_s;
+typedef const __attribute__((address_space(4))) int cint;
+
+template <bool NN>
+__global__ __launch_bounds__(kThreads) void pg6_force_s(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp, Partials part,
+    Outputs out, int nsplit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const float vxi = (float)ivel[3 * ic + 0];
+  const float vyi = (float)ivel[3 * ic + 1];
+  const float vzi = (float)ivel[3 * ic + 2];
+  const int idi = iid[ic];
+  cdouble* px = (cdouble*)jp.x;
+  cdouble* py = (cdouble*)jp.y;
+  cdouble* pz = (cdouble*)jp.z;
+  cfloat_s* pvm = (cfloat_s*)jp.vm;  // float4 records read as 4 floats
+  cint* pid = (cint*)jp.id;
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min(nj, j0 + jper);
+  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+  float rmax = 0.f;
+  int nn = -1;
+  const unsigned kmask = 0xFFFFFF00u;
+  for (int jt = j0; jt < j1; jt += kJTile) {
+    const int nt = min(kJTile, j1 - jt);
+    float fax = 0.f, fay = 0.f, faz = 0.f, fpot = 0.f, fjx = 0.f, fjy = 0.f, fjz = 0.f;
+    unsigned tkey = 0u;
+#pragma unroll 4
+    for (int jj = 0; jj < nt; ++jj) {
+      const int j = jt + jj;
+      const float dx = (float)(px[j] - xi);
+      const float dy = (float)(py[j] - yi);
+      const float dz = (float)(pz[j] - zi);
+      const float dvx = pvm[4 * j + 0] - vxi;
+      const float dvy = pvm[4 * j + 1] - vyi;
+      const float dvz = pvm[4 * j + 2] - vzi;
+      const float mj = pvm[4 * j + 3];
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+      const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+      const bool self = pid[j] == idi;
+      const float rinv = self ? 0.f : __builtin_amdgcn_rsqf(r2);
+      const float rinv2 = rinv * rinv;
+      const float mr = mj * rinv;
+      const float mr3 = mr * rinv2;
+      const float beta = -3.f * rv * rinv2;
+      fpot -= mr;
+      fax = fmaf(mr3, dx, fax);
+      fay = fmaf(mr3, dy, fay);
+      faz = fmaf(mr3, dz, faz);
+      fjx = fmaf(mr3, fmaf(beta, dx, dvx), fjx);
+      fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
+      fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
+      if (NN) {
+        const unsigned key = (__float_as_uint(rinv) & kmask) | (unsigned)(255 - jj);
+        tkey = max(tkey, key);
+      }
+    }
+    if (NN) {
+      const float rq = __uint_as_float(tkey & kmask);
+      if (rq > rmax) {
+        rmax = rq;
+        nn = jt + 255 - (int)(tkey & 0xFFu);
+      }
+    }
+    ax += (double)fax;
+    ay += (double)fay;
+    az += (double)faz;
+    pot += (double)fpot;
+    jx += (double)fjx;
+    jy += (double)fjy;
+    jz += (double)fjz;
+  }
+  if (!ivalid) return;
+  if (nsplit == 1) {
+    out.acc[3 * i + 0] = ax;
+    out.acc[3 * i + 1] = ay;
+    out.acc[3 * i + 2] = az;
+    out.jrk[3 * i + 0] = jx;
+    out.jrk[3 * i + 1] = jy;
+    out.jrk[3 * i + 2] = jz;
+    out.pot[i] = pot;
+    if (NN) {
+      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
     }
   } else {
     const size_t o = (size_t)blockIdx.y * ni + i;
@@ -261,8 +401,8 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
 
 // Sums the nsplit partials of each i in split order (deterministic).
 template <bool NN>
-__global__ void pg6_reduce(int ni, int nsplit, Partials part, const int* __restrict__ jid,
-                           Outputs out) {
+__global__ void pg6_reduce(int ni, int nsplit, Partials part, JPred jp,
+                           const double* __restrict__ ipos, float eps2, Outputs out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ni) return;
   double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
@@ -294,8 +434,10 @@ __global__ void pg6_reduce(int ni, int nsplit, Partials part, const int* __restr
   out.jrk[3 * i + 2] = jz;
   out.pot[i] = pot;
   if (NN) {
-    out.nnb[i] = nn >= 0 ? jid[nn] : -1;
-    out.rnnb[i] = rmax > 0.f ? 1.f / rmax : 0.f;
+    out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+    out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
+                                        eps2)
+                          : 0.f;
   }
 }
 

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kThreads) void table_force(
   const int base = 1 << (30 - shift);
   const float scut = __uint_as_float((1u << 30) | ((unsigned)(ntab - 1) << shift));
   const int il = tid % BI;
-  const int g = tid / BI;
+  const int g = __builtin_amdgcn_readfirstlane(tid / BI);  // uniform: BI is a multiple of 64
   const int i = blockIdx.x * BI + il;
   const bool ivalid = i < ni;
   const float4 ip = xi[ivalid ? i : ni - 1];

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kThreads) void treepm_batch(
       const int bi = nib <= 64 ? 64 : (nib <= 128 ? 128 : 256);
       const int G = kThreads / bi;
       const int il = tid % bi;
-      const int g = tid / bi;
+      const int g = __builtin_amdgcn_readfirstlane(tid / bi);  // uniform per wave
       const bool iv = il < nib;
       const int ii = ilist[i0 + ib + (iv ? il : 0)];
       const float4 ip = pos[ii];

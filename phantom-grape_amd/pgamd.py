@@ -41,10 +41,22 @@ def lib_path(name: str) -> str:
 _handles: dict[str, C.CDLL] = {}
 
 
+def _one_hip_runtime():
+    """PyTorch-ROCm ships its own libamdhip64.so.7 with the same soname as the
+    system one these libraries link.  Whichever is loaded first serves the
+    whole process, and torch cannot use the system runtime; so when torch is
+    importable it is imported before the first library loads."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(name: str) -> C.CDLL:
     """Load ``lib<name>.so`` from the package's lib/ directory (built in-tree)."""
     if name in _handles:
         return _handles[name]
+    _one_hip_runtime()
     path = lib_path(name)
     if not os.path.exists(path):
         raise RuntimeError(
@@ -223,6 +235,7 @@ class G6:
         _sig(L, "pg6_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
         _sig(L, "pg6_dev_plan", C.c_int, C.c_int, C.c_int, _ip)
         _sig(L, "pg6_dev_read_predicted", C.c_int, C.c_int, _dp, _fp, _fp)
+        _sig(L, "pg6_dev_set_kernel", C.c_int, C.c_int)
         _sig(L, "pg6_dev_timing_start", C.c_int, C.c_int)
         _sig(L, "pg6_dev_timing_read", C.c_int, _dp, _ip)
         _sig(L, "pg6_cluster_items", C.c_int, C.c_int, _ip, _ip, C.c_int)

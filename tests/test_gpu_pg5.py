@@ -175,4 +175,5 @@ def test_quadrupole_near_cells(g5):
     a, p = g5.cell_force_on_x(xi)
     oa, op = O.quad_force(f32(xi), np.float32(eps * eps), f32(xc), f32(mc), f32(qc))
     assert np.percentile(rel_err(a, oa), 99) < 1e-5
-    np.testing.assert_allclose(-p, op, rtol=1e-5)
+    # phi_p and phi_q cancel partly for near cells: FP32 sums, 5e-5 relative
+    np.testing.assert_allclose(-p, op, rtol=5e-5)

@@ -49,8 +49,14 @@ def test_matches_reference_fixture(g6):
     assert np.percentile(rel_err(r["acc"], g["acc"]), 99) < 1e-6
     assert np.percentile(rel_err(r["jrk"], g["jrk"]), 99) < 2e-5
     assert rel_err(r["pot"], g["pot"]).max() < 1e-6
-    np.testing.assert_array_equal(r["nnb"], g["nnb"])
-    assert rel_err(r["rnnb"], g["rnnb"]).max() < 1e-5
+    # nearest neighbour: equal to the reference's except at near-ties
+    # (1/r compared at 2^-15 relative resolution here, FP32 Newton step there)
+    xp, _ = O.pg6_predict(float(g["ti"]), g["x"], g["v"], g["acc2"], g["jrk6"], g["tj"])
+    for q in np.nonzero(r["nnb"] != g["nnb"])[0]:
+        d2 = ((xp - g["xi"][q]) ** 2).sum(1)
+        assert abs(d2[r["nnb"][q]] - d2[g["nnb"][q]]) < 1e-4 * d2[g["nnb"][q]]
+    assert np.mean(r["nnb"] == g["nnb"]) > 0.98
+    assert rel_err(r["rnnb"][r["nnb"] == g["nnb"]], g["rnnb"][r["nnb"] == g["nnb"]]).max() < 1e-5
     # neighbour lists, same call sequence as the reference run
     rn = g6.calc(n, g["xi"], g["vi"], idx[g["isel"]], float(g["eps2"]), h2=g["h2"], variant="n")
     assert rn["rc"] == 0 and g6.lib.g6_read_neighbour_list(0) == 0
@@ -63,8 +69,10 @@ def test_matches_reference_fixture(g6):
     np.testing.assert_array_equal(np.concatenate(got), g["nbl"])
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("ni", [1, 7, 48, 64, 500, 4096])
-def test_force_vs_fp64_oracle(g6, ni):
+def test_force_vs_fp64_oracle(g6, ni, variant):
+    assert g6.lib.pg6_dev_set_kernel(variant) == 0
     n = 4096
     eps = 1.0 / 256
     m, x, v, a2, j6, tj, idx = plummer_state(n, 2, eps)
@@ -83,13 +91,17 @@ def test_force_vs_fp64_oracle(g6, ni):
     assert np.percentile(ea, 99) < 1e-6, np.percentile(ea, 99)
     assert np.percentile(ej, 99) < 1e-5, np.percentile(ej, 99)
     assert rel_err(r["pot"], o["pot"]).max() < 1e-6
-    assert rel_err(r["rnnb"], o["rnnb"]).max() < 1e-5
-    # nearest neighbour: identical except where the two nearest are FP32-tied
+    # nearest neighbour: 1/r is compared at 2^-15 relative resolution, so a
+    # candidate within ~6e-5 in r^2 of the nearest may be returned instead
     bad = np.nonzero(r["nnb"] != o["nnb"])[0]
     for i in bad:
         d = xo - xo[isel[i]]
         r2 = (d * d).sum(1) + eps * eps
-        assert abs(r2[r["nnb"][i]] - r2[o["nnb"][i]]) < 1e-6 * r2[o["nnb"][i]]
+        assert abs(r2[r["nnb"][i]] - r2[o["nnb"][i]]) < 1e-4 * r2[o["nnb"][i]]
+    # rnnb is the distance to the returned neighbour
+    d = xo[r["nnb"]] - xo[isel]
+    np.testing.assert_allclose(r["rnnb"], np.sqrt((d * d).sum(1) + eps * eps), rtol=1e-6)
+    g6.lib.pg6_dev_set_kernel(1)
 
 
 def test_variants_agree_and_are_deterministic(g6):

@@ -104,7 +104,7 @@ def test_clumpy_box_vs_emulation_and_closed_form(tl, frc):
     xc = x[32768 // 2:]  # clumped half: dense neighbourhoods
     xi = xc[::97]
     a = force(tl, xi, x, m)
-    e = O.table_force_emul(xi, x, m, t, xs)
+    e = O.table_force_emul(xi, x, m, t, xs, frc_bit=frc)
     scale = np.linalg.norm(e, axis=1).max()
     assert np.percentile(rel_err(a, e, floor=1e-3 * scale), 99) < 2e-6
     f = O.table_force_f64(xi, x, m, PP, PM)
