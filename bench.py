@@ -85,6 +85,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", type=int, default=-1,
                     help="PG6 force kernel: 0 LDS j-tiles, 1 SGPR broadcast (default: library's)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the library's default launch shapes instead of autotuning them")
+    ap.add_argument("--launch", default="",
+                    help="fixed PG6 launch 'variant,bi,blocks_per_cu' (no autotuning), e.g. 1,256,16")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the i-batch size sweep")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--configs", default="c1,c3,c4,c5",
                     help="extra configurations to measure (comma list, '' for none)")
     args = ap.parse_args()
@@ -96,6 +104,15 @@ def main():
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
+        return
+
     import ics
     import pgamd
 
@@ -105,6 +122,10 @@ def main():
     L = g6.lib
     if args.kernel >= 0:
         assert L.pg6_dev_set_kernel(args.kernel) == 0
+    if args.launch:
+        lv, lbi, lbpc = (int(t) for t in args.launch.split(","))
+        assert L.pg6_dev_set_kernel(lv) == 0 and L.pg6_dev_set_launch(lbi, lbpc, 0) == 0
+        args.no_autotune = True
     n = N
     m, x, v = ics.plummer(n, SEED)
     idx = np.arange(n, dtype=np.int32)
@@ -161,8 +182,16 @@ def main():
         assert L.pg6_dev_predict(n, TBLOCK, sp) == 0
         force(n, d_xi, d_vi, d_id, out)
 
+    tuned = None
+    if not args.no_autotune:
+        # untimed: pick kernel and launch shape per i-count on these j-particles
+        assert L.pg6_dev_predict(n, TBLOCK, sp) == 0
+        t0 = time.time()
+        tuned = g6.autotune(n, EPS * EPS)
+        log(f"autotune ({time.time() - t0:.1f} s): " + ", ".join(
+            f"{ni}:v{c[0]}/bi{c[1]}/bpc{c[2]}" for ni, c in tuned.items()))
     plan = g6.plan(n, n)
-    log(f"launch plan ni=nj={n}: bi={plan[0]} nsplit={plan[1]} jper={plan[2]}")
+    log(f"launch plan ni=nj={n}: {plan}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -193,7 +222,7 @@ def main():
     # i-batch sweep (block-step sizes) against all j
     sweep = []
     rng = np.random.default_rng(SEED + 1)
-    for ni in [64 << k for k in range(11)]:
+    for ni in ([] if args.no_sweep else [64 << k for k in range(11)]):
         sel = np.sort(rng.choice(n, ni, replace=False))
         ip = d_xi[torch.from_numpy(sel).to(dev)].contiguous()
         iv = d_vi[torch.from_numpy(sel).to(dev)].contiguous()
@@ -218,7 +247,7 @@ def main():
                           ginteractions_s=round(ni * (n - 1) / (t * 1e-3) / 1e9, 2),
                           kernel_pct_fp32_peak=round(100 * FLOPS_PER_INTERACTION * ni * (n - 1)
                                                      / (tk * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
-                          plan=dict(bi=p[0], nsplit=p[1], jper=p[2])))
+                          plan=p))
         log(f"sweep ni={ni}: {sweep[-1]['ginteractions_s']} Gint/s, kernel "
             f"{sweep[-1]['kernel_pct_fp32_peak']}% of FP32 peak")
 
@@ -238,7 +267,8 @@ def main():
         "config": {"workload": "pg6_hermite_plummer", "n": n, "eps": EPS,
                    "i_batch": n, "j": n, "block_levels": "2^-3..2^-12",
                    "step": "predict all j + force/jerk/pot/nn of all i",
-                   "parallelism": "single GPU, j-range split"},
+                   "parallelism": "single GPU, j-range split",
+                   "launch": plan, "autotuned": tuned is not None},
         "pct_fp32_peak": round(100 * achieved / FP32_PEAK_TFLOPS, 2),
         "roofline": {"bound": "fp32-valu", "kernel": "pg6_force",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
@@ -349,6 +379,15 @@ def bench_c1(args, stream, cpu):
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
+        return
+
     import ics
     import pgamd
     n, eps = 16384, 1.0 / 64
@@ -400,6 +439,15 @@ def bench_c3(args, stream, cpu):
     all groups in one graph-captured persistent launch."""
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
+        return
 
     import ics
     import pgamd
@@ -460,6 +508,15 @@ def bench_c4(args, stream, cpu):
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
+        return
+
     import ics
     import pgamd
     n, eps = 1 << 21, 1.0 / 128
@@ -509,6 +566,15 @@ def bench_c5(args, stream, cpu):
     force + jerk + pot of every member, one graph-captured persistent launch."""
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
+        return
 
     import ics
     import pgamd

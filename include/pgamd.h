@@ -128,12 +128,24 @@ int pg6_dev_force(int nj, int ni, const double *pos, const double *vel, const in
                   void *stream);
 int pg6_dev_sync(void);
 void *pg6_dev_stream(void);
+/* Launch-shape overrides: bi in {0 = auto, 64, 128, 256}; blocks_per_cu
+ * (0 = auto) sets how many j-ranges the j-set is split into; min_jper the
+ * smallest j-range (0 = 128). */
 int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 /* Force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through scalar
- * registers (default).  Same arithmetic and results up to summation order. */
+ * registers, -1 = automatic (autotuned or default per i-count).  Same
+ * arithmetic and results up to summation order. */
 int pg6_dev_set_kernel(int variant);
-/* Launch shape for (ni, nj): out3 = {bi, nsplit, jper}. */
-int pg6_dev_plan(int ni, int nj, int *out3);
+/* Launch shape for (ni, nj): out5 = {variant, bi, blocks_per_cu, nsplit,
+ * jper}. */
+int pg6_dev_plan(int ni, int nj, int *out5);
+/* Autotuner: times every (kernel, bi, blocks_per_cu) candidate for
+ * ni = 64, 128, ..., min(nj, 65536) on the first nj predicted j-particles and
+ * keeps the fastest per i-count bucket.  out[4k..4k+3] = {variant, bi,
+ * blocks_per_cu, microseconds} for bucket k (may be NULL).  Returns the
+ * number of buckets tuned, or -1.  No counterpart in the reference. */
+int pg6_dev_autotune(int nj, double eps2, int reps, int *out);
+int pg6_dev_autotune_reset(void);
 /* Read-back of the predicted j-particles (what the force kernel reads):
  * pos[nj][3] FP64, vel[nj][3] FP32, mass[nj] FP32 (may be NULL).  The
  * reference keeps them in its static Pred_Mem (PG6/libpg6/gravity.c:42-47). */

@@ -17,6 +17,15 @@
 // Training kernels: mlp_sample_grad writes each sample's weighted-MSE
 // gradient (transposed, [param][sample]); mlp_reduce_adam sums them in a
 // fixed order (deterministic) and applies one Adam step.
+//
+// Inference form.  Since tanh(z) = 2 s(c z) - 1 with s(u) = 1 / (1 + 2^-u)
+// and c = 2 log2(e), the same network is evaluated with the cheaper
+// activation s (one exp2, one add, one reciprocal) after an exact affine
+// change of parameters, done once on the host in FP64 (to_inference_form):
+//   layer 1: u1 = (2 c w1) v + c (b1 - w1)     (input v, not x = 2 v - 1)
+//   layer 2: u2 = (2 c W2) s1 + c (b2 - sum_j W2[k][j])
+//   output:  y  = (2 w3) . s2 + (b3 - sum_k w3[k])
+// stored at the same offsets as the tanh-form parameters.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -64,6 +73,41 @@ __device__ inline float eval(cfloat* w, float v) {
     y = fmaf(w[kW3 + k], tanh_fast(a), y);
   }
   return y;
+}
+
+// Logistic activation of the inference form: 1 / (1 + 2^-u).
+__device__ inline float sig2(float u) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-u));
+}
+
+// Inference-form forward pass for K inputs at once: every parameter is loaded
+// once (scalar loads, uniform across the wavefront) and used for K pairs.
+template <int K>
+__device__ inline void eval_k(cfloat* w, const float (&v)[K], float (&y)[K]) {
+  float s1[K][kH];
+#pragma unroll
+  for (int j = 0; j < kH; ++j) {
+    const float a = w[kW1 + j], b = w[kB1 + j];
+#pragma unroll
+    for (int u = 0; u < K; ++u) s1[u][j] = sig2(fmaf(a, v[u], b));
+  }
+#pragma unroll
+  for (int u = 0; u < K; ++u) y[u] = w[kB3];
+#pragma unroll
+  for (int k = 0; k < kH; ++k) {
+    float z[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) z[u] = w[kB2 + k];
+#pragma unroll
+    for (int j = 0; j < kH; ++j) {
+      const float wkj = w[kW2 + kH * k + j];
+#pragma unroll
+      for (int u = 0; u < K; ++u) z[u] = fmaf(wkj, s1[u][j], z[u]);
+    }
+    const float w3 = w[kW3 + k];
+#pragma unroll
+    for (int u = 0; u < K; ++u) y[u] = fmaf(w3, sig2(z[u]), y[u]);
+  }
 }
 
 // Per-sample gradient of wt * (y - t)^2 (divided by n), transposed output
@@ -151,15 +195,38 @@ __global__ __launch_bounds__(256) void mlp_reduce_adam(int n, const float* __res
 }
 
 // Evaluates the learned f(r) at n physical radii (for tests and error
-// measurement): f = Y(sqrt(r/rc)) / (r^3 + r0^3), 0 for r >= rc.
-__global__ void mlp_eval_f(int n, const float* __restrict__ r, const float* __restrict__ w,
+// measurement) with the inference-form parameters `wi`, as the force kernel
+// does: f = Y(sqrt(r/rc)) / (r^3 + r0^3), 0 for r >= rc.
+__global__ void mlp_eval_f(int n, const float* __restrict__ r, const float* __restrict__ wi,
                            float rc, float r03, float* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float rr = r[k];
-  const float v = sqrtf(fminf(rr / rc, 1.f));
-  const float y = eval(fresh(as_const(w)), v);
-  out[k] = rr < rc ? y / (rr * rr * rr + r03) : 0.f;
+  float v[1] = {sqrtf(fminf(rr / rc, 1.f))};
+  float y[1];
+  eval_k<1>(fresh(as_const(wi)), v, y);
+  out[k] = rr < rc ? y[0] / (rr * rr * rr + r03) : 0.f;
+}
+
+// Host: tanh-form parameters -> inference form (see above), in FP64.
+inline void to_inference_form(const float* p, float* q) {
+  const double c = 2.0 / 0.69314718055994530942;  // 2 log2(e)
+  for (int j = 0; j < kH; ++j) {
+    q[kW1 + j] = (float)(2.0 * c * p[kW1 + j]);
+    q[kB1 + j] = (float)(c * ((double)p[kB1 + j] - p[kW1 + j]));
+  }
+  double sw3 = 0.0;
+  for (int k = 0; k < kH; ++k) {
+    double sw = 0.0;
+    for (int j = 0; j < kH; ++j) {
+      q[kW2 + kH * k + j] = (float)(2.0 * c * p[kW2 + kH * k + j]);
+      sw += p[kW2 + kH * k + j];
+    }
+    q[kB2 + k] = (float)(c * ((double)p[kB2 + k] - sw));
+    q[kW3 + k] = (float)(2.0 * p[kW3 + k]);
+    sw3 += p[kW3 + k];
+  }
+  q[kB3] = (float)((double)p[kB3] - sw3);
 }
 
 }  // namespace mlp

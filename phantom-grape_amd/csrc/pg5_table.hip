@@ -341,11 +341,15 @@ bool check_range(const char* fn, int adr, int nj) {
 }
 
 
+// Device copy of the parameters in inference form (mlp_kernels.h), which the
+// force and evaluation kernels read.
 const float* mlp_dev(hipStream_t st) {
   Mlp& m = g_mlp;
   if (m.dirty) {
+    float q[mlp::kParams];
+    mlp::to_inference_form(m.params.data(), q);
     m.dev.reserve(sizeof(float) * mlp::kParams);
-    PGAMD_HIP_CHECK(hipMemcpyAsync(m.dev.ptr, m.params.data(), sizeof(float) * mlp::kParams,
+    PGAMD_HIP_CHECK(hipMemcpyAsync(m.dev.ptr, q, sizeof(float) * mlp::kParams,
                                    hipMemcpyHostToDevice, st));
     PGAMD_HIP_CHECK(hipStreamSynchronize(st));
     m.dirty = false;

@@ -40,7 +40,31 @@ struct LaunchPlan {
   int nsplit;   // j-ranges
   int jper;     // j-particles per range
   int variant;  // 0: j-tiles in LDS (pg6_force), 1: j broadcast in SGPRs (pg6_force_s)
+  int bpc;      // target blocks (of 256 threads) per CU
 };
+
+// Launch choice for one i-count bucket; bucket k holds ni <= 64 << k (the
+// last bucket also every larger ni).
+struct LaunchChoice {
+  int variant, bi, bpc;
+};
+constexpr int kBuckets = 11;
+
+inline int ni_bucket(int ni) {
+  int k = 0;
+  while (k < kBuckets - 1 && ni > (64 << k)) ++k;
+  return k;
+}
+
+// Default choice, from a sweep at nj = 65536 on MI355X: the LDS-tile kernel
+// for small and medium i-sets, the scalar-broadcast kernel for large ones,
+// and many j-ranges (16 blocks of 256 threads per CU) so that the last wave
+// of blocks is a small fraction of the launch.
+inline LaunchChoice default_choice(int ni) {
+  if (ni <= 512) return {0, 64, 8};
+  if (ni <= 8192) return {0, 128, 16};
+  return {1, 256, 16};
+}
 
 struct Board {
   bool open = false;
@@ -66,11 +90,14 @@ struct Board {
   std::vector<int> nbl, nblen;
   int nb_ni = 0;
   int nblerror = 0;
-  // launch tuning overrides (0 = automatic)
+  // launch tuning: explicit overrides (0 / -1 = automatic), then the
+  // autotuned choice of the ni bucket, then the default
   int force_bi = 0;
-  int blocks_per_cu = 8;
+  int force_bpc = 0;
+  int force_variant = -1;
   int min_jper = 128;
-  int variant = 1;  // force kernel: 0 LDS j-tiles, 1 SGPR broadcast
+  LaunchChoice tuned[kBuckets] = {};
+  bool tuned_ok[kBuckets] = {};
 };
 
 Board g_board;
@@ -251,15 +278,14 @@ void board_predict(Board& b, int nj, hipStream_t st) {
 
 LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   LaunchPlan p;
-  if (b.force_bi > 0) {
-    p.bi = b.force_bi;
-  } else {
-    p.bi = ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64);
-  }
-  p.variant = b.variant;
+  const int k = ni_bucket(ni);
+  const LaunchChoice c = b.tuned_ok[k] ? b.tuned[k] : default_choice(ni);
+  p.bi = b.force_bi > 0 ? b.force_bi : c.bi;
+  p.variant = b.force_variant >= 0 ? b.force_variant : c.variant;
+  p.bpc = b.force_bpc > 0 ? b.force_bpc : c.bpc;
   const int nbi = ceil_div(ni, p.bi);
   // the SGPR-broadcast kernel runs bi threads per block, the LDS one 256
-  const int target = device_cu_count() * b.blocks_per_cu * (p.variant == 1 ? 256 / p.bi : 1);
+  const int target = device_cu_count() * p.bpc * (p.variant == 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
   const int maxs = std::max(1, nj / std::max(1, b.min_jper));
   ns = std::min(ns, maxs);
@@ -726,28 +752,36 @@ PGAMD_API int pg6_dev_sync(void) {
 
 PGAMD_API void* pg6_dev_stream(void) { return g_board.open ? (void*)g_board.stream : nullptr; }
 
-// Launch-shape override used by the autotuner: bi in {0 (auto), 64, 128, 256},
-// blocks_per_cu > 0 sets the j-split target, min_jper > 0 the smallest j-range.
+// Launch-shape overrides: bi in {0 (auto), 64, 128, 256}; blocks_per_cu in
+// [1, 256] sets the j-split target (0 = auto); min_jper > 0 the smallest
+// j-range (0 = 128).
 PGAMD_API int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (bi != 0 && bi != 64 && bi != 128 && bi != 256) {
     api_error(3, "pg6_dev_set_launch: bi=%d not in {0,64,128,256}", bi);
     return -1;
   }
+  if (blocks_per_cu < 0 || blocks_per_cu > 256) {
+    api_error(3, "pg6_dev_set_launch: blocks_per_cu=%d not in [0, 256]", blocks_per_cu);
+    return -1;
+  }
   g_board.force_bi = bi;
-  if (blocks_per_cu > 0) g_board.blocks_per_cu = blocks_per_cu;
-  if (min_jper > 0) g_board.min_jper = min_jper;
+  g_board.force_bpc = blocks_per_cu;
+  g_board.min_jper = min_jper > 0 ? min_jper : 128;
   return 0;
 }
 
-// The launch shape the library would use for (ni, nj): {bi, nsplit, jper}.
-PGAMD_API int pg6_dev_plan(int ni, int nj, int* out3) {
+// The launch shape the library would use for (ni, nj):
+// {variant, bi, blocks_per_cu, nsplit, jper}.
+PGAMD_API int pg6_dev_plan(int ni, int nj, int* out5) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!board_check_open("pg6_dev_plan")) return -1;
   LaunchPlan p = plan_launch(g_board, ni > 0 ? ni : 1, nj > 0 ? nj : 1);
-  out3[0] = p.bi;
-  out3[1] = p.nsplit;
-  out3[2] = p.jper;
+  out5[0] = p.variant;
+  out5[1] = p.bi;
+  out5[2] = p.bpc;
+  out5[3] = p.nsplit;
+  out5[4] = p.jper;
   return 0;
 }
 
@@ -975,14 +1009,131 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
 }
 
 // Force kernel choice: 0 = j-particles staged in LDS tiles (pg6_force),
-// 1 = j-particles broadcast through scalar registers (pg6_force_s, default).
-// Both compute the same arithmetic; the choice affects speed only.
+// 1 = j-particles broadcast through scalar registers (pg6_force_s), -1 = the
+// autotuned or default choice for each i-count.  Both kernels compute the
+// same arithmetic; the choice affects speed only.
 PGAMD_API int pg6_dev_set_kernel(int variant) {
   std::lock_guard<std::mutex> lk(g_mutex);
-  if (variant != 0 && variant != 1) {
-    api_error(3, "pg6_dev_set_kernel: variant %d not in {0, 1}", variant);
+  if (variant < -1 || variant > 1) {
+    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1}", variant);
     return -1;
   }
-  g_board.variant = variant;
+  g_board.force_variant = variant;
+  return 0;
+}
+
+namespace {
+// Gathers i-particle arrays (AoS FP64 position and velocity, id) from the
+// first n predicted j-particles, for autotuning on the caller's own data.
+__global__ void pg6_gather_i(int n, JPred jp, double* pos, double* vel, int* id) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  pos[3 * k + 0] = jp.x[k];
+  pos[3 * k + 1] = jp.y[k];
+  pos[3 * k + 2] = jp.z[k];
+  const float4 vm = jp.vm[k];
+  vel[3 * k + 0] = vm.x;
+  vel[3 * k + 1] = vm.y;
+  vel[3 * k + 2] = vm.z;
+  id[k] = jp.id[k];
+}
+}  // namespace
+
+// Autotuner for the direct force: for each i-count bucket ni = 64, 128, ...,
+// min(65536, nj), times every (kernel, bi, blocks_per_cu) candidate on the
+// first nj predicted j-particles (i-set = the first ni of them; force, jerk,
+// potential and nearest neighbour, j-range reduction included) and keeps the
+// fastest for that bucket; plan_launch uses it for every ni in the bucket.
+// Call after pg6_dev_predict (or g6calc_firsthalf) so the predicted set is
+// representative.  out[4 * k .. 4k+3] = {variant, bi, bpc, microseconds} per
+// bucket k (may be NULL); returns the number of buckets tuned.  Explicit
+// overrides (pg6_dev_set_kernel, pg6_dev_set_launch) still take precedence.
+PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_dev_autotune")) return -1;
+  Board& b = g_board;
+  if (nj < 64 || nj > b.cap || b.nj_pred < nj) {
+    api_error(3, "pg6_dev_autotune: nj=%d must be >= 64 and predicted (last prediction %d)",
+              nj, b.nj_pred);
+    return -1;
+  }
+  if (reps <= 0) reps = 3;
+  const int nimax = std::min(nj, 65536);
+  DevBuf buf;
+  // per i: pos[3], vel[3], acc[3], jrk[3], pot (FP64); id, nnb (int); rnnb
+  const size_t per_i = 13 * sizeof(double) + 2 * sizeof(int) + sizeof(float);
+  buf.reserve((size_t)nimax * per_i);
+  double* ipos = buf.as<double>();
+  double* ivel = ipos + 3 * (size_t)nimax;
+  double* acc = ivel + 3 * (size_t)nimax;
+  double* jrk = acc + 3 * (size_t)nimax;
+  double* pot = jrk + 3 * (size_t)nimax;
+  int* iid = reinterpret_cast<int*>(pot + (size_t)nimax);
+  int* nnb = iid + nimax;
+  float* rnnb = reinterpret_cast<float*>(nnb + nimax);
+  hipStream_t st = b.stream;
+  hipLaunchKernelGGL(pg6_gather_i, dim3(ceil_div(nimax, 256)), dim3(256), 0, st, nimax, b.jp, ipos,
+                     ivel, iid);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const Outputs o{acc, jrk, pot, nnb, rnnb};
+  hipEvent_t e0, e1;
+  PGAMD_HIP_CHECK(hipEventCreate(&e0));
+  PGAMD_HIP_CHECK(hipEventCreate(&e1));
+  const bool timer_was_on = g_timer.on;
+  g_timer.on = false;
+  const int save_bi = b.force_bi, save_bpc = b.force_bpc, save_var = b.force_variant;
+  static const int kBpc[] = {4, 8, 16, 32, 64};
+  int tuned = 0;
+  for (int k = 0; k < kBuckets && (64 << k) <= nimax; ++k) {
+    const int ni = 64 << k;
+    float best = 1e30f;
+    LaunchChoice bc = default_choice(ni);
+    for (int var = 0; var <= 1; ++var)
+      for (int bi : {64, 128, 256})
+        for (int bpc : kBpc) {
+          b.force_variant = var;
+          b.force_bi = bi;
+          b.force_bpc = bpc;
+          board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);  // warm-up
+          PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+          for (int r = 0; r < reps; ++r) board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);
+          PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+          PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          PGAMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bc = {var, bi, bpc};
+          }
+        }
+    b.tuned[k] = bc;
+    b.tuned_ok[k] = true;
+    // the largest bucket also serves every larger ni
+    if (out) {
+      out[4 * k + 0] = bc.variant;
+      out[4 * k + 1] = bc.bi;
+      out[4 * k + 2] = bc.bpc;
+      out[4 * k + 3] = (int)(1000.f * best / reps);
+    }
+    ++tuned;
+  }
+  for (int k = tuned; k < kBuckets; ++k) {
+    b.tuned[k] = b.tuned[tuned - 1];
+    b.tuned_ok[k] = true;
+  }
+  b.force_bi = save_bi;
+  b.force_bpc = save_bpc;
+  b.force_variant = save_var;
+  g_timer.on = timer_was_on;
+  PGAMD_HIP_CHECK(hipEventDestroy(e0));
+  PGAMD_HIP_CHECK(hipEventDestroy(e1));
+  buf.release();
+  return tuned;
+}
+
+// Forgets the autotuned choices (back to the defaults).
+PGAMD_API int pg6_dev_autotune_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  for (int k = 0; k < kBuckets; ++k) g_board.tuned_ok[k] = false;
   return 0;
 }

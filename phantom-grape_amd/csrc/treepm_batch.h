@@ -91,29 +91,72 @@ __global__ __launch_bounds__(kThreads) void treepm_batch(
         if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f;
-#pragma unroll 2
-        for (int jj = g; jj < nt; jj += G) {
-          const float4 q = s_j[jj];
-          const float dx = q.x - ip.x;
-          const float dy = q.y - ip.y;
-          const float dz = q.z - ip.z;
-          float f;
-          if (MLP) {
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * rinv;
-            const float u = r * mc.inv_rc_s;
-            const float v = u * __builtin_amdgcn_rsqf(u);
-            const float y = mlp::eval(mlp::fresh(mlp::as_const(mc.w)), v);
-            const float fv = mc.xs3 * y * __builtin_amdgcn_rcpf(fmaf(r2, r, mc.r03_s));
-            f = (r2 > 0.f && r2 < mc.rc2_s) ? fv : 0.f;
-          } else {
-            const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
-            f = pgt::table_f(s_tab, s, scut, shift, base);
+        if (MLP) {
+          // This wave's j are jj = g + G t.  Pass 1 marks (64 t at a time,
+          // bit t of a wave-uniform mask) the j that some lane has inside
+          // the cut-off; pass 2 evaluates the network only for those, four
+          // j per parameter pass.  Pairs outside contribute exactly 0, as
+          // in the table path, so skipping them changes no sum.
+          const int nsub = nt > g ? (nt - g + G - 1) / G : 0;
+          for (int c0 = 0; c0 < nsub; c0 += 64) {
+            const int cn = min(64, nsub - c0);
+            unsigned long long mask = 0ull;
+            for (int t = 0; t < cn; ++t) {
+              const float4 q = s_j[g + G * (c0 + t)];
+              const float dx = q.x - ip.x;
+              const float dy = q.y - ip.y;
+              const float dz = q.z - ip.z;
+              const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+              const bool in = iv && r2 > 0.f && r2 < mc.rc2_s;
+              if (__builtin_amdgcn_ballot_w64(in)) mask |= 1ull << t;
+            }
+            while (mask) {
+              float v[4], fac[4], ddx[4], ddy[4], ddz[4];
+              bool inb[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const bool ok = mask != 0ull;
+                const int t = ok ? __builtin_ctzll(mask) : 0;
+                if (ok) mask &= mask - 1ull;
+                const float4 q = s_j[g + G * (c0 + t)];
+                const float dx = q.x - ip.x;
+                const float dy = q.y - ip.y;
+                const float dz = q.z - ip.z;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const float uu = fminf(r * mc.inv_rc_s, 1.f);
+                v[u] = uu * __builtin_amdgcn_rsqf(uu);
+                inb[u] = ok && r2 > 0.f && r2 < mc.rc2_s;
+                fac[u] = mc.xs3 * __builtin_amdgcn_rcpf(fmaf(r2, r, mc.r03_s));
+                ddx[u] = q.w * dx;
+                ddy[u] = q.w * dy;
+                ddz[u] = q.w * dz;
+              }
+              float y[4];
+              mlp::eval_k<4>(mlp::fresh(mlp::as_const(mc.w)), v, y);
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const float f = inb[u] ? y[u] * fac[u] : 0.f;
+                fax = fmaf(f, ddx[u], fax);
+                fay = fmaf(f, ddy[u], fay);
+                faz = fmaf(f, ddz[u], faz);
+              }
+            }
           }
-          fax = fmaf(f, q.w * dx, fax);
-          fay = fmaf(f, q.w * dy, fay);
-          faz = fmaf(f, q.w * dz, faz);
+        } else {
+#pragma unroll 2
+          for (int jj = g; jj < nt; jj += G) {
+            const float4 q = s_j[jj];
+            const float dx = q.x - ip.x;
+            const float dy = q.y - ip.y;
+            const float dz = q.z - ip.z;
+            const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
+            const float f = pgt::table_f(s_tab, s, scut, shift, base);
+            fax = fmaf(f, q.w * dx, fax);
+            fay = fmaf(f, q.w * dy, fay);
+            faz = fmaf(f, q.w * dz, faz);
+          }
         }
         ax += (double)fax;
         ay += (double)fay;

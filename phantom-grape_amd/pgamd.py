@@ -236,6 +236,8 @@ class G6:
         _sig(L, "pg6_dev_plan", C.c_int, C.c_int, C.c_int, _ip)
         _sig(L, "pg6_dev_read_predicted", C.c_int, C.c_int, _dp, _fp, _fp)
         _sig(L, "pg6_dev_set_kernel", C.c_int, C.c_int)
+        _sig(L, "pg6_dev_autotune", C.c_int, C.c_int, C.c_double, C.c_int, _ip)
+        _sig(L, "pg6_dev_autotune_reset", C.c_int)
         _sig(L, "pg6_dev_timing_start", C.c_int, C.c_int)
         _sig(L, "pg6_dev_timing_read", C.c_int, _dp, _ip)
         _sig(L, "pg6_cluster_items", C.c_int, C.c_int, _ip, _ip, C.c_int)
@@ -311,9 +313,18 @@ class G6:
         return x, v, m
 
     def plan(self, ni, nj):
-        o = np.zeros(3, np.int32)
+        """Launch shape for (ni, nj): dict of variant, bi, bpc, nsplit, jper."""
+        o = np.zeros(5, np.int32)
         self.lib.pg6_dev_plan(ni, nj, iptr(o))
-        return tuple(int(v) for v in o)
+        return dict(zip(("variant", "bi", "bpc", "nsplit", "jper"), (int(v) for v in o)))
+
+    def autotune(self, nj, eps2, reps=3):
+        """Runs pg6_dev_autotune; returns {ni: (variant, bi, bpc, us)}."""
+        o = np.zeros(4 * 11, np.int32)
+        k = self.lib.pg6_dev_autotune(nj, eps2, reps, iptr(o))
+        if k < 0:
+            raise RuntimeError(last_error(self.lib))
+        return {64 << b: tuple(int(v) for v in o[4 * b:4 * b + 4]) for b in range(k)}
 
     def cluster_items(self, coff):
         coff = i32(coff)

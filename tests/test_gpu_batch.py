@@ -141,10 +141,13 @@ def test_mlp_training_and_learned_force(tab, treepm_case):
     assert tab.lib.pg5_mlp_eval_dev(len(r), ptr(dr), ptr(out)) == 0
     f = out.cpu().numpy().astype(np.float64)
     fo = O.s2_force(PP, PM, r.astype(np.float64))
-    # the oracle forward pass agrees with the GPU's
+    # the oracle forward pass agrees with the GPU's, compared in the network's
+    # output units Y = f (r^3 + r0^3), which are O(1); near r_c, Y -> 0 and a
+    # relative comparison of f would only measure FP32 cancellation there
     v, _ = M.target(lambda rr: O.s2_force(PP, PM, rr), r.astype(np.float64), rc, r0)
-    yo = M.forward(p, v) / (r.astype(np.float64) ** 3 + r0 ** 3)
-    assert np.max(np.abs(f - yo) / np.abs(yo)) < 1e-4
+    r3 = r.astype(np.float64) ** 3 + r0 ** 3
+    yo = M.forward(p, v)
+    assert np.max(np.abs(f * r3 - yo)) < 2e-5 * np.max(np.abs(yo))
     e = np.abs(f - fo) / fo
     assert np.median(e) < 5e-3, (loss, np.median(e))
     # forces through the batch kernel with the learned cut-off

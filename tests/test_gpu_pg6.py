@@ -101,7 +101,42 @@ def test_force_vs_fp64_oracle(g6, ni, variant):
     # rnnb is the distance to the returned neighbour
     d = xo[r["nnb"]] - xo[isel]
     np.testing.assert_allclose(r["rnnb"], np.sqrt((d * d).sum(1) + eps * eps), rtol=1e-6)
-    g6.lib.pg6_dev_set_kernel(1)
+    g6.lib.pg6_dev_set_kernel(-1)
+
+
+def test_autotune_and_launch_shapes(g6):
+    n = 4096
+    eps = 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 4, eps)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    ti = 1.0 / 512
+    g6.set_ti(ti)
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.arange(0, n, 5)
+    base = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+    tuned = g6.autotune(n, eps * eps, reps=1)
+    assert sorted(tuned) == [64 << k for k in range(7)]
+    for ni, (var, bi, bpc, us) in tuned.items():
+        assert var in (0, 1) and bi in (64, 128, 256) and bpc in (4, 8, 16, 32, 64) and us > 0
+        p = g6.plan(ni, n)
+        assert (p["variant"], p["bi"], p["bpc"]) == (var, bi, bpc)
+        assert p["nsplit"] * p["jper"] >= n
+    # every launch shape gives the same result up to the j-range summation order
+    for var, bi, bpc in [(0, 64, 4), (0, 256, 64), (1, 64, 64), (1, 128, 4), (-1, 0, 0)]:
+        assert g6.lib.pg6_dev_set_kernel(var) == 0
+        assert g6.lib.pg6_dev_set_launch(bi, bpc, 64) == 0
+        r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+        for k in ("acc", "jrk", "pot"):
+            assert rel_err(r[k], base[k]).max() < 1e-6, (var, bi, bpc, k)
+    assert g6.lib.pg6_dev_set_launch(0, 0, 0) == 0
+    assert g6.lib.pg6_dev_autotune_reset() == 0
+    g6.lib.pgamd_set_abort_on_error(0)
+    try:
+        assert g6.lib.pg6_dev_set_kernel(2) == -1 and g6.lib.pg6_dev_set_launch(96, 0, 0) == -1
+        assert g6.lib.pg6_dev_set_launch(0, 1000, 0) == -1
+        assert g6.lib.pg6_dev_autotune(32, eps * eps, 1, None) == -1
+    finally:
+        g6.lib.pgamd_set_abort_on_error(1)
 
 
 def test_variants_agree_and_are_deterministic(g6):
