@@ -368,6 +368,10 @@ def _time(fn, stream, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def _threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+
+
 def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -378,15 +382,6 @@ def bench_c1(args, stream, cpu):
     """PG5 direct sum: Plummer N = 16384, eps = 1/64, one N x N force+pot."""
     import numpy as np
     import torch
-
-    if args.copy_calibration:
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
-        return
 
     import ics
     import pgamd
@@ -440,15 +435,6 @@ def bench_c3(args, stream, cpu):
     import numpy as np
     import torch
 
-    if args.copy_calibration:
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
-        return
-
     import ics
     import pgamd
     pp, pm = 0.1 / 128, 3.0 / 128
@@ -498,6 +484,16 @@ def bench_c3(args, stream, cpu):
         a = acc[mode].cpu().numpy()[sel]
         res[name]["accuracy_vs_closed_form_s2"] = _errs(a, f)
     T.lib.pg5_set_table_frc_bit(5)
+    if cpu and O.ref_available("pg5table"):
+        thr = _threads()
+        groups = np.arange(ng)
+        t, ra = O.ref_treepm(L, P, mm, pp, pm, thr, groups)
+        res["cpu_baseline"] = {
+            "ginteractions_s": round(L["work_pp"] / t / 1e9, 3), "kind": "reference",
+            "sample": f"all {ng} groups: g5_set_xmjMC + g5_calculate_force_on_xMC per group, "
+                      "OpenMP over groups with devid = thread (reference 512-entry table)",
+            "threads": min(thr, 24), "seconds": round(t, 3),
+            "accuracy_vs_closed_form_s2": _errs(ra[sel], f)}
     return res
 
 
@@ -507,15 +503,6 @@ def bench_c4(args, stream, cpu):
     all groups in one graph-captured persistent launch."""
     import numpy as np
     import torch
-
-    if args.copy_calibration:
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
-        return
 
     import ics
     import pgamd
@@ -558,6 +545,16 @@ def bench_c4(args, stream, cpu):
     a = acc.cpu().numpy()[sel]
     ad, _ = O.pg5_force(P[sel], eps * eps, P, mm)
     res["accuracy_vs_fp64_direct"] = _errs(a, ad)
+    if cpu and O.ref_available("pg5quad"):
+        thr = _threads()
+        t, ra, _ = O.ref_bh(L, P, mm, L["cells"], eps, thr, np.arange(ng))
+        res["cpu_baseline"] = {
+            "ginteractions_s": round((L["work_pp"] + L["work_pc"]) / t / 1e9, 3),
+            "kind": "reference",
+            "sample": f"all {ng} groups: g5 particle + g5c cell calls per group, OpenMP over "
+                      "groups with devid = thread",
+            "threads": min(thr, 24), "seconds": round(t, 3),
+            "accuracy_vs_fp64_direct": _errs(ra[sel], ad)}
     return res
 
 
@@ -566,15 +563,6 @@ def bench_c5(args, stream, cpu):
     force + jerk + pot of every member, one graph-captured persistent launch."""
     import numpy as np
     import torch
-
-    if args.copy_calibration:
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 5 copies of {a.numel() * 4} bytes")
-        return
 
     import ics
     import pgamd
@@ -622,6 +610,15 @@ def bench_c5(args, stream, cpu):
     res["accuracy_vs_fp64"] = {"acc": _errs(np.concatenate(ea), np.concatenate(oa)),
                                "jerk": _errs(np.concatenate(ej), np.concatenate(oj)),
                                "sample": "24 whole clusters"}
+    if cpu and O.ref_available("pg6"):
+        sel = np.sort(np.random.default_rng(7).choice(len(sizes), 512, replace=False))
+        t, *_ = O.ref_clusters(sel, coff, X, V, Mm, (4.0 / sizes) ** 2)
+        w = float((sizes[sel] * (sizes[sel] - 1)).sum())
+        res["cpu_baseline"] = {
+            "ginteractions_s": round(w / t / 1e9, 3), "kind": "reference",
+            "sample": "512 random clusters, one PG6 session each (set_j_particle per member, "
+                      "firsthalf + lasthalf2 in chunks of 48)",
+            "threads": _threads(), "seconds": round(t, 3)}
     return res
 
 

@@ -337,3 +337,90 @@ class RefPG6:
         if h2 is not None:
             out["nbl"] = nbls
         return out
+
+
+# ------------------------------------------- CPU baselines of the batches
+_cb = None
+
+
+def _cpubatch() -> C.CDLL:
+    global _cb
+    if _cb is None:
+        path = os.path.join(HERE, "libcpubatch.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -f oracle/Makefile)")
+        L = C.CDLL(path, mode=os.RTLD_LOCAL)
+        _sig(L, "cb_treepm", C.c_double, C.c_char_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip,
+             _dp, _dp, C.c_double, C.c_double, C.c_int, _dp)
+        _sig(L, "cb_bh", C.c_double, C.c_char_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip,
+             _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp)
+        _sig(L, "cb_clusters", C.c_double, C.c_char_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
+             _dp, _dp, _dp)
+        _cb = L
+    return _cb
+
+
+def _refpath(name: str) -> bytes:
+    path = os.path.join(REFDIR, f"libref_{name}.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built (make -f oracle/ref.mk)")
+    return path.encode()
+
+
+def _csr_subset(off, cnt, groups):
+    """Sub-CSR (offsets, counts) of the selected groups, same lists."""
+    return _i(np.asarray(off)[groups]), _i(np.asarray(cnt)[groups])
+
+
+def ref_treepm(lists, pos, mass, eps_pp, eps_pm, nthreads, groups):
+    """Reference table library (PG5/table/libpg5) over the selected groups, in
+    an OpenMP loop with devid = thread.  Returns (seconds, acc[n][3])."""
+    pos, mass = _d(pos), _d(mass)
+    g = np.asarray(groups)
+    ioff, icnt = _csr_subset(lists["ioff"], lists["icnt"], g)
+    joff, jcnt = _csr_subset(lists["joff"], lists["jcnt"], g)
+    ilist, jlist = _i(lists["ilist"]), _i(lists["jlist"])
+    acc = np.zeros((len(pos), 3))
+    t = _cpubatch().cb_treepm(_refpath("pg5table"), len(g), _P(ioff, _ip), _P(icnt, _ip),
+                              _P(ilist, _ip), _P(joff, _ip), _P(jcnt, _ip), _P(jlist, _ip),
+                              _P(pos), _P(mass), float(eps_pp), float(eps_pm), int(nthreads),
+                              _P(acc))
+    if t < 0:
+        raise RuntimeError("cb_treepm failed")
+    return t, acc
+
+
+def ref_bh(lists, pos, mass, cells10, eps, nthreads, groups):
+    """Reference GRAPE-5 + quad_tree libraries (PG5/newton_quad/quad) over the
+    selected groups, OpenMP loop with devid = thread.  Returns (seconds, acc,
+    pot) with the libraries' sign conventions."""
+    pos, mass, cells10 = _d(pos), _d(mass), _d(cells10)
+    g = np.asarray(groups)
+    ioff, icnt = _csr_subset(lists["ioff"], lists["icnt"], g)
+    joff, jcnt = _csr_subset(lists["joff"], lists["jcnt"], g)
+    coff, ccnt = _csr_subset(lists["coff"], lists["ccnt"], g)
+    ilist, jlist, clist = _i(lists["ilist"]), _i(lists["jlist"]), _i(lists["clist"])
+    acc = np.zeros((len(pos), 3))
+    pot = np.zeros(len(pos))
+    t = _cpubatch().cb_bh(_refpath("pg5quad"), len(g), _P(ioff, _ip), _P(icnt, _ip),
+                          _P(ilist, _ip), _P(joff, _ip), _P(jcnt, _ip), _P(jlist, _ip),
+                          _P(coff, _ip), _P(ccnt, _ip), _P(clist, _ip), _P(pos), _P(mass),
+                          _P(cells10), float(eps), int(nthreads), _P(acc), _P(pot))
+    if t < 0:
+        raise RuntimeError("cb_bh failed")
+    return t, acc, pot
+
+
+def ref_clusters(sel, coff, pos, vel, mass, eps2):
+    """Reference GRAPE-6 library, one session per selected cluster (members at
+    t = 0).  Returns (seconds, acc, jrk, pot) for the members of `sel`."""
+    pos, vel, mass, eps2 = _d(pos), _d(vel), _d(mass), _d(eps2)
+    sel, coff = _i(sel), _i(coff)
+    acc = np.zeros((len(mass), 3))
+    jrk = np.zeros((len(mass), 3))
+    pot = np.zeros(len(mass))
+    t = _cpubatch().cb_clusters(_refpath("pg6"), len(sel), _P(sel, _ip), _P(coff, _ip),
+                                _P(pos), _P(vel), _P(mass), _P(eps2), _P(acc), _P(jrk), _P(pot))
+    if t < 0:
+        raise RuntimeError("cb_clusters failed")
+    return t, acc, jrk, pot

@@ -321,3 +321,81 @@ def test_mlp_target_is_bounded_and_smooth():
     v, y = M.target(lambda rr: O.s2_force(pp, pm, rr), r, pm, pp)
     assert y.max() < 13 and y.min() > -1e-6
     assert np.max(np.abs(np.diff(y))) < 0.05  # no jumps on a fine grid
+
+
+# ------------------------------------------ CPU baseline drivers of the batches
+def _need_ref(*names):
+    for n in names:
+        if not O.ref_available(n):
+            pytest.skip(f"reference library {n} not built")
+
+
+def test_cpu_treepm_driver_matches_per_group_calls():
+    _need_ref("pg5table")
+    import pgamd
+    pp, pm = 0.1 / 16, 3.0 / 16
+    m, x = ics.clumpy_box(4096, 5)
+    L = pgamd.treepm_lists(x, pm, 64)
+    P, mm = L["pos"], m[L["src"]]
+    groups = np.arange(0, len(L["ioff"]), 3)
+    t, acc = O.ref_treepm(L, P, mm, pp, pm, 4, groups)
+    assert t > 0
+    R = O.RefTable()
+    R.gen_s2(pp, pm)
+    for g in groups[:12]:
+        ii = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        jj = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        xj, mj = P[jj], mm[jj]
+        if len(jj) % 2:  # the driver pads odd lists with one massless j
+            xj, mj = np.vstack([xj, np.zeros((1, 3))]), np.append(mj, 0.0)
+        a = R.force(P[ii], xj, mj, devid=0)
+        np.testing.assert_array_equal(acc[ii], a)
+    # and it is the short-range force: compare with the FP64 closed form
+    sel = L["ilist"][L["ioff"][groups[0]]:L["ioff"][groups[0]] + L["icnt"][groups[0]]]
+    f = O.table_force_f64(P[sel], P, mm, pp, pm)
+    assert np.median(rel_err(acc[sel], f)) < 1e-2
+
+
+def test_cpu_bh_driver_matches_per_group_calls():
+    _need_ref("pg5quad")
+    import pgamd
+    eps = 1.0 / 64
+    m, x = ics.hernquist(4096, 9)
+    L = pgamd.bh_lists(x, m, 0.5, 128, 8)
+    P, mm = L["pos"], m[L["src"]]
+    groups = np.arange(len(L["ioff"]))
+    t, acc, pot = O.ref_bh(L, P, mm, L["cells"], eps, 4, groups)
+    assert t > 0
+    R = O.RefPG5(quad=True)
+    for g in groups[:6]:
+        ii = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        jj = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        cc = L["clist"][L["coff"][g]:L["coff"][g] + L["ccnt"][g]]
+        a1, p1 = R.force(P[ii], P[jj], mm[jj], eps)
+        a2, p2 = (R.cell_force(P[ii], L["cells"][cc, :3], L["cells"][cc, 3], L["cells"][cc, 4:],
+                               eps) if len(cc) else (0.0, 0.0))
+        np.testing.assert_allclose(acc[ii], a1 + a2, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(pot[ii], p1 + p2, rtol=1e-12, atol=0)
+    # the tree force approximates the direct sum
+    ad, _ = O.pg5_force(P, eps * eps, P, mm)
+    assert np.median(rel_err(acc, ad)) < 5e-3
+
+
+def test_cpu_cluster_driver_matches_reference_sessions():
+    _need_ref("pg6")
+    sizes = np.array([64, 100, 77])
+    parts = [ics.plummer(int(n), 70 + k) for k, n in enumerate(sizes)]
+    M = np.concatenate([p[0] for p in parts])
+    X = np.concatenate([p[1] for p in parts])
+    V = np.concatenate([p[2] for p in parts])
+    coff = np.concatenate([[0], np.cumsum(sizes)])
+    e2 = (4.0 / sizes) ** 2
+    t, acc, jrk, pot = O.ref_clusters([0, 2], coff, X, V, M, e2)
+    assert t > 0
+    for c in (0, 2):
+        s = slice(coff[c], coff[c + 1])
+        idx = np.arange(sizes[c], dtype=np.int32)
+        o = O.pg6_force(X[s], V[s], idx, e2[c], X[s], V[s], M[s], idx)
+        assert np.percentile(rel_err(acc[s], o["acc"]), 99) < 1e-6
+        assert np.percentile(rel_err(jrk[s], o["jrk"]), 99) < 1e-4
+    assert np.all(acc[coff[1]:coff[2]] == 0)
