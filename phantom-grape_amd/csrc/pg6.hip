@@ -312,8 +312,17 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
   PGAMD_HIP_CHECK(hipGetLastError());
   g_timer.after(st);
   if (p.nsplit > 1) {
-    hipLaunchKernelGGL((pg6::pg6_reduce<NN>), dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
-                       p.nsplit, part, jp, ipos, eps2, out);
+    // one thread per i walks the splits; with many splits a wavefront per i
+    // (few i) or four threads per i (many i, coalesced loads)
+    if (p.nsplit > 16 && ni <= 4096)
+      hipLaunchKernelGGL((pg6::pg6_reduce_w<NN>), dim3(ceil_div(ni, 4)), dim3(256), 0, st, ni,
+                         p.nsplit, part, jp, ipos, eps2, out);
+    else if (p.nsplit > 8)
+      hipLaunchKernelGGL((pg6::pg6_reduce_q4<NN>), dim3(ceil_div(ni, 64)), dim3(256), 0, st, ni,
+                         p.nsplit, part, jp, ipos, eps2, out);
+    else
+      hipLaunchKernelGGL((pg6::pg6_reduce<NN>), dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
+                         p.nsplit, part, jp, ipos, eps2, out);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
 }

@@ -107,6 +107,14 @@ __global__ void pg6_predict(int nj, double ti, JState js, JPred jp) {
   jp.id[j] = js.idx[j];
 }
 
+// Nearest-neighbour key of one pair: the top 24 bits of 1/r (a larger key is
+// a nearer j) with the tile slot 255 - jj in the low byte, so that max()
+// keeps the lowest slot among equal 1/r.  One v_perm_b32: bytes 3..1 from
+// 1/r, byte 0 from the (wave-uniform) slot.
+__device__ inline unsigned nn_key(float rinv, unsigned, unsigned slot) {
+  return __builtin_amdgcn_perm(__float_as_uint(rinv), slot, 0x07060500u);
+}
+
 // sqrt(|dx|^2 + eps^2) to j-particle j, with dx formed as in the force loop
 // (FP64 difference rounded to FP32): the reference's rnnb
 // (PG6/libpg6/gravity.c:652-673).
@@ -206,8 +214,7 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
       fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
       fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
       if (NN) {
-        const unsigned key = (__float_as_uint(rinv) & kmask) | (unsigned)(255 - jj);
-        tkey = max(tkey, key);
+        tkey = max(tkey, nn_key(rinv, kmask, (unsigned)(255 - jj)));
       }
     }
     if (NN) {
@@ -351,8 +358,7 @@ __global__ __launch_bounds__(kThreads) void pg6_force_s(
       fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
       fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
       if (NN) {
-        const unsigned key = (__float_as_uint(rinv) & kmask) | (unsigned)(255 - jj);
-        tkey = max(tkey, key);
+        tkey = max(tkey, nn_key(rinv, kmask, (unsigned)(255 - jj)));
       }
     }
     if (NN) {
@@ -396,6 +402,142 @@ __global__ __launch_bounds__(kThreads) void pg6_force_s(
       part.rmax[o] = rmax;
       part.nn[o] = nn;
     }
+  }
+}
+
+// Nearest-neighbour candidate order: larger (quantised) 1/r wins, equal
+// values go to the lower j address; (0, -1) is "none".  The order is total,
+// so any reduction tree gives the same winner.
+__device__ inline bool nn_better(float r, int n, float rbest, int nbest) {
+  return r > rbest || (r == rbest && n >= 0 && (nbest < 0 || n < nbest));
+}
+
+// Sums the nsplit partials of each i with one wavefront per i: lane l takes
+// splits l, l+64, ... in order, then a fixed xor-butterfly combines the
+// lanes (every lane ends with the same sums, so the result does not depend
+// on which lane writes).  For many splits and few i, where one thread per i
+// would walk nsplit dependent loads.  Block = 256 threads = 4 i.
+template <bool NN>
+__global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials part, JPred jp,
+                                                    const double* __restrict__ ipos, float eps2,
+                                                    Outputs out) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= ni) return;  // uniform per wavefront
+  double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  float rmax = 0.f;
+  int nn = -1;
+  for (int s = lane; s < nsplit; s += 64) {
+    const size_t o = (size_t)s * ni + i;
+    v[0] += part.ax[o];
+    v[1] += part.ay[o];
+    v[2] += part.az[o];
+    v[3] += part.pot[o];
+    v[4] += part.jx[o];
+    v[5] += part.jy[o];
+    v[6] += part.jz[o];
+    if (NN) {
+      const float r = part.rmax[o];
+      const int n = part.nn[o];
+      if (nn_better(r, n, rmax, nn)) {
+        rmax = r;
+        nn = n;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    if (NN) {
+      const float r = __shfl_xor(rmax, off, 64);
+      const int n = __shfl_xor(nn, off, 64);
+      if (nn_better(r, n, rmax, nn)) {
+        rmax = r;
+        nn = n;
+      }
+    }
+  }
+  if (lane != 0) return;
+  out.acc[3 * i + 0] = v[0];
+  out.acc[3 * i + 1] = v[1];
+  out.acc[3 * i + 2] = v[2];
+  out.jrk[3 * i + 0] = v[4];
+  out.jrk[3 * i + 1] = v[5];
+  out.jrk[3 * i + 2] = v[6];
+  out.pot[i] = v[3];
+  if (NN) {
+    out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+    out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
+                                        eps2)
+                          : 0.f;
+  }
+}
+
+// Sums the nsplit partials of each i with four threads per i: thread q of i
+// takes splits q, q+4, ... in order (consecutive threads hold consecutive i,
+// so every load is coalesced), then thread 0 of each i adds the four sums in
+// q order.  Deterministic.  For many i and many splits.  Block = 256 threads
+// = 64 i.
+template <bool NN>
+__global__ __launch_bounds__(256) void pg6_reduce_q4(int ni, int nsplit, Partials part,
+                                                     JPred jp, const double* __restrict__ ipos,
+                                                     float eps2, Outputs out) {
+  __shared__ double s_v[7][4][64];
+  __shared__ float s_r[4][64];
+  __shared__ int s_n[4][64];
+  const int li = threadIdx.x & 63;
+  const int q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + li;
+  double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  float rmax = 0.f;
+  int nn = -1;
+  if (i < ni) {
+    for (int s = q; s < nsplit; s += 4) {
+      const size_t o = (size_t)s * ni + i;
+      v[0] += part.ax[o];
+      v[1] += part.ay[o];
+      v[2] += part.az[o];
+      v[3] += part.pot[o];
+      v[4] += part.jx[o];
+      v[5] += part.jy[o];
+      v[6] += part.jz[o];
+      if (NN) {
+        const float r = part.rmax[o];
+        const int n = part.nn[o];
+        if (nn_better(r, n, rmax, nn)) {
+          rmax = r;
+          nn = n;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) s_v[k][q][li] = v[k];
+  s_r[q][li] = rmax;
+  s_n[q][li] = nn;
+  __syncthreads();
+  if (q != 0 || i >= ni) return;
+  for (int t = 1; t < 4; ++t) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] += s_v[k][t][li];
+    if (NN && nn_better(s_r[t][li], s_n[t][li], rmax, nn)) {
+      rmax = s_r[t][li];
+      nn = s_n[t][li];
+    }
+  }
+  out.acc[3 * i + 0] = v[0];
+  out.acc[3 * i + 1] = v[1];
+  out.acc[3 * i + 2] = v[2];
+  out.jrk[3 * i + 0] = v[4];
+  out.jrk[3 * i + 1] = v[5];
+  out.jrk[3 * i + 2] = v[6];
+  out.pot[i] = v[3];
+  if (NN) {
+    out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+    out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
+                                        eps2)
+                          : 0.f;
   }
 }
 
