@@ -270,7 +270,10 @@ def main():
                    "parallelism": "single GPU, j-range split",
                    "launch": plan, "autotuned": tuned is not None},
         "pct_fp32_peak": round(100 * achieved / FP32_PEAK_TFLOPS, 2),
-        "roofline": {"bound": "fp32-valu", "kernel": "pg6_force",
+        "roofline": {"bound": "fp32-valu",
+                     "bound_note": "compute-bound on the FP32 vector ALU (no MFMA, HBM traffic "
+                                   "< 1% of its roofline); peak = MI355X FP32 157.3 TFLOP/s",
+                     "kernel": "pg6_force",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "flops_per_interaction": FLOPS_PER_INTERACTION,
@@ -293,8 +296,9 @@ def main():
 
 
 def cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk, gpu_pot, gpu_value):
-    """Accuracy against the FP64 oracle and the reference library timed on this
-    host's cores, on a seeded sample of 2048 i-particles against all N j."""
+    """Accuracy against the FP64 oracle on a seeded sample of 2048 i-particles,
+    and the reference library timed on this host's cores over the whole step
+    (all N i-particles against all N j)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -329,19 +333,21 @@ def cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk, gpu_pot,
     r = None
     for k in range(6):
         t0 = time.perf_counter()
-        r = R.calc(n, TBLOCK, xi[sel], vi[sel], idx[sel], eps2, predict=True)
+        r = R.calc(n, TBLOCK, xi, vi, idx, eps2, predict=True)
         times.append(time.perf_counter() - t0)
     tmed = float(np.median(times[1:]))
-    cpu_value = len(sel) * (n - 1) / tmed / 1e9
-    log(f"reference CPU: {tmed * 1e3:.1f} ms for {len(sel)} i ({cpu_value:.3f} Gint/s, "
+    cpu_value = n * (n - 1) / tmed / 1e9
+    log(f"reference CPU: {tmed * 1e3:.1f} ms per step ({cpu_value:.3f} Gint/s, "
         f"{threads} threads)")
     out["cpu_baseline"] = {
         "value": round(cpu_value, 4), "unit": "Ginteractions/s", "cores": threads,
         "kind": "reference",
-        "sample": f"{len(sel)} i x {n} j, g6_set_ti + g6calc_firsthalf + g6calc_lasthalf2p in "
-                  f"chunks of 48 i (PG6/libpg6 AVX2+OpenMP), median of 5 after 1 warm-up",
+        "sample": f"the whole step, {n} i x {n} j: g6_set_ti + g6calc_firsthalf + "
+                  f"g6calc_lasthalf2p in chunks of 48 i (PG6/libpg6 AVX2+OpenMP), median of 5 "
+                  f"after 1 warm-up, {tmed:.2f} s per step",
         "cpu": cpu_model(), "gpu_over_cpu": round(gpu_value / cpu_value, 1),
-        "reference_accuracy": {"acc": stats(r["acc"], o["acc"]), "jerk": stats(r["jrk"], o["jrk"])},
+        "reference_accuracy": {"acc": stats(r["acc"][sel], o["acc"]),
+                               "jerk": stats(r["jrk"][sel], o["jrk"])},
     }
     return out
 
