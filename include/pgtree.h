@@ -28,6 +28,32 @@ int pgt_info(void *h, long long *info);
 int pgt_get(void *h, int what, void *out);
 void pgt_free(void *h);
 
+/* ------------------------------------------------------------------ libpgtree_dev.so
+ * pgamd-library: pgtree_dev
+ * The same lists built on the GPU from positions already in HBM (all
+ * pointers are device pointers; `stream` is a hipStream_t, null = the
+ * library's own).  Array for array identical to the host builder's, except
+ * the numbering of Barnes-Hut cells (the lists hold the same cell contents
+ * in the same order).  No counterpart in the reference: its tree code builds
+ * its lists on the CPU (PG5/newton_quad/source.cpp:215-707). */
+void *pgt_dev_treepm_build(int n, const double *pos, double rcut, int ncrit, void *stream);
+void *pgt_dev_bh_build(int n, const double *pos, const double *mass, double theta, int ncrit,
+                       int nleaf, void *stream);
+/* info[8] as pgt_info. */
+int pgt_dev_info(void *h, long long *info);
+/* Device pointer of array `what` (codes of pgt_get), valid until pgt_dev_free. */
+const void *pgt_dev_ptr(void *h, int what);
+/* Copies array `what` to host memory (sizes as pgt_get). */
+int pgt_dev_get(void *h, int what, void *out);
+/* Batch-call inputs on the device: pos4[np] = float4 {float(x) * float(scale),
+ * ..., float(mass[src])}; cells12[ncells] = {x, y, z, M, Qxx, Qxy, Qxz, Qyy,
+ * Qyz, Qzz, 0, 0} as floats.  Either output may be null. */
+int pgt_dev_pack(void *h, const double *mass, double scale, void *pos4, void *cells12,
+                 void *stream);
+void pgt_dev_free(void *h);
+void pgamd_set_abort_on_error(int flag);
+int pgamd_last_error(char *buf, int buflen);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -133,35 +133,64 @@ void build_octree(int n, const double* x, const double* m, const double lo[3], d
       a = b;
     }
   }
-  // moments (about each node's centre of mass), from its particles
+  // Moments, bottom-up (children are created after their parent, so a reverse
+  // sweep sees every child first): a leaf sums its particles; an internal
+  // node combines its children's mass, centre of mass and quadrupole with the
+  // parallel-axis shift.  The GPU builder (treebuild_dev.hip) evaluates the
+  // same operations in the same order, so both give identical bits.
   const int nn = (int)t.nodes.size();
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int k = 0; k < nn; ++k) {
+  for (int k = nn - 1; k >= 0; --k) {
     Node& nd = t.nodes[k];
-    double M = 0, c[3] = {0, 0, 0};
-    for (int p = nd.lo; p < nd.hi; ++p) {
-      const int i = t.perm[p];
-      const double mi = m ? m[i] : 1.0;
-      M += mi;
-      for (int d = 0; d < 3; ++d) c[d] += mi * x[3 * i + d];
+    bool leaf = true;
+    for (int o = 0; o < 8; ++o) leaf = leaf && nd.child[o] < 0;
+    double M = 0, c[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
+    if (leaf) {
+      for (int p = nd.lo; p < nd.hi; ++p) {
+        const int i = t.perm[p];
+        const double mi = m ? m[i] : 1.0;
+        M += mi;
+        for (int d = 0; d < 3; ++d) c[d] += mi * x[3 * i + d];
+      }
+      for (int d = 0; d < 3; ++d) nd.com[d] = M > 0 ? c[d] / M : nd.c[d];
+      for (int p = nd.lo; p < nd.hi; ++p) {
+        const int i = t.perm[p];
+        const double mi = m ? m[i] : 1.0;
+        double dd[3];
+        for (int d = 0; d < 3; ++d) dd[d] = x[3 * i + d] - nd.com[d];
+        const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        int kk = 0;
+        for (int a = 0; a < 2; ++a)
+          for (int b = a; b < 3; ++b) {
+            double v = 3.0 * dd[a] * dd[b];
+            if (a == b) v -= r2;
+            q[kk++] += mi * v;
+          }
+      }
+    } else {
+      for (int o = 0; o < 8; ++o) {
+        if (nd.child[o] < 0) continue;
+        const Node& ch = t.nodes[nd.child[o]];
+        M += ch.m;
+        for (int d = 0; d < 3; ++d) c[d] += ch.m * ch.com[d];
+      }
+      for (int d = 0; d < 3; ++d) nd.com[d] = M > 0 ? c[d] / M : nd.c[d];
+      for (int o = 0; o < 8; ++o) {
+        if (nd.child[o] < 0) continue;
+        const Node& ch = t.nodes[nd.child[o]];
+        double dd[3];
+        for (int d = 0; d < 3; ++d) dd[d] = ch.com[d] - nd.com[d];
+        const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        int kk = 0;
+        for (int a = 0; a < 2; ++a)
+          for (int b = a; b < 3; ++b) {
+            double v = 3.0 * dd[a] * dd[b];
+            if (a == b) v -= r2;
+            q[kk] += ch.q[kk] + ch.m * v;
+            ++kk;
+          }
+      }
     }
-    for (int d = 0; d < 3; ++d) nd.com[d] = M > 0 ? c[d] / M : nd.c[d];
     nd.m = M;
-    double q[6] = {0, 0, 0, 0, 0, 0};
-    for (int p = nd.lo; p < nd.hi; ++p) {
-      const int i = t.perm[p];
-      const double mi = m ? m[i] : 1.0;
-      double dd[3];
-      for (int d = 0; d < 3; ++d) dd[d] = x[3 * i + d] - nd.com[d];
-      const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
-      int kk = 0;
-      for (int a = 0; a < 2; ++a)
-        for (int b = a; b < 3; ++b) {
-          double v = 3.0 * dd[a] * dd[b];
-          if (a == b) v -= r2;
-          q[kk++] += mi * v;
-        }
-    }
     q[5] = -(q[0] + q[3]);
     std::memcpy(nd.q, q, sizeof(q));
   }

@@ -1,0 +1,1071 @@
+// libpgtree_dev.so: the interaction lists of the batched TreePM and
+// Barnes-Hut calls, built on the GPU from positions already in HBM.
+//
+// Same algorithm and same output as the host builder libpgtree.so
+// (treebuild.cpp), array for array: Morton keys (21 bits per axis), a stable
+// radix sort, the octree split level by level, bottom-up moments, i-groups of
+// <= ncrit particles in Morton order, TreePM ghosts and grid-based j-lists
+// sorted by particle, the Barnes-Hut walk with the opening test of
+// PG5/newton_quad/source.cpp:660-707, and groups ordered by decreasing cost.
+// Every FP64 operation that decides or produces a value is evaluated in the
+// host builder's order with contraction off, so the two agree bit for bit;
+// only the numbering of the Barnes-Hut cells differs (here: the order of the
+// octree levels), so cell lists match as lists of cell contents.
+//
+// The build is synchronous on the caller's stream: list lengths are read back
+// between phases to size the next allocation.
+#include "pgamd_common.h"
+#include "pgamd_api.h"
+
+#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
+#include <hipcub/device/device_segmented_radix_sort.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+using namespace pgamd;
+
+namespace {
+
+using u64 = unsigned long long;
+constexpr int kLevels = 21;  // bits per axis of the Morton key
+
+// ------------------------------------------------------------ allocations
+struct Mem {
+  std::vector<void*> ptrs;
+  template <class T>
+  T* get(size_t n) {
+    void* p = nullptr;
+    PGAMD_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  void free_all() {
+    for (void* p : ptrs) PGAMD_HIP_CHECK(hipFree(p));
+    ptrs.clear();
+  }
+};
+
+// Result of one build: device arrays in the host builder's layout.
+struct Lists {
+  Mem mem;
+  long long np = 0, ng = 0, ni = 0, nj = 0, nc = 0, ncells = 0, work_pp = 0, work_pc = 0;
+  double* pos = nullptr;  // [np][3]
+  int* src = nullptr;
+  int *ioff = nullptr, *icnt = nullptr, *ilist = nullptr;
+  int *joff = nullptr, *jcnt = nullptr, *jlist = nullptr;
+  int *coff = nullptr, *ccnt = nullptr, *clist = nullptr;
+  int* order = nullptr;
+  double* cells = nullptr;  // [ncells][10]
+};
+
+// Temporary storage for one build, released at its end.
+struct Build {
+  hipStream_t st;
+  Mem tmp;
+  void* cub = nullptr;
+  size_t cub_bytes = 0;
+  explicit Build(hipStream_t s) : st(s) {}
+  ~Build() {
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    if (cub) PGAMD_HIP_CHECK(hipFree(cub));
+    tmp.free_all();
+  }
+  void* cub_tmp(size_t bytes) {
+    if (bytes > cub_bytes) {
+      PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+      if (cub) PGAMD_HIP_CHECK(hipFree(cub));
+      PGAMD_HIP_CHECK(hipMalloc(&cub, bytes));
+      cub_bytes = bytes;
+    }
+    return cub;
+  }
+  template <class T>
+  T read(const T* d) {
+    T v;
+    PGAMD_HIP_CHECK(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    return v;
+  }
+  // out[k] = sum in[0..k), out[n] = total (out has n + 1 entries); returns total.
+  int scan(const int* in, int* out, int n) {
+    size_t b = 0;
+    PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
+    PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub_tmp(b), b, in, out, n, st));
+    int last[2] = {0, 0};
+    if (n > 0) {
+      PGAMD_HIP_CHECK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                     st));
+      PGAMD_HIP_CHECK(hipMemcpyAsync(&last[1], in + n - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                     st));
+    }
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    const int tot = last[0] + last[1];
+    PGAMD_HIP_CHECK(hipMemcpyAsync(out + n, &tot, sizeof(int), hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    return tot;
+  }
+  // Stable radix sort of (key, value) pairs; int keys must be >= 0.
+  void sort_pairs(const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit) {
+    sort_pairs(reinterpret_cast<const unsigned*>(kin), reinterpret_cast<unsigned*>(kout), vin,
+               vout, n, end_bit);
+  }
+  template <class K>
+  void sort_pairs(const K* kin, K* kout, const int* vin, int* vout, int n, int end_bit) {
+    if (n <= 0) return;
+    size_t b = 0;
+    PGAMD_HIP_CHECK(
+        hipcub::DeviceRadixSort::SortPairs(nullptr, b, kin, kout, vin, vout, n, 0, end_bit, st));
+    PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(cub_tmp(b), b, kin, kout, vin, vout, n, 0,
+                                                       end_bit, st));
+  }
+};
+
+// Bits needed to hold every value in [0, v].
+inline int bits_for(long long v) {
+  int b = 1;
+  while ((1LL << b) <= v) ++b;
+  return b;
+}
+
+inline dim3 grid_for(long long n, int threads = 256) {
+  return dim3((unsigned)std::max<long long>(1, (n + threads - 1) / threads));
+}
+
+// ------------------------------------------------------------ Morton keys
+__device__ inline u64 spread3(u64 v) {
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+__global__ void k_keys(int n, const double* __restrict__ x, double lo0, double lo1, double lo2,
+                       double s, u64* __restrict__ key, int* __restrict__ idx) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lo0, lo1, lo2};
+  u64 k = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double u = (x[3 * i + d] - lo[d]) * s;
+    long long q = (long long)floor(u);
+    q = q < 0 ? 0 : (q > (1 << kLevels) - 1 ? (1 << kLevels) - 1 : q);
+    k |= spread3((u64)q) << (2 - d);
+  }
+  key[i] = k;
+  idx[i] = i;
+}
+
+// Bounding box: per-block min/max of each axis, then one block over those.
+__global__ void k_bbox_part(int n, const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double s[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int d = 0; d < 3; ++d) {
+      v[d] = fmin(v[d], x[3 * i + d]);
+      v[3 + d] = fmax(v[3 + d], x[3 * i + d]);
+    }
+  for (int k = 0; k < 6; ++k) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 6; ++k)
+        s[k][threadIdx.x] = k < 3 ? fmin(s[k][threadIdx.x], s[k][threadIdx.x + h])
+                                  : fmax(s[k][threadIdx.x], s[k][threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = s[k][0];
+}
+
+__global__ void k_bbox_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 6; ++k)
+      v[k] = k < 3 ? fmin(v[k], part[6 * b + k]) : fmax(v[k], part[6 * b + k]);
+  for (int k = 0; k < 6; ++k) out[k] = v[k];
+}
+
+// Particles in Morton order: pos[p] = x[perm[p]], mass likewise (optional).
+__global__ void k_gather(int n, const int* __restrict__ perm, const double* __restrict__ x,
+                         const double* __restrict__ m, double* __restrict__ pos,
+                         double* __restrict__ mm, int* __restrict__ src) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p];
+  for (int d = 0; d < 3; ++d) pos[3 * p + d] = x[3 * i + d];
+  if (m) mm[p] = m[i];
+  src[p] = i;
+}
+
+// ------------------------------------------------------------ octree
+// Nodes in level order.  Geometry: centre c (FP64) and side L / 2^level.
+struct Nodes {
+  int *lo, *hi, *par, *lev, *child;  // child[8 * k + octant], -1 if empty
+  double* c;                         // [k][3]
+  double *com, *m, *q;               // [k][3], [k], [k][6]
+  int* cell;                         // cell index, -1 if none
+};
+
+__device__ inline int digit(u64 key, int shift) { return (int)((key >> shift) & 7ULL); }
+
+// First index in [a, b) whose octant digit at `shift` is >= o (digits are
+// non-decreasing inside a node).
+__device__ inline int lower_digit(const u64* key, int a, int b, int shift, int o) {
+  while (a < b) {
+    const int mid = a + (b - a) / 2;
+    if (digit(key[mid], shift) < o)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return a;
+}
+
+__global__ void k_split_count(int a, int b, Nodes nd, const u64* __restrict__ key, int split,
+                              int* __restrict__ nchild) {
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  const int lo = nd.lo[k], hi = nd.hi[k], lev = nd.lev[k];
+  int cnt = 0;
+  if (hi - lo > split && lev < kLevels) {
+    const int shift = 3 * (kLevels - 1 - lev);
+    int s0 = lo;
+    for (int o = 0; o < 8; ++o) {
+      const int s1 = o == 7 ? hi : lower_digit(key, s0, hi, shift, o + 1);
+      cnt += s1 > s0;
+      s0 = s1;
+    }
+  }
+  nchild[k - a] = cnt;
+}
+
+__global__ void k_split_fill(int a, int b, Nodes nd, const u64* __restrict__ key, int split,
+                             const int* __restrict__ off, double L) {
+#pragma clang fp contract(off)
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  const int lo = nd.lo[k], hi = nd.hi[k], lev = nd.lev[k];
+  if (!(hi - lo > split && lev < kLevels)) return;
+  const int shift = 3 * (kLevels - 1 - lev);
+  const double l = ldexp(L, -lev);  // L halved lev times, exactly
+  int id = b + off[k - a];
+  int s0 = lo;
+  for (int o = 0; o < 8; ++o) {
+    const int s1 = o == 7 ? hi : lower_digit(key, s0, hi, shift, o + 1);
+    if (s1 > s0) {
+      nd.lo[id] = s0;
+      nd.hi[id] = s1;
+      nd.par[id] = k;
+      nd.lev[id] = lev + 1;
+      for (int t = 0; t < 8; ++t) nd.child[8 * id + t] = -1;
+      // octant bits: x is bit 2, y bit 1, z bit 0
+      nd.c[3 * id + 0] = nd.c[3 * k + 0] + ((o & 4) ? 0.25 : -0.25) * l;
+      nd.c[3 * id + 1] = nd.c[3 * k + 1] + ((o & 2) ? 0.25 : -0.25) * l;
+      nd.c[3 * id + 2] = nd.c[3 * k + 2] + ((o & 1) ? 0.25 : -0.25) * l;
+      nd.child[8 * k + o] = id;
+      ++id;
+    }
+    s0 = s1;
+  }
+}
+
+// Moments of the nodes [a, b) of one level, children already done.
+__global__ void k_moments(int a, int b, Nodes nd, const double* __restrict__ pos,
+                          const double* __restrict__ mm) {
+#pragma clang fp contract(off)
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  bool leaf = true;
+  for (int o = 0; o < 8; ++o) leaf = leaf && nd.child[8 * k + o] < 0;
+  double M = 0, c[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0}, com[3];
+  if (leaf) {
+    for (int p = nd.lo[k]; p < nd.hi[k]; ++p) {
+      const double mi = mm ? mm[p] : 1.0;
+      M += mi;
+      for (int d = 0; d < 3; ++d) c[d] += mi * pos[3 * p + d];
+    }
+    for (int d = 0; d < 3; ++d) com[d] = M > 0 ? c[d] / M : nd.c[3 * k + d];
+    for (int p = nd.lo[k]; p < nd.hi[k]; ++p) {
+      const double mi = mm ? mm[p] : 1.0;
+      double dd[3];
+      for (int d = 0; d < 3; ++d) dd[d] = pos[3 * p + d] - com[d];
+      const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+      int kk = 0;
+      for (int x = 0; x < 2; ++x)
+        for (int y = x; y < 3; ++y) {
+          double v = 3.0 * dd[x] * dd[y];
+          if (x == y) v -= r2;
+          q[kk++] += mi * v;
+        }
+    }
+  } else {
+    for (int o = 0; o < 8; ++o) {
+      const int ch = nd.child[8 * k + o];
+      if (ch < 0) continue;
+      M += nd.m[ch];
+      for (int d = 0; d < 3; ++d) c[d] += nd.m[ch] * nd.com[3 * ch + d];
+    }
+    for (int d = 0; d < 3; ++d) com[d] = M > 0 ? c[d] / M : nd.c[3 * k + d];
+    for (int o = 0; o < 8; ++o) {
+      const int ch = nd.child[8 * k + o];
+      if (ch < 0) continue;
+      double dd[3];
+      for (int d = 0; d < 3; ++d) dd[d] = nd.com[3 * ch + d] - com[d];
+      const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+      int kk = 0;
+      for (int x = 0; x < 2; ++x)
+        for (int y = x; y < 3; ++y) {
+          double v = 3.0 * dd[x] * dd[y];
+          if (x == y) v -= r2;
+          q[kk] += nd.q[6 * ch + kk] + nd.m[ch] * v;
+          ++kk;
+        }
+    }
+  }
+  nd.m[k] = M;
+  for (int d = 0; d < 3; ++d) nd.com[3 * k + d] = com[d];
+  q[5] = -(q[0] + q[3]);
+  for (int t = 0; t < 6; ++t) nd.q[6 * k + t] = q[t];
+}
+
+// i-groups: nodes of <= ncrit particles whose parent holds more.
+__global__ void k_group_flag(int nn, Nodes nd, int ncrit, int* __restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const int cnt = nd.hi[k] - nd.lo[k];
+  const int par = nd.par[k];
+  flag[k] = cnt <= ncrit && (par < 0 || nd.hi[par] - nd.lo[par] > ncrit);
+}
+
+__global__ void k_compact(int n, const int* __restrict__ flag, const int* __restrict__ off,
+                          int* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && flag[k]) out[off[k]] = k;
+}
+
+__global__ void k_group_lo(int ng, const int* __restrict__ gnode, Nodes nd, int* __restrict__ lo) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ng) lo[g] = nd.lo[gnode[g]];
+}
+
+__global__ void k_group_icnt(int ng, const int* __restrict__ gnode, Nodes nd,
+                             int* __restrict__ icnt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ng) icnt[g] = nd.hi[gnode[g]] - nd.lo[gnode[g]];
+}
+
+__global__ void k_group_ilist(int ng, const int* __restrict__ gnode, Nodes nd,
+                              const int* __restrict__ ioff, int* __restrict__ ilist) {
+  const int g = blockIdx.x;
+  const int lo = nd.lo[gnode[g]], hi = nd.hi[gnode[g]];
+  for (int p = lo + threadIdx.x; p < hi; p += blockDim.x) ilist[ioff[g] + p - lo] = p;
+}
+
+// ------------------------------------------------------------ TreePM
+__global__ void k_ghost_count(int n, const double* __restrict__ pos, double rcut,
+                              int* __restrict__ cnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int c = 1;
+  for (int d = 0; d < 3; ++d) {
+    const double v = pos[3 * p + d];
+    c *= 1 + (v >= 1.0 - rcut ? 1 : 0) + (v < rcut ? 1 : 0);
+  }
+  cnt[p] = c - 1;
+}
+
+__global__ void k_ghost_fill(int n, const int* __restrict__ goff, double rcut,
+                             double* __restrict__ pos, int* __restrict__ src) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
+  int lo_s[3], hi_s[3];
+  const double xp[3] = {x0, x1, x2};
+  for (int d = 0; d < 3; ++d) {
+    lo_s[d] = xp[d] >= 1.0 - rcut ? -1 : 0;
+    hi_s[d] = xp[d] < rcut ? 1 : 0;
+  }
+  int o = n + goff[p];
+  for (int sx = lo_s[0]; sx <= hi_s[0]; ++sx)
+    for (int sy = lo_s[1]; sy <= hi_s[1]; ++sy)
+      for (int sz = lo_s[2]; sz <= hi_s[2]; ++sz) {
+        if (!sx && !sy && !sz) continue;
+        pos[3 * o + 0] = x0 + sx;
+        pos[3 * o + 1] = x1 + sy;
+        pos[3 * o + 2] = x2 + sz;
+        src[o] = src[p];
+        ++o;
+      }
+}
+
+struct Grid {
+  double glo, h;
+  int nc;
+};
+
+__device__ inline int cidx(const Grid& g, double v) {
+#pragma clang fp contract(off)
+  int c = (int)floor((v - g.glo) / g.h);
+  return c < 0 ? 0 : (c > g.nc - 1 ? g.nc - 1 : c);
+}
+
+__global__ void k_cell_of(int np, const double* __restrict__ pos, Grid g, int* __restrict__ cell,
+                          int* __restrict__ idx) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  cell[p] = (cidx(g, pos[3 * p]) * g.nc + cidx(g, pos[3 * p + 1])) * g.nc +
+            cidx(g, pos[3 * p + 2]);
+  idx[p] = p;
+}
+
+// cstart[c] = first position of cell c in the sorted cell array (c <= ncell).
+__global__ void k_cell_start(int ncell, int np, const int* __restrict__ scell,
+                             int* __restrict__ cstart) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell) return;
+  int a = 0, b = np;
+  while (a < b) {
+    const int mid = a + (b - a) / 2;
+    if (scell[mid] < c)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  cstart[c] = a;
+}
+
+// Group bounding boxes: [g][6] = min xyz, max xyz of the group's particles.
+__global__ void k_group_box(int ng, const int* __restrict__ gnode, Nodes nd,
+                            const double* __restrict__ pos, double* __restrict__ box) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  double bmin[3] = {1e30, 1e30, 1e30}, bmax[3] = {-1e30, -1e30, -1e30};
+  for (int p = nd.lo[gnode[g]]; p < nd.hi[gnode[g]]; ++p)
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = fmin(bmin[d], pos[3 * p + d]);
+      bmax[d] = fmax(bmax[d], pos[3 * p + d]);
+    }
+  for (int d = 0; d < 3; ++d) {
+    box[6 * g + d] = bmin[d];
+    box[6 * g + 3 + d] = bmax[d];
+  }
+}
+
+__device__ inline bool in_reach(const double* __restrict__ box, const double* __restrict__ v,
+                                double rc2) {
+#pragma clang fp contract(off)
+  double d2 = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double e = v[d] < box[d] ? box[d] - v[d] : (v[d] > box[3 + d] ? v[d] - box[3 + d] : 0.0);
+    d2 += e * e;
+  }
+  return d2 < rc2;
+}
+
+// One wavefront per group walks the grid cells around the group box and
+// counts (fill == false) or writes (fill == true) the particles in reach.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __restrict__ box,
+                                                      const double* __restrict__ pos, Grid gr,
+                                                      const int* __restrict__ cstart,
+                                                      const int* __restrict__ cpart, double rcut,
+                                                      int* __restrict__ jcnt,
+                                                      const int* __restrict__ joff,
+                                                      int* __restrict__ jlist) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;  // uniform per wavefront
+  const double* bx = box + 6 * g;
+  const double rc2 = rcut * rcut;
+  int c0[3], c1[3];
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = cidx(gr, bx[d] - rcut);
+    c1[d] = cidx(gr, bx[3 + d] + rcut);
+  }
+  int count = 0;
+  const int base = FILL ? joff[g] : 0;
+  for (int a = c0[0]; a <= c1[0]; ++a)
+    for (int b = c0[1]; b <= c1[1]; ++b)
+      for (int c = c0[2]; c <= c1[2]; ++c) {
+        const int cc = (a * gr.nc + b) * gr.nc + c;
+        const int k0 = cstart[cc], k1 = cstart[cc + 1];
+        for (int k = k0; k < k1; k += 64) {
+          const int kk = k + lane;
+          int p = -1;
+          bool in = false;
+          if (kk < k1) {
+            p = cpart[kk];
+            in = in_reach(bx, pos + 3 * p, rc2);
+          }
+          const u64 mask = __ballot(in);
+          if (FILL && in) {
+            const int below = __popcll(mask & ((1ULL << lane) - 1ULL));
+            jlist[base + count + below] = p;
+          }
+          count += __popcll(mask);
+        }
+      }
+  if (!FILL && lane == 0) jcnt[g] = count;
+}
+
+// ------------------------------------------------------------ Barnes-Hut
+__global__ void k_cell_flag(int nn, Nodes nd, int* __restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nn) flag[k] = nd.hi[k] - nd.lo[k] > 1;
+}
+
+__global__ void k_cell_fill(int nn, Nodes nd, const int* __restrict__ flag,
+                            const int* __restrict__ off, double* __restrict__ cells) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  if (!flag[k]) {
+    nd.cell[k] = -1;
+    return;
+  }
+  const int c = off[k];
+  nd.cell[k] = c;
+  double* o = cells + 10 * (size_t)c;
+  o[0] = nd.com[3 * k];
+  o[1] = nd.com[3 * k + 1];
+  o[2] = nd.com[3 * k + 2];
+  o[3] = nd.m[k];
+  for (int t = 0; t < 6; ++t) o[4 + t] = nd.q[6 * k + t];
+}
+
+__device__ inline double bh_box_d2(const double* __restrict__ gc, double li,
+                                   const double* __restrict__ c) {
+#pragma clang fp contract(off)
+  double d2 = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double e = fmax(fabs(c[d] - gc[d]) - li, 0.0);
+    d2 += e * e;
+  }
+  return d2;
+}
+
+// One thread per group: the depth-first walk of the host builder (children
+// visited in octant order 7..0 onto the stack), counting (FILL == false) or
+// writing the particle and cell lists.
+template <bool FILL>
+__global__ void k_bh_walk(int ng, const int* __restrict__ gnode, Nodes nd, double L, double th2,
+                          int nleaf, int* __restrict__ pcnt, int* __restrict__ ccnt,
+                          const int* __restrict__ poff, const int* __restrict__ coff,
+                          int* __restrict__ plist, int* __restrict__ clist) {
+#pragma clang fp contract(off)
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int gk = gnode[g];
+  const double gc[3] = {nd.c[3 * gk], nd.c[3 * gk + 1], nd.c[3 * gk + 2]};
+  const double li = 0.5 * ldexp(L, -nd.lev[gk]);
+  int np = 0, nc = 0;
+  int* pl = FILL ? plist + poff[g] : nullptr;
+  int* cl = FILL ? clist + coff[g] : nullptr;
+  // the whole tree as one cell (source.cpp:832-840)
+  if (bh_box_d2(gc, li, nd.com) * th2 >= L * L) {
+    if (FILL) cl[0] = nd.cell[0];
+    nc = 1;
+  } else {
+    int stack[8 * kLevels + 8];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      const int k = stack[--sp];
+      const int lo = nd.lo[k], hi = nd.hi[k];
+      if (hi - lo <= nleaf) {
+        for (int p = lo; p < hi; ++p) {
+          if (FILL) pl[np] = p;
+          ++np;
+        }
+        continue;
+      }
+      const double l = ldexp(L, -nd.lev[k]);
+      const double s2 = 0.25 * l * l;  // child side squared
+      for (int o = 7; o >= 0; --o) {
+        const int ck = nd.child[8 * k + o];
+        if (ck < 0) continue;
+        if (nd.hi[ck] - nd.lo[ck] == 1) {
+          if (FILL) pl[np] = nd.lo[ck];
+          ++np;
+        } else if (bh_box_d2(gc, li, nd.com + 3 * ck) * th2 > s2) {
+          if (FILL) cl[nc] = nd.cell[ck];
+          ++nc;
+        } else {
+          stack[sp++] = ck;
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    pcnt[g] = np;
+    ccnt[g] = nc;
+  }
+}
+
+// ------------------------------------------------------------ common tail
+// cost key: groups by decreasing icnt * (jcnt + 2 ccnt), ties by index.
+__global__ void k_cost(int ng, const int* __restrict__ icnt, const int* __restrict__ jcnt,
+                       const int* __restrict__ ccnt, u64* __restrict__ key, int* __restrict__ idx,
+                       unsigned long long* __restrict__ work) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const long long c = (long long)icnt[g] * (jcnt[g] + (ccnt ? 2 * ccnt[g] : 0));
+  key[g] = (u64)(LLONG_MAX - c);
+  idx[g] = g;
+  atomicAdd(&work[0], (unsigned long long)((long long)icnt[g] * jcnt[g]));
+  if (ccnt) atomicAdd(&work[1], (unsigned long long)((long long)icnt[g] * ccnt[g]));
+}
+
+__global__ void k_pack(int np, const double* __restrict__ pos, const int* __restrict__ src,
+                       const double* __restrict__ mass, float scale, float4* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float4 v;
+  v.x = (float)pos[3 * p] * scale;
+  v.y = (float)pos[3 * p + 1] * scale;
+  v.z = (float)pos[3 * p + 2] * scale;
+  v.w = (float)mass[src[p]];
+  out[p] = v;
+}
+
+__global__ void k_pack_cells(int nc, const double* __restrict__ cells, float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  for (int t = 0; t < 10; ++t) out[12 * (size_t)c + t] = (float)cells[10 * (size_t)c + t];
+  out[12 * (size_t)c + 10] = 0.f;
+  out[12 * (size_t)c + 11] = 0.f;
+}
+
+// ------------------------------------------------------------ host drivers
+struct Octree {
+  Nodes nd{};
+  int nn = 0;
+  std::vector<int> level_start;  // level l occupies [level_start[l], level_start[l+1])
+};
+
+// Sorted keys, particles in Morton order and the octree.  m may be null.
+void build_octree(Build& B, Lists& R, int n, const double* x, const double* m,
+                  const double lo[3], double L, int split, Octree& T, u64** skey_out,
+                  double** mm_out) {
+  hipStream_t st = B.st;
+  u64* key = B.tmp.get<u64>(n);
+  u64* skey = B.tmp.get<u64>(n);
+  int* idx = B.tmp.get<int>(n);
+  int* perm = B.tmp.get<int>(n);
+  const double s = (double)(1 << kLevels) / L;
+  hipLaunchKernelGGL(k_keys, grid_for(n), dim3(256), 0, st, n, x, lo[0], lo[1], lo[2], s, key, idx);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  B.sort_pairs(key, skey, idx, perm, n, 3 * kLevels);
+  double* mm = m ? B.tmp.get<double>(n) : nullptr;
+  hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, st, n, perm, x, m, R.pos, mm, R.src);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  // node storage: grown per level (a level holds at most 8x the previous)
+  int cap = std::max(64, 2 * (n / std::max(1, split) + 1));
+  auto alloc_nodes = [&](int c) {
+    Nodes a;
+    a.lo = B.tmp.get<int>(c);
+    a.hi = B.tmp.get<int>(c);
+    a.par = B.tmp.get<int>(c);
+    a.lev = B.tmp.get<int>(c);
+    a.child = B.tmp.get<int>(8 * (size_t)c);
+    a.c = B.tmp.get<double>(3 * (size_t)c);
+    a.com = B.tmp.get<double>(3 * (size_t)c);
+    a.m = B.tmp.get<double>(c);
+    a.q = B.tmp.get<double>(6 * (size_t)c);
+    a.cell = B.tmp.get<int>(c);
+    return a;
+  };
+  T.nd = alloc_nodes(cap);
+  {
+    const int z = 0, minus1 = -1;
+    PGAMD_HIP_CHECK(hipMemcpyAsync(T.nd.lo, &z, sizeof(int), hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(T.nd.hi, &n, sizeof(int), hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(T.nd.par, &minus1, sizeof(int), hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(T.nd.lev, &z, sizeof(int), hipMemcpyHostToDevice, st));
+    std::vector<int> ch(8, -1);
+    PGAMD_HIP_CHECK(
+        hipMemcpyAsync(T.nd.child, ch.data(), 8 * sizeof(int), hipMemcpyHostToDevice, st));
+    const double c0[3] = {lo[0] + 0.5 * L, lo[1] + 0.5 * L, lo[2] + 0.5 * L};
+    PGAMD_HIP_CHECK(
+        hipMemcpyAsync(T.nd.c, c0, 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  T.nn = 1;
+  T.level_start = {0, 1};
+  for (int lev = 0; lev < kLevels; ++lev) {
+    const int a = T.level_start[lev], b = T.level_start[lev + 1];
+    if (b == a) break;
+    int* nch = B.tmp.get<int>(b - a);
+    int* off = B.tmp.get<int>(b - a + 1);
+    hipLaunchKernelGGL(k_split_count, grid_for(b - a), dim3(256), 0, st, a, b, T.nd, skey, split,
+                       nch);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    const int add = B.scan(nch, off, b - a);
+    if (add == 0) {
+      T.level_start.push_back(b);
+      break;
+    }
+    if (b + add > cap) {  // grow: copy the nodes so far into larger arrays
+      int ncap = std::max(2 * cap, b + add);
+      Nodes g = alloc_nodes(ncap);
+      auto cp = [&](void* d, const void* s2, size_t bytes) {
+        PGAMD_HIP_CHECK(hipMemcpyAsync(d, s2, bytes, hipMemcpyDeviceToDevice, st));
+      };
+      cp(g.lo, T.nd.lo, b * sizeof(int));
+      cp(g.hi, T.nd.hi, b * sizeof(int));
+      cp(g.par, T.nd.par, b * sizeof(int));
+      cp(g.lev, T.nd.lev, b * sizeof(int));
+      cp(g.child, T.nd.child, 8 * (size_t)b * sizeof(int));
+      cp(g.c, T.nd.c, 3 * (size_t)b * sizeof(double));
+      T.nd = g;
+      cap = ncap;
+    }
+    hipLaunchKernelGGL(k_split_fill, grid_for(b - a), dim3(256), 0, st, a, b, T.nd, skey, split,
+                       off, L);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    T.level_start.push_back(b + add);
+    T.nn = b + add;
+  }
+  *skey_out = skey;
+  *mm_out = mm;
+}
+
+// Groups of the octree sorted by first particle; fills ioff/icnt/ilist.
+int build_groups(Build& B, Lists& R, const Octree& T, int ncrit, int** gnode_out) {
+  hipStream_t st = B.st;
+  const int nn = T.nn;
+  int* flag = B.tmp.get<int>(nn);
+  int* off = B.tmp.get<int>(nn + 1);
+  hipLaunchKernelGGL(k_group_flag, grid_for(nn), dim3(256), 0, st, nn, T.nd, ncrit, flag);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const int ng = B.scan(flag, off, nn);
+  int* gn = B.tmp.get<int>(ng);
+  int* gn_sorted = B.tmp.get<int>(ng);
+  int* lo = B.tmp.get<int>(ng);
+  int* lo_sorted = B.tmp.get<int>(ng);
+  hipLaunchKernelGGL(k_compact, grid_for(nn), dim3(256), 0, st, nn, flag, off, gn);
+  hipLaunchKernelGGL(k_group_lo, grid_for(ng), dim3(256), 0, st, ng, gn, T.nd, lo);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  B.sort_pairs(lo, lo_sorted, gn, gn_sorted, ng, 31);
+  R.ng = ng;
+  R.icnt = R.mem.get<int>(ng);
+  R.ioff = R.mem.get<int>(ng + 1);
+  hipLaunchKernelGGL(k_group_icnt, grid_for(ng), dim3(256), 0, st, ng, gn_sorted, T.nd, R.icnt);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  R.ni = B.scan(R.icnt, R.ioff, ng);
+  R.ilist = R.mem.get<int>(R.ni);
+  if (ng > 0)
+    hipLaunchKernelGGL(k_group_ilist, dim3(ng), dim3(256), 0, st, ng, gn_sorted, T.nd, R.ioff,
+                       R.ilist);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  *gnode_out = gn_sorted;
+  return ng;
+}
+
+void finish(Build& B, Lists& R, bool cells) {
+  hipStream_t st = B.st;
+  const int ng = (int)R.ng;
+  u64* key = B.tmp.get<u64>(ng);
+  u64* skey = B.tmp.get<u64>(ng);
+  int* idx = B.tmp.get<int>(ng);
+  unsigned long long* work = B.tmp.get<unsigned long long>(2);
+  PGAMD_HIP_CHECK(hipMemsetAsync(work, 0, 2 * sizeof(unsigned long long), st));
+  R.order = R.mem.get<int>(ng);
+  hipLaunchKernelGGL(k_cost, grid_for(ng), dim3(256), 0, st, ng, R.icnt, R.jcnt,
+                     cells ? R.ccnt : nullptr, key, idx, work);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  B.sort_pairs(key, skey, idx, R.order, ng, 64);
+  unsigned long long w[2];
+  PGAMD_HIP_CHECK(hipMemcpyAsync(w, work, sizeof(w), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  R.work_pp = (long long)w[0];
+  R.work_pc = cells ? (long long)w[1] : 0;
+}
+
+Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t st) {
+  Lists* R = new Lists();
+  Build B(st);
+  // positions: originals then ghosts (at most 7 per particle)
+  double* pos_all = R->mem.get<double>(3 * (size_t)n * 8);
+  int* src_all = R->mem.get<int>((size_t)n * 8);
+  R->pos = pos_all;
+  R->src = src_all;
+  Octree T;
+  u64* skey = nullptr;
+  double* mm = nullptr;
+  const double lo[3] = {0, 0, 0};
+  build_octree(B, *R, n, x, nullptr, lo, 1.0, std::max(1, ncrit), T, &skey, &mm);
+  int* gnode = nullptr;
+  const int ng = build_groups(B, *R, T, ncrit, &gnode);
+  // ghosts
+  int* gcnt = B.tmp.get<int>(n);
+  int* goff = B.tmp.get<int>(n + 1);
+  hipLaunchKernelGGL(k_ghost_count, grid_for(n), dim3(256), 0, st, n, R->pos, rcut, gcnt);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const int nghost = B.scan(gcnt, goff, n);
+  hipLaunchKernelGGL(k_ghost_fill, grid_for(n), dim3(256), 0, st, n, goff, rcut, R->pos, R->src);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const int np = n + nghost;
+  R->np = np;
+  // grid over [-rcut, 1 + rcut) with cells of side >= rcut
+  Grid gr;
+  gr.glo = -rcut;
+  const double gL = 1.0 + 2 * rcut;
+  gr.nc = std::max(1, (int)std::floor(gL / rcut));
+  gr.h = gL / gr.nc;
+  const int ncell = gr.nc * gr.nc * gr.nc;
+  int* cell = B.tmp.get<int>(np);
+  int* scell = B.tmp.get<int>(np);
+  int* idx = B.tmp.get<int>(np);
+  int* cpart = B.tmp.get<int>(np);
+  int* cstart = B.tmp.get<int>(ncell + 1);
+  hipLaunchKernelGGL(k_cell_of, grid_for(np), dim3(256), 0, st, np, R->pos, gr, cell, idx);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  B.sort_pairs(cell, scell, idx, cpart, np, bits_for(ncell));
+  hipLaunchKernelGGL(k_cell_start, grid_for(ncell + 1), dim3(256), 0, st, ncell, np, scell,
+                     cstart);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  // j-lists: count, offsets, fill (cell order), sort each list by particle
+  double* box = B.tmp.get<double>(6 * (size_t)std::max(ng, 1));
+  hipLaunchKernelGGL(k_group_box, grid_for(ng), dim3(256), 0, st, ng, gnode, T.nd, R->pos, box);
+  R->jcnt = R->mem.get<int>(ng);
+  R->joff = R->mem.get<int>(ng + 1);
+  hipLaunchKernelGGL((k_treepm_jlist<false>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
+                     gr, cstart, cpart, rcut, R->jcnt, nullptr, nullptr);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  R->nj = B.scan(R->jcnt, R->joff, ng);
+  int* jl = B.tmp.get<int>(R->nj);
+  R->jlist = R->mem.get<int>(R->nj);
+  hipLaunchKernelGGL((k_treepm_jlist<true>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
+                     gr, cstart, cpart, rcut, R->jcnt, R->joff, jl);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  if (R->nj > 0) {
+    size_t b = 0;
+    const unsigned* kin = reinterpret_cast<const unsigned*>(jl);
+    unsigned* kout = reinterpret_cast<unsigned*>(R->jlist);
+    PGAMD_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(
+        nullptr, b, kin, kout, (int)R->nj, ng, R->joff, R->joff + 1, 0, bits_for(np), st));
+    PGAMD_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(
+        B.cub_tmp(b), b, kin, kout, (int)R->nj, ng, R->joff, R->joff + 1, 0, bits_for(np), st));
+  }
+  R->coff = R->mem.get<int>(ng + 1);
+  R->ccnt = R->mem.get<int>(ng);
+  PGAMD_HIP_CHECK(hipMemsetAsync(R->coff, 0, sizeof(int) * (size_t)(ng + 1), st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(R->ccnt, 0, sizeof(int) * (size_t)std::max(ng, 1), st));
+  R->clist = R->mem.get<int>(1);
+  R->cells = R->mem.get<double>(10);
+  finish(B, *R, false);
+  return R;
+}
+
+Lists* bh_build(int n, const double* x, const double* m, double theta, int ncrit, int nleaf,
+                hipStream_t st) {
+  Lists* R = new Lists();
+  Build B(st);
+  // bounding box -> cube side, as the host builder
+  const int nb = 256;
+  double* part = B.tmp.get<double>(6 * nb);
+  double* bb = B.tmp.get<double>(6);
+  hipLaunchKernelGGL(k_bbox_part, dim3(nb), dim3(256), 0, st, n, x, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, nb, part, bb);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  double h[6];
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  const double lo[3] = {h[0], h[1], h[2]};
+  double L = 0;
+  for (int d = 0; d < 3; ++d) L = std::max(L, h[3 + d] - h[d]);
+  L *= 1.0 + 1e-9;
+  R->pos = R->mem.get<double>(3 * (size_t)n);
+  R->src = R->mem.get<int>(n);
+  R->np = n;
+  Octree T;
+  u64* skey = nullptr;
+  double* mm = nullptr;
+  build_octree(B, *R, n, x, m, lo, L, std::max(1, std::min(ncrit, nleaf)), T, &skey, &mm);
+  // moments, deepest level first
+  for (int lev = (int)T.level_start.size() - 2; lev >= 0; --lev) {
+    const int a = T.level_start[lev], b = T.level_start[lev + 1];
+    if (b > a)
+      hipLaunchKernelGGL(k_moments, grid_for(b - a, 64), dim3(64), 0, st, a, b, T.nd, R->pos,
+                         mm);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
+  // cells: nodes with more than one particle, in node order
+  int* flag = B.tmp.get<int>(T.nn);
+  int* off = B.tmp.get<int>(T.nn + 1);
+  hipLaunchKernelGGL(k_cell_flag, grid_for(T.nn), dim3(256), 0, st, T.nn, T.nd, flag);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  R->ncells = B.scan(flag, off, T.nn);
+  R->cells = R->mem.get<double>(10 * (size_t)R->ncells);
+  hipLaunchKernelGGL(k_cell_fill, grid_for(T.nn), dim3(256), 0, st, T.nn, T.nd, flag, off,
+                     R->cells);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  int* gnode = nullptr;
+  const int ng = build_groups(B, *R, T, ncrit, &gnode);
+  const double th2 = theta * theta;
+  R->jcnt = R->mem.get<int>(ng);
+  R->joff = R->mem.get<int>(ng + 1);
+  R->ccnt = R->mem.get<int>(ng);
+  R->coff = R->mem.get<int>(ng + 1);
+  hipLaunchKernelGGL((k_bh_walk<false>), grid_for(ng, 64), dim3(64), 0, st, ng, gnode, T.nd, L,
+                     th2, nleaf, R->jcnt, R->ccnt, nullptr, nullptr, nullptr, nullptr);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  R->nj = B.scan(R->jcnt, R->joff, ng);
+  R->nc = B.scan(R->ccnt, R->coff, ng);
+  R->jlist = R->mem.get<int>(R->nj);
+  R->clist = R->mem.get<int>(R->nc);
+  hipLaunchKernelGGL((k_bh_walk<true>), grid_for(ng, 64), dim3(64), 0, st, ng, gnode, T.nd, L,
+                     th2, nleaf, nullptr, nullptr, R->joff, R->coff, R->jlist, R->clist);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  finish(B, *R, true);
+  return R;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C ABI
+namespace {
+hipStream_t g_stream = nullptr;
+std::mutex g_mutex;
+
+hipStream_t pick_stream(void* s) {
+  if (s) return (hipStream_t)s;
+  if (!g_stream) PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return g_stream;
+}
+}  // namespace
+
+// TreePM lists of pgt_treepm_build (libpgtree.so), built on the GPU.
+// x: device [n][3] FP64 positions in the periodic unit box [0, 1).  Returns
+// a handle (null after an API error).  Synchronous on `stream` (null = the
+// library's stream).
+PGAMD_API void* pgt_dev_treepm_build(int n, const double* x, double rcut, int ncrit, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_device();
+  if (n <= 0 || !x || !(rcut > 0.0 && rcut < 0.5) || ncrit < 1) {
+    api_error(3, "pgt_dev_treepm_build: n=%d rcut=%g ncrit=%d (need n > 0, 0 < rcut < 0.5, "
+                 "ncrit >= 1, x non-null)", n, rcut, ncrit);
+    return nullptr;
+  }
+  return treepm_build(n, x, rcut, ncrit, pick_stream(stream));
+}
+
+// Barnes-Hut lists of pgt_bh_build (libpgtree.so), built on the GPU.
+// x: device [n][3], m: device [n], FP64.
+PGAMD_API void* pgt_dev_bh_build(int n, const double* x, const double* m, double theta, int ncrit,
+                                 int nleaf, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  ensure_device();
+  if (n <= 1 || !x || !m || !(theta > 0.0) || ncrit < 1 || nleaf < 1) {
+    api_error(3, "pgt_dev_bh_build: n=%d theta=%g ncrit=%d nleaf=%d (need n > 1, theta > 0, "
+                 "ncrit, nleaf >= 1, x and m non-null)", n, theta, ncrit, nleaf);
+    return nullptr;
+  }
+  return bh_build(n, x, m, theta, ncrit, nleaf, pick_stream(stream));
+}
+
+// info[8] = {np, ngroups, len(ilist), len(jlist), len(clist), ncells,
+//            sum ni*nj, sum ni*ncells}, as pgt_info.
+PGAMD_API int pgt_dev_info(void* h, long long* info) {
+  if (!h || !info) {
+    api_error(3, "pgt_dev_info: null argument");
+    return -1;
+  }
+  const Lists* r = static_cast<Lists*>(h);
+  info[0] = r->np;
+  info[1] = r->ng;
+  info[2] = r->ni;
+  info[3] = r->nj;
+  info[4] = r->nc;
+  info[5] = r->ncells;
+  info[6] = r->work_pp;
+  info[7] = r->work_pc;
+  return 0;
+}
+
+namespace {
+// Device pointer and byte size of array `what` (codes of pgt_get).
+const void* array_of(const Lists* r, int what, size_t* bytes) {
+  const size_t ng = (size_t)r->ng;
+  switch (what) {
+    case 0: *bytes = 3 * sizeof(double) * (size_t)r->np; return r->pos;
+    case 1: *bytes = sizeof(int) * (size_t)r->np; return r->src;
+    case 2: *bytes = sizeof(int) * ng; return r->ioff;
+    case 3: *bytes = sizeof(int) * ng; return r->icnt;
+    case 4: *bytes = sizeof(int) * (size_t)r->ni; return r->ilist;
+    case 5: *bytes = sizeof(int) * ng; return r->joff;
+    case 6: *bytes = sizeof(int) * ng; return r->jcnt;
+    case 7: *bytes = sizeof(int) * (size_t)r->nj; return r->jlist;
+    case 8: *bytes = sizeof(int) * ng; return r->coff;
+    case 9: *bytes = sizeof(int) * ng; return r->ccnt;
+    case 10: *bytes = sizeof(int) * (size_t)r->nc; return r->clist;
+    case 11: *bytes = sizeof(int) * ng; return r->order;
+    case 12: *bytes = 10 * sizeof(double) * (size_t)r->ncells; return r->cells;
+    default: *bytes = 0; return nullptr;
+  }
+}
+}  // namespace
+
+// Device pointer of array `what` (0 pos, 1 src, 2 ioff, 3 icnt, 4 ilist,
+// 5 joff, 6 jcnt, 7 jlist, 8 coff, 9 ccnt, 10 clist, 11 order, 12 cells),
+// valid until pgt_dev_free.
+PGAMD_API const void* pgt_dev_ptr(void* h, int what) {
+  size_t b = 0;
+  const void* p = h ? array_of(static_cast<Lists*>(h), what, &b) : nullptr;
+  if (!p) api_error(3, "pgt_dev_ptr: bad handle or array code %d", what);
+  return p;
+}
+
+// Copies array `what` to host memory `out` (sized as pgt_get's).  Synchronous.
+PGAMD_API int pgt_dev_get(void* h, int what, void* out) {
+  size_t b = 0;
+  const void* p = h ? array_of(static_cast<Lists*>(h), what, &b) : nullptr;
+  if (!p || !out) {
+    api_error(3, "pgt_dev_get: bad handle, array code %d or output", what);
+    return -1;
+  }
+  if (b) PGAMD_HIP_CHECK(hipMemcpy(out, p, b, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Packs the batch-call inputs on the device: pos4[np] = float4 {float(x) *
+// float(scale), ..., float(mass[src])} and, for Barnes-Hut lists, cells12
+// [ncells] = 12 floats {x, y, z, M, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, 0, 0}
+// (either output may be null).  mass: device [n] FP64 of the original
+// particles.  Asynchronous on `stream`.
+PGAMD_API int pgt_dev_pack(void* h, const double* mass, double scale, void* pos4, void* cells12,
+                           void* stream) {
+  if (!h || (pos4 && !mass)) {
+    api_error(3, "pgt_dev_pack: null handle or mass");
+    return -1;
+  }
+  const Lists* r = static_cast<Lists*>(h);
+  hipStream_t st = pick_stream(stream);
+  if (pos4 && r->np > 0)
+    hipLaunchKernelGGL(k_pack, grid_for(r->np), dim3(256), 0, st, (int)r->np, r->pos, r->src,
+                       mass, (float)scale, (float4*)pos4);
+  if (cells12 && r->ncells > 0)
+    hipLaunchKernelGGL(k_pack_cells, grid_for(r->ncells), dim3(256), 0, st, (int)r->ncells,
+                       r->cells, (float*)cells12);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+PGAMD_API void pgt_dev_free(void* h) {
+  if (!h) return;
+  Lists* r = static_cast<Lists*>(h);
+  PGAMD_HIP_CHECK(hipDeviceSynchronize());
+  r->mem.free_all();
+  delete r;
+}

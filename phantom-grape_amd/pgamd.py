@@ -11,6 +11,8 @@ The product is the three shared libraries built from ``csrc/``:
   (reference ``PG5/table/libpg5/gp5util.h``, ``pg5_table.h``)
 * ``libpg6.so``       GRAPE-6 API ``g6_*`` / ``g6calc_*`` + ``pg6_dev_*``
   (reference ``PG6/libpg6/gp6util.h``)
+* ``libpgtree.so`` / ``libpgtree_dev.so``  interaction lists for the batched
+  calls, built on the host (OpenMP) or on the GPU from positions in HBM
 
 This module loads them with ctypes (one private handle each, so the ``g5_*``
 names of ``libpg5.so`` and ``libpg5_table.so`` never clash) and wraps the calls
@@ -26,7 +28,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
-LIBS = ("pg5", "pg5_sym", "pg5_table", "pg6", "pgtree")
+LIBS = ("pg5", "pg5_sym", "pg5_table", "pg6", "pgtree", "pgtree_dev")
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -463,3 +465,82 @@ def bh_lists(x, m, theta=0.5, ncrit=512, nleaf=8):
     x, m = f64(x), f64(m)
     return _tree_result(L, L.pgt_bh_build(len(x), dptr(x), dptr(m), float(theta), int(ncrit),
                                           int(nleaf)))
+
+
+class DevLists:
+    """Interaction lists built on the GPU (libpgtree_dev.so) from positions
+    already in HBM; the arrays stay on the device.  ``ptr(name)`` is a device
+    pointer for the batched calls, ``host()`` copies every array into the dict
+    layout of :func:`treepm_lists` / :func:`bh_lists`."""
+
+    FIELDS = _TREE_FIELDS
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("pgt_dev build failed: " + str(last_error(self.lib())))
+        self.h = handle
+
+    @staticmethod
+    def lib():
+        L = load("pgtree_dev")
+        _sig(L, "pgt_dev_treepm_build", _vp, C.c_int, _vp, C.c_double, C.c_int, _vp)
+        _sig(L, "pgt_dev_bh_build", _vp, C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp)
+        _sig(L, "pgt_dev_info", C.c_int, _vp, _vp)
+        _sig(L, "pgt_dev_ptr", _vp, _vp, C.c_int)
+        _sig(L, "pgt_dev_get", C.c_int, _vp, C.c_int, _vp)
+        _sig(L, "pgt_dev_pack", C.c_int, _vp, _vp, C.c_double, _vp, _vp, _vp)
+        _sig(L, "pgt_dev_free", None, _vp)
+        _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
+        _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
+        return L
+
+    @classmethod
+    def treepm(cls, d_x, n, rcut, ncrit=256, stream=None):
+        """d_x: device pointer to [n][3] FP64 positions in [0, 1)."""
+        return cls(cls.lib().pgt_dev_treepm_build(int(n), d_x, float(rcut), int(ncrit), stream))
+
+    @classmethod
+    def bh(cls, d_x, d_m, n, theta=0.5, ncrit=512, nleaf=8, stream=None):
+        return cls(cls.lib().pgt_dev_bh_build(int(n), d_x, d_m, float(theta), int(ncrit),
+                                              int(nleaf), stream))
+
+    def info(self):
+        a = np.zeros(8, np.int64)
+        self.lib().pgt_dev_info(self.h, a.ctypes.data_as(C.c_void_p))
+        return dict(zip(("np", "ngroups", "ni", "nj", "nc", "ncells", "work_pp", "work_pc"),
+                        (int(v) for v in a)))
+
+    def ptr(self, name):
+        return self.lib().pgt_dev_ptr(self.h, self.FIELDS.index(name))
+
+    def host(self):
+        i = self.info()
+        np_, ng = i["np"], i["ngroups"]
+        sizes = dict(pos=(np_, 3), src=np_, ioff=ng, icnt=ng, ilist=i["ni"], joff=ng, jcnt=ng,
+                     jlist=i["nj"], coff=ng, ccnt=ng, clist=i["nc"], order=ng,
+                     cells=(i["ncells"], 10))
+        out = {}
+        for k, name in enumerate(self.FIELDS):
+            dt = np.float64 if name in ("pos", "cells") else np.int32
+            a = np.zeros(sizes[name], dt)
+            if a.size:
+                self.lib().pgt_dev_get(self.h, k, a.ctypes.data_as(C.c_void_p))
+            out[name] = a
+        out["work_pp"], out["work_pc"] = i["work_pp"], i["work_pc"]
+        return out
+
+    def pack(self, d_mass, scale, d_pos4=None, d_cells12=None, stream=None):
+        """float4 {x*scale, y*scale, z*scale, m} per list particle and 12-float cells."""
+        assert self.lib().pgt_dev_pack(self.h, d_mass, float(scale), d_pos4, d_cells12,
+                                       stream) == 0
+
+    def free(self):
+        if self.h:
+            self.lib().pgt_dev_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
