@@ -257,8 +257,11 @@ PGT_API void* pgt_treepm_build(int n, const double* x, double rcut, int ncrit) {
     r->pos.insert(r->pos.end(), {x[3 * i], x[3 * i + 1], x[3 * i + 2]});
     r->src.push_back(i);
   }
+  r->pos.reserve(3 * (size_t)n * 8);  // at most 7 ghosts per particle
+  r->src.reserve((size_t)n * 8);
   for (int p = 0; p < n; ++p) {
-    const double* xp = &r->pos[3 * p];
+    // a copy: appending to r->pos below may move its storage
+    const double xp[3] = {r->pos[3 * p], r->pos[3 * p + 1], r->pos[3 * p + 2]};
     int lo_s[3], hi_s[3];
     for (int d = 0; d < 3; ++d) {
       lo_s[d] = xp[d] >= 1.0 - rcut ? -1 : 0;

@@ -15,9 +15,13 @@ PP, PM = 0.1 / 128, 3.0 / 128
 
 
 def dev(a, dtype=None):
+    """Device copy, complete before return: the libraries run on their own
+    streams, which do not wait for torch's."""
     import torch
     a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
-    return torch.from_numpy(a).cuda()
+    t = torch.from_numpy(a).cuda()
+    torch.cuda.synchronize()
+    return t
 
 
 def ptr(t):

@@ -12,8 +12,12 @@ FIELDS = ("pos", "src", "ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "coff"
 
 
 def dev(a):
+    """Device copy, complete before return: the libraries run on their own
+    streams, which do not wait for torch's."""
     import torch
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    torch.cuda.synchronize()
+    return t
 
 
 def same_lists(h, d, cells):
@@ -33,7 +37,8 @@ def test_treepm_lists_match_host_builder(n, ncrit):
     rcut = 3.0 / 64
     m, x = ics.clumpy_box(n, 11)
     h = pgamd.treepm_lists(x, rcut, ncrit)
-    L = pgamd.DevLists.treepm(dev(x).data_ptr(), n, rcut, ncrit)
+    d_x = dev(x)
+    L = pgamd.DevLists.treepm(d_x.data_ptr(), n, rcut, ncrit)
     d = L.host()
     L.free()
     same_lists(h, d, cells=False)
@@ -44,7 +49,8 @@ def test_bh_lists_match_host_builder(n, ncrit, nleaf):
     import pgamd
     m, x = ics.hernquist(n, 12)
     h = pgamd.bh_lists(x, m, 0.5, ncrit, nleaf)
-    L = pgamd.DevLists.bh(dev(x).data_ptr(), dev(m).data_ptr(), n, 0.5, ncrit, nleaf)
+    d_x, d_m = dev(x), dev(m)
+    L = pgamd.DevLists.bh(d_x.data_ptr(), d_m.data_ptr(), n, 0.5, ncrit, nleaf)
     d = L.host()
     L.free()
     same_lists(h, d, cells=True)
@@ -57,7 +63,8 @@ def test_coincident_particles_behave_as_host_builder():
     rng = np.random.default_rng(5)
     x = np.vstack([rng.uniform(0.1, 0.9, (2000, 3)), np.full((300, 3), 0.5)])
     h = pgamd.treepm_lists(x, 0.05, 128)
-    L = pgamd.DevLists.treepm(dev(x).data_ptr(), len(x), 0.05, 128)
+    d_x = dev(x)
+    L = pgamd.DevLists.treepm(d_x.data_ptr(), len(x), 0.05, 128)
     d = L.host()
     L.free()
     same_lists(h, d, cells=False)
@@ -82,18 +89,23 @@ def test_gpu_lists_give_identical_batch_forces():
     keys = ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")
     dh = {k: dev(h[k]) for k in keys}
     acc_h = torch.zeros((len(P), 3), dtype=torch.float64, device="cuda")
+    h_pos4 = dev(pos4)
     ng = len(h["ioff"])
-    assert T.lib.pg5t_batch_force(0, ng, *[dh[k].data_ptr() for k in keys], dev(pos4).data_ptr(),
+    assert T.lib.pg5t_batch_force(0, ng, *[dh[k].data_ptr() for k in keys], h_pos4.data_ptr(),
                                   acc_h.data_ptr(), 0, None) == 0
+    torch.cuda.synchronize()
     # GPU lists, packed on the GPU
-    L = pgamd.DevLists.treepm(dev(x).data_ptr(), n, pm, 256)
+    d_x = dev(x)
+    L = pgamd.DevLists.treepm(d_x.data_ptr(), n, pm, 256)
     i = L.info()
     d_m = dev(m)
     d_pos4 = torch.empty((i["np"], 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     L.pack(d_m.data_ptr(), xs, d_pos4.data_ptr())
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d_pos4.cpu().numpy(), pos4)
     acc_d = torch.zeros((i["np"], 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     assert T.lib.pg5t_batch_force(0, i["ngroups"], *[L.ptr(k) for k in keys], d_pos4.data_ptr(),
                                   acc_d.data_ptr(), 0, None) == 0
     torch.cuda.synchronize()
