@@ -445,9 +445,23 @@ def bench_c3(args, stream, cpu):
     import pgamd
     pp, pm = 0.1 / 128, 3.0 / 128
     m, x = ics.clumpy_box(1 << 20, 20)
+    sp = stream.cuda_stream
     t0 = time.time()
-    L = pgamd.treepm_lists(x, pm, 256)
+    Lh = pgamd.treepm_lists(x, pm, 256)
     tb = time.time() - t0
+    # the same lists built on the GPU from positions in HBM (identical arrays)
+    d_x = torch.from_numpy(x).cuda()
+    d_m = torch.from_numpy(m).cuda()
+    torch.cuda.synchronize()
+    tgpu = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        DL = pgamd.DevLists.treepm(d_x.data_ptr(), len(x), pm, 256, stream=sp)
+        tgpu.append(time.perf_counter() - t0)
+        if len(tgpu) < 4:
+            DL.free()
+    L = DL.host()
+    assert all(np.array_equal(L[k], Lh[k]) for k in ("pos", "src", "ioff", "jlist", "order"))
     T = pgamd.G5Table()
     T.lib.pg5_set_table_frc_bit(6)  # 1024 entries
     T.lib.pg5_gen_s2_force_table(pp, pm)
@@ -457,22 +471,20 @@ def bench_c3(args, stream, cpu):
     ttrain = time.time() - t0
     P = L["pos"]
     mm = m[L["src"]]
-    pos4 = np.zeros((len(P), 4), np.float32)
-    pos4[:, :3] = P.astype(np.float32) * np.float32(xs)
-    pos4[:, 3] = mm
-    d = {k: torch.from_numpy(np.ascontiguousarray(L[k])).cuda()
-         for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
-    d_pos = torch.from_numpy(pos4).cuda()
+    d_pos = torch.empty((len(P), 4), dtype=torch.float32, device="cuda")
+    DL.pack(d_m.data_ptr(), xs, d_pos.data_ptr(), None, sp)
+    torch.cuda.synchronize()
+    d = {k: DL.ptr(k) for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
     acc = {k: torch.zeros((len(P), 3), dtype=torch.float64, device="cuda") for k in (0, 1)}
     ng = len(L["ioff"])
     res = {"workload": "treepm_clumpy_box_n2^20_mesh128", "groups": ng,
-           "interactions": L["work_pp"], "list_build_s": round(tb, 2),
+           "interactions": L["work_pp"], "list_build_host_s": round(tb, 3),
+           "list_build_gpu_s": round(float(np.median(tgpu[1:])), 4),
            "mlp_train_s": round(ttrain, 2), "mlp_train_loss": loss}
     sp = stream.cuda_stream
     for mode, name in ((0, "table1024"), (1, "mlp")):
-        args_ = (0, ng, d["ioff"].data_ptr(), d["icnt"].data_ptr(), d["ilist"].data_ptr(),
-                 d["joff"].data_ptr(), d["jcnt"].data_ptr(), d["jlist"].data_ptr(),
-                 d["order"].data_ptr(), d_pos.data_ptr(), acc[mode].data_ptr(), mode)
+        args_ = (0, ng, d["ioff"], d["icnt"], d["ilist"], d["joff"], d["jcnt"], d["jlist"],
+                 d["order"], d_pos.data_ptr(), acc[mode].data_ptr(), mode)
         best = np.zeros(1, np.int32)
         T.lib.pg5t_batch_autotune(*args_, 2, pgamd.iptr(best), None)
         h = T.lib.pg5t_batch_graph(*args_)
@@ -490,6 +502,7 @@ def bench_c3(args, stream, cpu):
         a = acc[mode].cpu().numpy()[sel]
         res[name]["accuracy_vs_closed_form_s2"] = _errs(a, f)
     T.lib.pg5_set_table_frc_bit(5)
+    DL.free()
     if cpu and O.ref_available("pg5table"):
         thr = _threads()
         groups = np.arange(ng)
@@ -514,25 +527,36 @@ def bench_c4(args, stream, cpu):
     import pgamd
     n, eps = 1 << 21, 1.0 / 128
     m, x = ics.hernquist(n, 21)
+    sp = stream.cuda_stream
     t0 = time.time()
-    L = pgamd.bh_lists(x, m, 0.5, 512, 8)
+    Lh = pgamd.bh_lists(x, m, 0.5, 512, 8)
     tb = time.time() - t0
+    # the same lists built on the GPU (cell numbering aside, identical arrays)
+    d_x = torch.from_numpy(x).cuda()
+    d_m = torch.from_numpy(m).cuda()
+    torch.cuda.synchronize()
+    tgpu = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        DL = pgamd.DevLists.bh(d_x.data_ptr(), d_m.data_ptr(), n, 0.5, 512, 8, stream=sp)
+        tgpu.append(time.perf_counter() - t0)
+        if len(tgpu) < 4:
+            DL.free()
+    L = DL.host()
+    assert all(np.array_equal(L[k], Lh[k]) for k in ("pos", "src", "ioff", "jlist", "order"))
     g5 = pgamd.G5()
     P = L["pos"]
     mm = m[L["src"]]
-    pos4 = np.zeros((len(P), 4), np.float32)
-    pos4[:, :3] = P
-    pos4[:, 3] = mm
-    cells = np.zeros((len(L["cells"]), 12), np.float32)
-    cells[:, :10] = L["cells"]
     keys = ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "coff", "ccnt", "clist", "order")
-    d = {k: torch.from_numpy(np.ascontiguousarray(L[k])).cuda() for k in keys}
-    d_pos = torch.from_numpy(pos4).cuda()
-    d_cells = torch.from_numpy(cells).cuda()
+    d = {k: DL.ptr(k) for k in keys}
+    d_pos = torch.empty((len(P), 4), dtype=torch.float32, device="cuda")
+    d_cells = torch.empty((len(L["cells"]), 12), dtype=torch.float32, device="cuda")
+    DL.pack(d_m.data_ptr(), 1.0, d_pos.data_ptr(), d_cells.data_ptr(), sp)
+    torch.cuda.synchronize()
     acc = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
     pot = torch.zeros(n, dtype=torch.float64, device="cuda")
     ng = len(L["ioff"])
-    args_ = (0, ng, *[d[k].data_ptr() for k in keys], d_pos.data_ptr(), d_cells.data_ptr(),
+    args_ = (0, ng, *[d[k] for k in keys], d_pos.data_ptr(), d_cells.data_ptr(),
              eps * eps, acc.data_ptr(), pot.data_ptr())
     best = np.zeros(1, np.int32)
     g5.lib.pg5_bh_autotune(*args_, 2, pgamd.iptr(best), None)
@@ -542,7 +566,8 @@ def bench_c4(args, stream, cpu):
     flops = 38 * L["work_pp"] + 71 * L["work_pc"]
     res = {"workload": "bh_hernquist_n2^21_theta0.5", "groups": ng,
            "particle_interactions": L["work_pp"], "cell_interactions": L["work_pc"],
-           "list_build_s": round(tb, 2), "ms": round(ms, 3),
+           "list_build_host_s": round(tb, 3),
+           "list_build_gpu_s": round(float(np.median(tgpu[1:])), 4), "ms": round(ms, 3),
            "ginteractions_s": round((L["work_pp"] + L["work_pc"]) / ms / 1e6, 1),
            "pct_fp32_peak": round(100 * flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
            "flops": "38/particle + 71/cell interaction", "blocks_per_cu": int(best[0])}
@@ -551,6 +576,7 @@ def bench_c4(args, stream, cpu):
     a = acc.cpu().numpy()[sel]
     ad, _ = O.pg5_force(P[sel], eps * eps, P, mm)
     res["accuracy_vs_fp64_direct"] = _errs(a, ad)
+    DL.free()
     if cpu and O.ref_available("pg5quad"):
         thr = _threads()
         t, ra, _ = O.ref_bh(L, P, mm, L["cells"], eps, thr, np.arange(ng))
