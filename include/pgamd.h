@@ -146,6 +146,24 @@ int pg6_dev_plan(int ni, int nj, int *out5);
  * number of buckets tuned, or -1.  No counterpart in the reference. */
 int pg6_dev_autotune(int nj, double eps2, int reps, int *out);
 int pg6_dev_autotune_reset(void);
+/* Block-time-step Hermite integration on the GPU: the loop of the reference's
+ * sample integrator PG6/banana (banana01.c:57-166, hermite.c:39-208,
+ * timestep.c:24-92) with its state in HBM.  It owns the GRAPE-6 board (the
+ * particles are the j-memory, address = id = index).  init: host x[n][3],
+ * v[n][3], m[n]; start-up forces and steps (eta_s); eta = corrector step
+ * accuracy; dtmax = largest step and time window.  step: one block step;
+ * *t = time after it, *nact = particles advanced.  run: block steps until
+ * t >= t_end, returns their number.  state: host copies (each may be null),
+ * t inside the current window; info[4] = {windows done, particle steps, block
+ * steps, n}.  energy: 0.5 sum m v^2, 0.5 sum m pot. */
+int pg6_hermite_init(int n, const double *x, const double *v, const double *m, double eps2,
+                     double eta_s, double eta, double dtmax);
+int pg6_hermite_step(double *t, int *nact);
+long long pg6_hermite_run(double t_end);
+int pg6_hermite_state(double *x, double *v, double *a, double *j, double *pot, double *t,
+                      double *dt, long long *info);
+int pg6_hermite_energy(double *ek, double *ep);
+int pg6_hermite_end(void);
 /* Read-back of the predicted j-particles (what the force kernel reads):
  * pos[nj][3] FP64, vel[nj][3] FP32, mass[nj] FP32 (may be NULL).  The
  * reference keeps them in its static Pred_Mem (PG6/libpg6/gravity.c:42-47). */

@@ -15,6 +15,10 @@
  *               member, g6_set_ti, then g6calc_firsthalf + g6calc_lasthalf2
  *               over chunks of g6_npipes() i-particles (PG6/banana/hermite.c:
  *               221-277); the library parallelises each call internally.
+ *   cb_hermite  the block-time-step Hermite integration of the reference's
+ *               sample program PG6/banana (banana01.c:57-166) on the
+ *               reference library, with the integrator restated in
+ *               hermite_ref.h.
  *
  * Only bench.py's cpu_baseline leg and the tests load this library.
  */
@@ -23,6 +27,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include "hermite_ref.h"
 
 typedef void (*f_open)(void);
 typedef void (*f_eps)(double);
@@ -257,4 +263,139 @@ double cb_clusters(const char *lib, int nsel, const int *sel, const int *coff,
     }
   }
   return now() - t0;
+}
+
+typedef void (*f_g6last)(int, int, int, int *, double (*)[3], double (*)[3], double, double *,
+                         double (*)[3], double (*)[3], double *);
+
+/* Integrates n particles (x, v, m at t = 0) to t_end as banana01.c does:
+ * set every particle on the board, start-up forces and steps, then block
+ * steps (global_time, predict_exmotion, firsthalf + lasthalf over chunks of
+ * 48, correct + dt_criterion, set_j_particle) with the time reset at every
+ * dtmax.  x, v are overwritten with the final state; info[3] = {particle
+ * steps, block steps, windows}; energy[2 * w] = {ek, ep} at the end of
+ * window w (up to maxw).  Returns seconds, or -1. */
+double cb_hermite(const char *lib, int n, double *x, double *v, const double *m, double eps2,
+                  double eta_s, double eta, double dtmax, double t_end, long long *info,
+                  double *energy, int maxw) {
+  void *h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+  if (!h) {
+    fprintf(stderr, "cpu_batch: %s\n", dlerror());
+    return -1.0;
+  }
+  f_g6open g6_open = (f_g6open)need(h, "g6_open");
+  f_g6setj setj = (f_g6setj)need(h, "g6_set_j_particle");
+  f_g6ti set_ti = (f_g6ti)need(h, "g6_set_ti");
+  f_g6first first = (f_g6first)need(h, "g6calc_firsthalf");
+  f_g6last last = (f_g6last)need(h, "g6calc_lasthalf");
+  if (!g6_open || !setj || !set_ti || !first || !last) return -1.0;
+  g6_open(0);
+  const int np = 48;
+  const double dtmin = dtmax / pow(2.0, 51.0);
+  double *a = calloc(3 * (size_t)n, sizeof(double)), *j = calloc(3 * (size_t)n, sizeof(double));
+  double *t = calloc((size_t)n, sizeof(double)), *dt = calloc((size_t)n, sizeof(double));
+  double *pot = calloc((size_t)n, sizeof(double));
+  double *xp = malloc(sizeof(double) * 3 * (size_t)n), *vp = malloc(sizeof(double) * 3 * (size_t)n);
+  double *a1 = malloc(sizeof(double) * 3 * (size_t)n), *j1 = malloc(sizeof(double) * 3 * (size_t)n);
+  double *p1 = malloc(sizeof(double) * (size_t)n);
+  int *act = malloc(sizeof(int) * (size_t)n);
+  double z[3] = {0, 0, 0}, h2[48];
+  memset(h2, 0, sizeof h2);
+  const double over2 = 0.5, over6 = 1.0 / 6.0;
+#define SETJ(i)                                                                        \
+  do {                                                                                 \
+    double aj[3], jj[3];                                                               \
+    for (int d = 0; d < 3; ++d) {                                                      \
+      aj[d] = a[3 * (i) + d] * over2;                                                  \
+      jj[d] = j[3 * (i) + d] * over6;                                                  \
+    }                                                                                  \
+    setj(0, (i), (i), t[i], dt[i], m[i], z, jj, aj, v + 3 * (size_t)(i), x + 3 * (size_t)(i)); \
+  } while (0)
+  const double t0 = now();
+  for (int i = 0; i < n; ++i) {
+    dt[i] = dtmax;
+    SETJ(i);
+  }
+  double t_gl = 0.0;
+  long long t_mod = 0, nsteps = 0, nblocks = 0;
+  int nact = n;
+  for (int i = 0; i < n; ++i) act[i] = i;
+  int startup = 1;
+  int w = 0;
+  for (;;) {
+    set_ti(0, t_gl);
+    for (int k = 0; k < nact; ++k) {
+      const int i = act[k];
+      hr_predict(t_gl, t[i], x + 3 * i, v + 3 * i, a + 3 * i, j + 3 * i, xp + 3 * k, vp + 3 * k);
+    }
+    for (int o = 0; o < nact; o += np) {
+      const int ni = nact - o < np ? nact - o : np;
+      first(0, n, ni, act + o, (double(*)[3])(xp + 3 * o), (double(*)[3])(vp + 3 * o),
+            (double(*)[3])(a1 + 3 * o), (double(*)[3])(j1 + 3 * o), p1 + o, eps2, h2);
+      last(0, n, ni, act + o, (double(*)[3])(xp + 3 * o), (double(*)[3])(vp + 3 * o), eps2, h2,
+           (double(*)[3])(a1 + 3 * o), (double(*)[3])(j1 + 3 * o), p1 + o);
+    }
+    for (int k = 0; k < nact; ++k) {
+      const int i = act[k];
+      if (startup) {
+        memcpy(a + 3 * i, a1 + 3 * k, sizeof(double) * 3);
+        memcpy(j + 3 * i, j1 + 3 * k, sizeof(double) * 3);
+        dt[i] = hr_dt_startup(eta_s, dtmax, dtmin, t[i], a + 3 * i, j + 3 * i);
+      } else {
+        double x1[3], v1[3];
+        const double dtn = hr_correct(eta, dtmax, dtmin, t[i], dt[i], a + 3 * i, j + 3 * i,
+                                      xp + 3 * k, vp + 3 * k, a1 + 3 * k, j1 + 3 * k, x1, v1);
+        memcpy(x + 3 * i, x1, sizeof x1);
+        memcpy(v + 3 * i, v1, sizeof v1);
+        memcpy(a + 3 * i, a1 + 3 * k, sizeof(double) * 3);
+        memcpy(j + 3 * i, j1 + 3 * k, sizeof(double) * 3);
+        t[i] = t[i] + dt[i];
+        dt[i] = dtn;
+      }
+      pot[i] = p1[k];
+      SETJ(i);
+    }
+    nsteps += nact;
+    nblocks += 1;
+    startup = 0;
+    if (t_gl == dtmax) {  /* end of a window: energy, reset_time, set_particle_on_grape */
+      if (w < maxw && energy) {
+        double ek = 0.0, ew = 0.0;
+        for (int i = 0; i < n; ++i) {
+          double v2 = v[3 * i] * v[3 * i];
+          v2 += v[3 * i + 1] * v[3 * i + 1];
+          v2 += v[3 * i + 2] * v[3 * i + 2];
+          ek += m[i] * v2;
+          ew += m[i] * pot[i];
+        }
+        energy[2 * w] = 0.5 * ek;
+        energy[2 * w + 1] = 0.5 * ew;
+      }
+      ++w;
+      t_gl = 0.0;
+      ++t_mod;
+      for (int i = 0; i < n; ++i) {
+        t[i] = 0.0;
+        SETJ(i);
+      }
+    }
+    if ((double)t_mod * dtmax + t_gl >= t_end) break;
+    /* global_time: the particles with the smallest t + dt, in index order */
+    double tn = 1e300;
+    for (int i = 0; i < n; ++i) tn = fmin(tn, t[i] + dt[i]);
+    nact = 0;
+    for (int i = 0; i < n; ++i)
+      if (t[i] + dt[i] == tn) act[nact++] = i;
+    t_gl = tn;
+  }
+#undef SETJ
+  const double sec = now() - t0;
+  if (info) {
+    info[0] = nsteps;
+    info[1] = nblocks;
+    info[2] = w;
+  }
+  free(a); free(j); free(t); free(dt); free(pot); free(xp); free(vp); free(a1); free(j1);
+  free(p1); free(act);
+  return sec;
 }

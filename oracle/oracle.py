@@ -73,6 +73,12 @@ def lib() -> C.CDLL:
         _sig(L, "or_quad_force", None, C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
              _dp)
         _sig(L, "or_quad_moments", None, C.c_int, _dp, _dp, _dp, _dp, _dp)
+        _sig(L, "or_calc_modulus", C.c_double, C.c_double, C.c_double, C.c_double)
+        _sig(L, "or_dt_startup", None, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp)
+        _sig(L, "or_hermite_predict", None, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
+             _dp)
+        _sig(L, "or_hermite_correct", None, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+             _dp, _dp, _dp, _dp, _dp, _dp, _dp)
         _lib = L
     return _lib
 
@@ -176,6 +182,39 @@ def quad_moments(x, m):
     q = np.zeros(6)
     lib().or_quad_moments(len(m), _P(x), _P(m), _P(com), _P(mass), _P(q))
     return com, float(mass[0]), q
+
+
+# ------------------------------------------------------- Hermite integrator
+def calc_modulus(num, mod, modmax):
+    """timestep.c:176-190."""
+    return lib().or_calc_modulus(float(num), float(mod), float(modmax))
+
+
+def dt_startup(eta_s, dtmax, t_gl, a, j):
+    """timestep.c:24-48 for every particle (0 where no step fits)."""
+    a, j = _d(a), _d(j)
+    dt = np.zeros(len(a))
+    lib().or_dt_startup(len(a), float(eta_s), float(dtmax), float(t_gl), _P(a), _P(j), _P(dt))
+    return dt
+
+
+def hermite_predict(t_gl, t, x, v, a, j):
+    """predict_exmotion, hermite.c:39-62."""
+    t, x, v, a, j = map(_d, (t, x, v, a, j))
+    xp, vp = np.zeros_like(x), np.zeros_like(v)
+    lib().or_hermite_predict(len(t), float(t_gl), _P(t), _P(x), _P(v), _P(a), _P(j), _P(xp),
+                             _P(vp))
+    return xp, vp
+
+
+def hermite_correct(eta, dtmax, t, dt, a0, j0, xp, vp, a1, j1):
+    """correct (hermite.c:66-110) + dt_criterion (timestep.c:50-79): returns
+    corrected x, v and the new steps."""
+    t, dt, a0, j0, xp, vp, a1, j1 = map(_d, (t, dt, a0, j0, xp, vp, a1, j1))
+    x1, v1, dtn = np.zeros_like(xp), np.zeros_like(vp), np.zeros(len(t))
+    lib().or_hermite_correct(len(t), float(eta), float(dtmax), _P(t), _P(dt), _P(a0), _P(j0),
+                             _P(xp), _P(vp), _P(a1), _P(j1), _P(x1), _P(v1), _P(dtn))
+    return x1, v1, dtn
 
 
 # ------------------------------------------------ the compiled reference
@@ -356,6 +395,9 @@ def _cpubatch() -> C.CDLL:
              _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp)
         _sig(L, "cb_clusters", C.c_double, C.c_char_p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
              _dp, _dp, _dp)
+        _sig(L, "cb_hermite", C.c_double, C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_double,
+             C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_longlong), _dp,
+             C.c_int)
         _cb = L
     return _cb
 
@@ -424,3 +466,20 @@ def ref_clusters(sel, coff, pos, vel, mass, eps2):
     if t < 0:
         raise RuntimeError("cb_clusters failed")
     return t, acc, jrk, pot
+
+
+def ref_hermite(x, v, m, eps2, eta_s, eta, dtmax, t_end, maxw=64):
+    """The reference's sample Hermite integration (PG6/banana) on the compiled
+    reference library.  Returns (seconds, x, v, info{steps, blocks, windows},
+    energies[w] = (ek, ep) at each window end)."""
+    x, v, m = _d(x).copy(), _d(v).copy(), _d(m)
+    info = (C.c_longlong * 3)()
+    en = np.zeros(2 * maxw)
+    t = _cpubatch().cb_hermite(_refpath("pg6"), len(m), _P(x), _P(v), _P(m), float(eps2),
+                               float(eta_s), float(eta), float(dtmax), float(t_end), info,
+                               _P(en), int(maxw))
+    if t < 0:
+        raise RuntimeError("cb_hermite failed")
+    w = int(info[2])
+    return t, x, v, dict(steps=int(info[0]), blocks=int(info[1]), windows=w), \
+        en[:2 * min(w, maxw)].reshape(-1, 2)

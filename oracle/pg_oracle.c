@@ -8,6 +8,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include "hermite_ref.h"
 
 /* ---------------------------------------------------------------- GRAPE-5 */
 /* PG5/newton/libpg5/gravity_avx2.c:95-154: dx = xj - xi, r2 = |dx|^2 + eps2,
@@ -336,4 +337,34 @@ void or_quad_moments(int n, const double *x, const double *m, double *com,
   memcpy(com, c, sizeof(c));
   *mass = M;
   memcpy(q6, q, sizeof(q));
+}
+
+
+/* ------------------------------------------------------------------ Hermite
+ * Vectorised entry points over hermite_ref.h (PG6/banana/hermite.c:39-110,
+ * timestep.c:24-79,176-190). */
+double or_calc_modulus(double num, double mod, double modmax) {
+  return hr_calc_modulus(num, mod, modmax);
+}
+
+void or_dt_startup(int n, double eta_s, double dtmax, double t_gl, const double *a,
+                   const double *j, double *dt) {
+  const double dtmin = dtmax / pow(2.0, 51.0);
+  for (int i = 0; i < n; ++i) dt[i] = hr_dt_startup(eta_s, dtmax, dtmin, t_gl, a + 3 * i, j + 3 * i);
+}
+
+void or_hermite_predict(int n, double t_gl, const double *t, const double *x, const double *v,
+                        const double *a, const double *j, double *xp, double *vp) {
+  for (int i = 0; i < n; ++i)
+    hr_predict(t_gl, t[i], x + 3 * i, v + 3 * i, a + 3 * i, j + 3 * i, xp + 3 * i, vp + 3 * i);
+}
+
+void or_hermite_correct(int n, double eta, double dtmax, const double *t, const double *dt,
+                        const double *a0, const double *j0, const double *xp, const double *vp,
+                        const double *a1, const double *j1, double *x1, double *v1,
+                        double *dtn) {
+  const double dtmin = dtmax / pow(2.0, 51.0);
+  for (int i = 0; i < n; ++i)
+    dtn[i] = hr_correct(eta, dtmax, dtmin, t[i], dt[i], a0 + 3 * i, j0 + 3 * i, xp + 3 * i,
+                        vp + 3 * i, a1 + 3 * i, j1 + 3 * i, x1 + 3 * i, v1 + 3 * i);
 }

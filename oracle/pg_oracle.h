@@ -101,6 +101,19 @@ void or_quad_force(int ni, const double *xi, double eps2, int nc,
 void or_quad_moments(int n, const double *x, const double *m, double *com,
                      double *mass, double *q6);
 
+/* Block-time-step Hermite pieces of the reference's sample integrator
+ * (PG6/banana/hermite.c:39-110, timestep.c:24-79,176-190), restated in
+ * hermite_ref.h; arrays [n][3] or [n]. */
+double or_calc_modulus(double num, double mod, double modmax);
+void or_dt_startup(int n, double eta_s, double dtmax, double t_gl, const double *a,
+                   const double *j, double *dt);
+void or_hermite_predict(int n, double t_gl, const double *t, const double *x, const double *v,
+                        const double *a, const double *j, double *xp, double *vp);
+void or_hermite_correct(int n, double eta, double dtmax, const double *t, const double *dt,
+                        const double *a0, const double *j0, const double *xp, const double *vp,
+                        const double *a1, const double *j1, double *x1, double *v1,
+                        double *dtn);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,8 +17,10 @@
 #include "pg6_kernels.h"
 #include "pg6_nblist.h"
 #include "cluster_batch.h"
+#include "hermite_kernels.h"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 using namespace pgamd;
@@ -1144,5 +1146,307 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
 PGAMD_API int pg6_dev_autotune_reset(void) {
   std::lock_guard<std::mutex> lk(g_mutex);
   for (int k = 0; k < kBuckets; ++k) g_board.tuned_ok[k] = false;
+  return 0;
+}
+
+// ======================================================================
+// Block-time-step Hermite integration on the GPU (the reference's sample
+// integrator PG6/banana, banana01.c:57-166, on the device).  It owns the
+// board: its particles are the j-memory (address = id = index).
+namespace {
+
+struct Hermite {
+  bool on = false;
+  int n = 0;
+  double eps2 = 0.0;
+  pg6h::Params P{};
+  double t_gl = 0.0;   // time inside the current window [0, dtmax]
+  long long t_mod = 0; // completed windows (globaltime.h: t = t_mod * dtmax + t_gl)
+  long long nsteps = 0, nblocks = 0;
+  DevBuf state, work;
+  pg6h::State S{};
+  // work arrays
+  int* act = nullptr;
+  double *ipos = nullptr, *ivel = nullptr, *acc1 = nullptr, *jrk1 = nullptr, *pot1 = nullptr;
+  int* iid = nullptr;
+  int *bcount = nullptr, *boff = nullptr, *scal_i = nullptr;  // scal_i[0] = nact, [1] = bad
+  double *part = nullptr, *scal_d = nullptr;                    // scal_d[0] = t_next
+  HostBuf host;  // pinned {t_next, nact, bad}
+};
+
+Hermite g_h;
+
+constexpr int kHermBlocks = 256;  // blocks of the min / energy reductions
+
+void hermite_release(Hermite& h) {
+  h.state.release();
+  h.work.release();
+  h.host.release();
+  h = Hermite();
+}
+
+// Runs the force on the first `ni` entries of ipos/ivel/iid against all n
+// j-particles (predicted to `ti` beforehand) into acc1/jrk1/pot1.
+void hermite_force(Board& b, Hermite& h, int ni) {
+  const Outputs o{h.acc1, h.jrk1, h.pot1, nullptr, nullptr};
+  board_force(b, ni, h.n, h.ipos, h.ivel, h.iid, h.eps2, o, b.stream);
+}
+
+}  // namespace
+
+// Starts an integration of n particles (host arrays x[n][3], v[n][3], m[n],
+// FP64) at t = 0: uploads them as the j-particles, computes every force, jerk
+// and potential, and the first block time steps (dt_startup with eta_s;
+// timestep.c:24-48).  eps2 = softening^2 (0 allowed), eta = accuracy of the
+// corrector's step criterion (timestep.c:50-79), dtmax = largest step (also
+// the window after which times restart at 0).  Returns 0, or -1 on bad
+// arguments or when a first step would fall below dtmax / 2^51.
+PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const double* m,
+                               double eps2, double eta_s, double eta, double dtmax) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_hermite_init")) return -1;
+  if (n < 2 || !x || !v || !m || !(eps2 >= 0.0) || !(eta_s > 0.0) || !(eta > 0.0) ||
+      !(dtmax > 0.0)) {
+    api_error(3, "pg6_hermite_init: n=%d eps2=%g eta_s=%g eta=%g dtmax=%g", n, eps2, eta_s, eta,
+              dtmax);
+    return -1;
+  }
+  Board& b = g_board;
+  if (!board_reserve(b, n)) return -1;
+  hermite_release(g_h);
+  Hermite& h = g_h;
+  h.n = n;
+  h.eps2 = eps2;
+  h.P.eta_s = eta_s;
+  h.P.eta = eta;
+  h.P.dtmax = dtmax;
+  h.P.dtmin = dtmax / std::pow(2.0, 51.0);  // initialize_timestep (timestep.c:18-22)
+  // state: x, v, a, j (3n each), pot, t, dt, m (n each)
+  h.state.reserve(sizeof(double) * (size_t)n * 16);
+  double* d = h.state.as<double>();
+  h.S.x = d;
+  h.S.v = d + 3 * (size_t)n;
+  h.S.a = d + 6 * (size_t)n;
+  h.S.j = d + 9 * (size_t)n;
+  h.S.pot = d + 12 * (size_t)n;
+  h.S.t = d + 13 * (size_t)n;
+  h.S.dt = d + 14 * (size_t)n;
+  h.S.m = d + 15 * (size_t)n;
+  // work: ipos, ivel, acc1, jrk1 (3n each), pot1 (n), part (2 * blocks), scal_d (4);
+  // act, iid (n each), bcount, boff (blocks of 256 + 1), scal_i (4)
+  const int nb = ceil_div(n, 256);
+  const size_t dbytes = sizeof(double) * ((size_t)n * 13 + 2 * kHermBlocks + 4);
+  const size_t ibytes = sizeof(int) * ((size_t)n * 2 + 2 * (size_t)nb + 1 + 4);
+  h.work.reserve(dbytes + ibytes);
+  double* w = h.work.as<double>();
+  h.ipos = w;
+  h.ivel = w + 3 * (size_t)n;
+  h.acc1 = w + 6 * (size_t)n;
+  h.jrk1 = w + 9 * (size_t)n;
+  h.pot1 = w + 12 * (size_t)n;
+  h.part = w + 13 * (size_t)n;
+  h.scal_d = h.part + 2 * kHermBlocks;
+  int* wi = reinterpret_cast<int*>(h.scal_d + 4);
+  h.act = wi;
+  h.iid = wi + n;
+  h.bcount = wi + 2 * (size_t)n;
+  h.boff = h.bcount + nb;
+  h.scal_i = h.boff + nb + 1;
+  h.host.reserve(64);
+  hipStream_t st = b.stream;
+  // upload x, v, m; t = 0, a = j = 0
+  std::vector<double> zero((size_t)n * 3, 0.0);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.x, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.v, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.m, m, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.S.a, 0, sizeof(double) * 6 * (size_t)n, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.S.t, 0, sizeof(double) * (size_t)n, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.S.dt, 0, sizeof(double) * (size_t)n, st));
+  // j-memory = the particles at t = 0 (set_particle_on_grape, banana01.c:206-236):
+  // the start-up kernel below writes it; the i-set is every particle as is.
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.ipos, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.ivel, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  std::vector<int> ids(n);
+  for (int i = 0; i < n; ++i) ids[i] = i;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.iid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.scal_i, 0, sizeof(int) * 4, st));
+  // j-state of the board from the particles (a = j = 0, t = 0), predicted to 0
+  b.npend = 0;
+  std::fill(b.slot.begin(), b.slot.end(), -1);
+  hipLaunchKernelGGL(pg6h::pg6h_store_all, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S,
+                     b.js);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  b.ti = 0.0;
+  board_predict(b, n, st);
+  // start-up forces of every particle, then the first steps (startup_sgmotion)
+  hermite_force(b, h, n);
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.scal_i, 0, sizeof(int) * 4, st));
+  hipLaunchKernelGGL(pg6h::pg6h_startup, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
+                     h.acc1, h.jrk1, h.pot1, b.js, h.scal_i + 1);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  int bad = 0;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(&bad, h.scal_i + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  b.predict_flag = 1;
+  b.nj_pred = -1;
+  h.on = true;
+  h.nsteps = n;
+  h.nblocks = 1;
+  if (bad) {
+    api_error(6, "pg6_hermite_init: %d particles have no admissible first step", bad);
+    return -1;
+  }
+  return 0;
+}
+
+// One block step (global_time + integrate_exmotion + correct_sgmotion of
+// banana01.c:120-166): the particles with the smallest t + dt advance to it.
+// After a step that ends a dtmax window every t restarts at 0 (reset_time).
+// *t_out = t_mod * dtmax + t_gl after the step, *nact = active particles.
+PGAMD_API int pg6_hermite_step(double* t_out, int* nact_out) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_hermite_step")) return -1;
+  Hermite& h = g_h;
+  if (!h.on) {
+    api_error(1, "pg6_hermite_step: no integration started (pg6_hermite_init)");
+    return -1;
+  }
+  Board& b = g_board;
+  hipStream_t st = b.stream;
+  const int n = h.n;
+  const int nb = ceil_div(n, 256);
+  hipLaunchKernelGGL(pg6h::pg6h_next_time_part, dim3(kHermBlocks), dim3(256), 0, st, n, h.S,
+                     h.part);
+  hipLaunchKernelGGL(pg6h::pg6h_next_time_final, dim3(1), dim3(256), 0, st, kHermBlocks, h.part,
+                     h.scal_d);
+  hipLaunchKernelGGL(pg6h::pg6h_count, dim3(nb), dim3(256), 0, st, n, h.S, h.scal_d, h.bcount);
+  hipLaunchKernelGGL(pg6h::pg6h_scan, dim3(1), dim3(64), 0, st, nb, h.bcount, h.boff, h.scal_i);
+  hipLaunchKernelGGL(pg6h::pg6h_compact, dim3(nb), dim3(256), 0, st, n, h.S, h.scal_d, h.boff,
+                     h.act);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  double* hp = h.host.as<double>();
+  PGAMD_HIP_CHECK(hipMemcpyAsync(hp, h.scal_d, sizeof(double), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(
+      hipMemcpyAsync(hp + 1, h.scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  const double tnext = hp[0];
+  int nact = 0;
+  std::memcpy(&nact, hp + 1, sizeof(int));
+  if (nact <= 0 || nact > n) {
+    api_error(6, "pg6_hermite_step: no active particle (t_next = %g)", tnext);
+    return -1;
+  }
+  // predict all j to t_next, the active i as predict_exmotion does, force, correct
+  b.ti = tnext;
+  board_predict(b, n, st);
+  b.predict_flag = 0;
+  hipLaunchKernelGGL(pg6h::pg6h_predict_i, dim3(ceil_div(nact, 256)), dim3(256), 0, st, nact,
+                     h.act, h.S, h.scal_d, h.ipos, h.ivel, h.iid);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  hermite_force(b, h, nact);
+  hipLaunchKernelGGL(pg6h::pg6h_correct, dim3(ceil_div(nact, 256)), dim3(256), 0, st, nact,
+                     h.act, h.S, h.P, h.ipos, h.ivel, h.acc1, h.jrk1, h.pot1, b.js,
+                     h.scal_i + 1);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  h.t_gl = tnext;
+  h.nsteps += nact;
+  h.nblocks += 1;
+  if (h.t_gl == h.P.dtmax) {  // end of a window: every particle is synchronised
+    hipLaunchKernelGGL(pg6h::pg6h_reset, dim3(nb), dim3(256), 0, st, n, h.S, b.js);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    h.t_gl = 0.0;
+    h.t_mod += 1;
+  }
+  b.nj_pred = -1;
+  if (t_out) *t_out = (double)h.t_mod * h.P.dtmax + h.t_gl;
+  if (nact_out) *nact_out = nact;
+  return 0;
+}
+
+// Block steps until t = t_mod * dtmax + t_gl reaches t_end.  Returns the
+// number of block steps taken, or -1.
+PGAMD_API long long pg6_hermite_run(double t_end) {
+  long long k = 0;
+  for (;;) {
+    double t = 0.0;
+    {
+      std::lock_guard<std::mutex> lk(g_mutex);
+      if (!g_h.on) {
+        api_error(1, "pg6_hermite_run: no integration started (pg6_hermite_init)");
+        return -1;
+      }
+      t = (double)g_h.t_mod * g_h.P.dtmax + g_h.t_gl;
+    }
+    if (t >= t_end) return k;
+    if (pg6_hermite_step(nullptr, nullptr) != 0) return -1;
+    ++k;
+  }
+}
+
+// Copies the integrator state to host arrays (each may be null): x, v, a, j
+// [n][3]; pot, t, dt [n] (t inside the current window); info[4] =
+// {t_mod, n_steps (particle steps), n_blocks, n}.  Synchronous.
+PGAMD_API int pg6_hermite_state(double* x, double* v, double* a, double* j, double* pot,
+                                double* t, double* dt, long long* info) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  Hermite& h = g_h;
+  if (!h.on) {
+    api_error(1, "pg6_hermite_state: no integration started (pg6_hermite_init)");
+    return -1;
+  }
+  hipStream_t st = g_board.stream;
+  const size_t n = (size_t)h.n;
+  auto get = [&](double* out, const double* src, size_t cnt) {
+    if (out)
+      PGAMD_HIP_CHECK(hipMemcpyAsync(out, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+  };
+  get(x, h.S.x, 3 * n);
+  get(v, h.S.v, 3 * n);
+  get(a, h.S.a, 3 * n);
+  get(j, h.S.j, 3 * n);
+  get(pot, h.S.pot, n);
+  get(t, h.S.t, n);
+  get(dt, h.S.dt, n);
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  if (info) {
+    info[0] = h.t_mod;
+    info[1] = h.nsteps;
+    info[2] = h.nblocks;
+    info[3] = h.n;
+  }
+  return 0;
+}
+
+// Kinetic and potential energy, 0.5 sum m v^2 and 0.5 sum m pot (the
+// reference's calc_energy.c:8-33; meaningful when every particle is
+// synchronised, i.e. at t_gl = 0).
+PGAMD_API int pg6_hermite_energy(double* ek, double* ep) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  Hermite& h = g_h;
+  if (!h.on) {
+    api_error(1, "pg6_hermite_energy: no integration started (pg6_hermite_init)");
+    return -1;
+  }
+  hipStream_t st = g_board.stream;
+  hipLaunchKernelGGL(pg6h::pg6h_energy_part, dim3(kHermBlocks), dim3(256), 0, st, h.n, h.S,
+                     h.part);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  std::vector<double> p(2 * kHermBlocks);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), h.part, sizeof(double) * p.size(),
+                                 hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  double k = 0.0, w = 0.0;
+  for (int b = 0; b < kHermBlocks; ++b) {
+    k += p[2 * b];
+    w += p[2 * b + 1];
+  }
+  if (ek) *ek = 0.5 * k;
+  if (ep) *ep = 0.5 * w;
+  return 0;
+}
+
+PGAMD_API int pg6_hermite_end(void) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  hermite_release(g_h);
   return 0;
 }

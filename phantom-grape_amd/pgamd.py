@@ -240,6 +240,14 @@ class G6:
         _sig(L, "pg6_dev_set_kernel", C.c_int, C.c_int)
         _sig(L, "pg6_dev_autotune", C.c_int, C.c_int, C.c_double, C.c_int, _ip)
         _sig(L, "pg6_dev_autotune_reset", C.c_int)
+        _sig(L, "pg6_hermite_init", C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+             C.c_double, C.c_double)
+        _sig(L, "pg6_hermite_step", C.c_int, _dp, _ip)
+        _sig(L, "pg6_hermite_run", C.c_longlong, C.c_double)
+        _sig(L, "pg6_hermite_state", C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+             C.POINTER(C.c_longlong))
+        _sig(L, "pg6_hermite_energy", C.c_int, _dp, _dp)
+        _sig(L, "pg6_hermite_end", C.c_int)
         _sig(L, "pg6_dev_timing_start", C.c_int, C.c_int)
         _sig(L, "pg6_dev_timing_read", C.c_int, _dp, _ip)
         _sig(L, "pg6_cluster_items", C.c_int, C.c_int, _ip, _ip, C.c_int)
@@ -319,6 +327,43 @@ class G6:
         o = np.zeros(5, np.int32)
         self.lib.pg6_dev_plan(ni, nj, iptr(o))
         return dict(zip(("variant", "bi", "bpc", "nsplit", "jper"), (int(v) for v in o)))
+
+    # ------------------------------------------------ Hermite integration
+    def hermite_init(self, x, v, m, eps2, eta_s, eta, dtmax):
+        x, v, m = f64(x), f64(v), f64(m)
+        self._hn = len(m)
+        rc = self.lib.pg6_hermite_init(len(m), dptr(x), dptr(v), dptr(m), float(eps2),
+                                       float(eta_s), float(eta), float(dtmax))
+        if rc != 0:
+            raise RuntimeError(last_error(self.lib))
+
+    def hermite_step(self):
+        t = np.zeros(1)
+        k = np.zeros(1, np.int32)
+        if self.lib.pg6_hermite_step(dptr(t), iptr(k)) != 0:
+            raise RuntimeError(last_error(self.lib))
+        return float(t[0]), int(k[0])
+
+    def hermite_run(self, t_end):
+        k = self.lib.pg6_hermite_run(float(t_end))
+        if k < 0:
+            raise RuntimeError(last_error(self.lib))
+        return int(k)
+
+    def hermite_state(self):
+        n = self._hn
+        out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), a=np.zeros((n, 3)),
+                   j=np.zeros((n, 3)), pot=np.zeros(n), t=np.zeros(n), dt=np.zeros(n))
+        info = (C.c_longlong * 4)()
+        assert self.lib.pg6_hermite_state(*[dptr(out[k]) for k in ("x", "v", "a", "j", "pot", "t",
+                                                                    "dt")], info) == 0
+        out.update(t_mod=int(info[0]), steps=int(info[1]), blocks=int(info[2]))
+        return out
+
+    def hermite_energy(self):
+        ek, ep = np.zeros(1), np.zeros(1)
+        assert self.lib.pg6_hermite_energy(dptr(ek), dptr(ep)) == 0
+        return float(ek[0]), float(ep[0])
 
     def autotune(self, nj, eps2, reps=3):
         """Runs pg6_dev_autotune; returns {ni: (variant, bi, bpc, us)}."""

@@ -41,7 +41,7 @@ def declared_symbols():
             for stmt in body.split(";"):
                 stmt = " ".join(stmt.split())
                 md = re.match(r"extern\s+\w+\s+(\w+)\s*\[", stmt)
-                mf = re.match(r"(?:const\s+)?\w+[\s\*]+(\w+)\s*\(", stmt)
+                mf = re.match(r"(?:const\s+)?(?:\w+\s+)*\w+[\s\*]+(\w+)\s*\(", stmt)
                 name = md.group(1) if md else (mf.group(1) if mf else None)
                 if name:
                     for lib in libs:

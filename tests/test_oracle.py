@@ -399,3 +399,95 @@ def test_cpu_cluster_driver_matches_reference_sessions():
         assert np.percentile(rel_err(acc[s], o["acc"]), 99) < 1e-6
         assert np.percentile(rel_err(jrk[s], o["jrk"]), 99) < 1e-4
     assert np.all(acc[coff[1]:coff[2]] == 0)
+
+
+# --------------------------------------------- Hermite integrator restatement
+def test_calc_modulus_is_zero_exactly_on_the_block_grid():
+    dtmax = 0.125
+    for k in range(1, 9):
+        mod = dtmax / 2 ** k
+        for m in range(1, 2 ** k + 1):
+            assert O.calc_modulus(m * mod, mod, dtmax) == 0.0
+            if m * mod + mod / 2 <= dtmax:
+                assert O.calc_modulus(m * mod + mod / 2, mod, dtmax) != 0.0
+    assert O.calc_modulus(0.0, 1e-3, dtmax) == 0.0
+
+
+def test_hermite_predictor_is_the_taylor_cubic():
+    rng = np.random.default_rng(1)
+    x, v, a, j = (rng.normal(size=(50, 3)) for _ in range(4))
+    t = rng.uniform(0, 0.1, 50)
+    tg = 0.125
+    xp, vp = O.hermite_predict(tg, t, x, v, a, j)
+    h = (tg - t)[:, None]
+    np.testing.assert_allclose(xp, x + v * h + a * h ** 2 / 2 + j * h ** 3 / 6, rtol=1e-14,
+                               atol=1e-15)
+    np.testing.assert_allclose(vp, v + a * h + j * h ** 2 / 2, rtol=1e-14, atol=1e-15)
+
+
+def test_hermite_corrector_is_exact_for_quintic_motion():
+    """The 4th-order Hermite corrector reproduces x(t), v(t) exactly when the
+    acceleration is a cubic in t (x a quintic): a closed-form check."""
+    rng = np.random.default_rng(2)
+    c = rng.normal(size=(6, 20, 3))  # x(t) = sum c_k t^k
+
+    def deriv(k, t):
+        out = 0.0
+        for p in range(k, 6):
+            f = np.prod(np.arange(p - k + 1, p + 1)) if k else 1.0
+            out = out + f * c[p] * t ** (p - k)
+        return out
+    dt = 0.03125
+    x0, v0, a0, j0 = (deriv(k, 0.0) for k in range(4))
+    a1, j1 = deriv(2, dt), deriv(3, dt)
+    t = np.zeros(20)
+    xp, vp = O.hermite_predict(dt, t, x0, v0, a0, j0)
+    x1, v1, _ = O.hermite_correct(0.2, 0.125, t, np.full(20, dt), a0, j0, xp, vp, a1, j1)
+    np.testing.assert_allclose(x1, deriv(0, dt), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(v1, deriv(1, dt), rtol=1e-12, atol=1e-14)
+
+
+def test_hermite_steps_follow_the_criteria():
+    """dt_startup: the largest block step <= eta_s |a|/|j| on the grid;
+    dt_criterion: halve below the Aarseth step, double only when commensurate."""
+    rng = np.random.default_rng(3)
+    a, j = rng.normal(size=(200, 3)), rng.normal(size=(200, 3)) * 10 ** rng.uniform(-2, 3, (200, 1))
+    dtmax = 0.125
+    dt = O.dt_startup(0.01, dtmax, 0.0, a, j)
+    crit = 0.01 * np.linalg.norm(a, axis=1) / np.linalg.norm(j, axis=1)
+    k = np.ceil(np.log2(dtmax / crit)).clip(0)
+    np.testing.assert_array_equal(dt, dtmax / 2 ** k)
+    # corrector step rule, with a0 = a1 and j0 = j1 (s0 and c0 then follow from them)
+    t = np.zeros(200)
+    dts = np.full(200, dtmax / 64)
+    xp = vp = np.zeros((200, 3))
+    _, _, dtn = O.hermite_correct(0.2, dtmax, t, dts, a, j, xp, vp, a, j)
+    s0 = 2.0 * (-(3 * j) * dts[:, None]) / dts[:, None] ** 2
+    c0 = 6.0 * (2 * j * dts[:, None]) / dts[:, None] ** 3
+    s1 = s0 + c0 * dts[:, None]
+    A, J, S, Cc = (np.linalg.norm(q, axis=1) for q in (a, j, s1, c0))
+    dti = 0.2 * np.sqrt((A * S + J * J) / (J * Cc + S * S))
+    for i in range(200):
+        if dti[i] < dts[i]:
+            assert dtn[i] <= dti[i] or dtn[i] == dts[i] / 2 ** 51
+            assert 2 * dtn[i] > dti[i]
+        elif dti[i] > 2 * dts[i]:
+            # t + dt = dtmax/64 is a multiple of 2 dt only for dt <= dtmax/128
+            assert dtn[i] == dts[i]
+        else:
+            assert dtn[i] == dts[i]
+
+
+def test_reference_integrator_conserves_energy():
+    """The CPU baseline integrator (reference library + restated banana loop)
+    on a Plummer sphere: relative energy drift over two windows."""
+    if not O.ref_available("pg6"):
+        pytest.skip("reference library pg6 not built")
+    m, x, v = ics.plummer(512, 21)
+    t, xe, ve, info, en = O.ref_hermite(x, v, m, (1 / 64) ** 2, 0.01, 0.2, 0.125, 0.25)
+    e = en.sum(1)
+    assert info["windows"] == 2 and info["blocks"] > 10
+    # potential from the library excludes self: E0 from the initial state
+    o = O.pg5_force(x, (1 / 64) ** 2, x, m)
+    e0 = 0.5 * (m * (v * v).sum(1)).sum() - 0.5 * (m * (o[1] - m / (1 / 64))).sum()
+    assert abs(e[-1] - e0) / abs(e0) < 1e-4, (e, e0)
