@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
-    ap.add_argument("--configs", default="c1,c3,c4,c5",
+    ap.add_argument("--configs", default="c1,c2i,c3,c4,c5",
                     help="extra configurations to measure (comma list, '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -654,7 +654,55 @@ def bench_c5(args, stream, cpu):
     return res
 
 
-EXTRA = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
+def bench_c2i(args, stream, cpu):
+    """Block-time-step Hermite integration on the GPU (the reference's sample
+    program PG6/banana with its published setup, user guide 5.3): Plummer
+    N = 16384, eps = 0, eta_s = 0.01, eta = 0.2, dtmax = 1/8, t = 0 -> 1/8."""
+    import numpy as np
+    import torch
+
+    import ics
+    import pgamd
+    n, t_end = 16384, 0.125
+    m, x, v = ics.plummer(n, 16391)
+    g6 = pgamd.G6()
+    t0 = time.perf_counter()
+    g6.hermite_init(x, v, m, 0.0, 0.01, 0.2, 0.125)
+    t_init = time.perf_counter() - t0
+    ek0, ep0 = g6.hermite_energy()
+    t0 = time.perf_counter()
+    blocks = g6.hermite_run(t_end)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = g6.hermite_state()
+    ek, ep = g6.hermite_energy()
+    g6.lib.pg6_hermite_end()
+    steps = st["steps"] - n  # particle steps after the start-up pass
+    inter = float(steps) * (n - 1)
+    res = {"workload": "hermite_block_steps_plummer_n16384_eps0_t0.125", "wall_s": round(wall, 3),
+           "init_s": round(t_init, 3), "block_steps": blocks, "particle_steps": steps,
+           "mean_block": round(steps / max(blocks, 1), 1),
+           "ginteractions_s": round(inter / wall / 1e9, 2),
+           "pct_fp32_peak": round(100 * 60 * inter / wall / 1e12 / FP32_PEAK_TFLOPS, 2),
+           "energy_error": float(f"{abs(ek + ep - ek0 - ep0) / abs(ek0 + ep0):.3e}"),
+           "published_reference": {"tcalc_s": 23.8, "gflops": 18.4, "mean_block": 232,
+                                   "source": "doc/user_guide.pdf 5.3 (pl016k.init, CPU not stated)"}}
+    O = _oracle()
+    if cpu and O.ref_available("pg6"):
+        t, _, _, info, en = O.ref_hermite(x, v, m, 0.0, 0.01, 0.2, 0.125, t_end)
+        csteps = info["steps"] - n
+        res["cpu_baseline"] = {
+            "ginteractions_s": round(csteps * (n - 1) / t / 1e9, 3), "kind": "reference",
+            "sample": "the whole integration: reference library PG6/libpg6 (AVX2 + OpenMP) "
+                      "driven by the banana loop (oracle/cpu_batch.c cb_hermite)",
+            "threads": _threads(), "seconds": round(t, 2), "block_steps": info["blocks"] - 1,
+            "particle_steps": csteps,
+            "energy_error": float(f"{abs(en[-1].sum() - ek0 - ep0) / abs(ek0 + ep0):.3e}"),
+            "gpu_speedup": round(t / wall, 1)}
+    return res
+
+
+EXTRA = {"c1": bench_c1, "c2i": bench_c2i, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
 
 
 if __name__ == "__main__":
