@@ -69,12 +69,14 @@ def test_matches_reference_fixture(g6):
     np.testing.assert_array_equal(np.concatenate(got), g["nbl"])
 
 
+@pytest.mark.parametrize("eps", [1.0 / 256, 0.0])
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("ni", [1, 7, 48, 64, 500, 4096])
-def test_force_vs_fp64_oracle(g6, ni, variant):
+def test_force_vs_fp64_oracle(g6, ni, variant, eps):
+    """Both kernels against the FP64 oracle, softened and unsoftened (the
+    reference's sample integrator runs with eps = 0)."""
     assert g6.lib.pg6_dev_set_kernel(variant) == 0
     n = 4096
-    eps = 1.0 / 256
     m, x, v, a2, j6, tj, idx = plummer_state(n, 2, eps)
     g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
     ti = 1.0 / 512
