@@ -110,6 +110,15 @@ int pg5t_batch_autotune(int devid, int ngroups, const int *ioff, const int *icnt
                         const int *order, const void *pos, double *acc, int mode, int reps,
                         int *best, float *ms);
 int pg5t_batch_set_blocks_per_cu(int mode, int bpc);
+/* Long-range (particle-mesh) side of the TreePM force, which the reference
+ * does not ship (PG5/table/libpg5/pppm.h:4-6 names its mesh): the periodic
+ * force of the S2 shape R(r, eps_pm) that the table subtracts, by TSC
+ * assignment on a mesh^3 grid, FFT, Green's function -4 pi S(k a)^2 /
+ * (k^2 W(k)^2), spectral gradient and TSC interpolation.  pos[n][3] in the
+ * unit box [0, 1), mass[n], acc[n][3]: device FP64; acc is written.  Table
+ * force + this = the S2-softened periodic Newtonian force. */
+int pg5t_pm_force(int n, const double *pos, const double *mass, int mesh, double eps_pm,
+                  double *acc, void *stream);
 
 /* ------------------------------------------------------------------ libpg6.so
  * pgamd-library: pg6 */

@@ -130,6 +130,11 @@ def pg6_nblist(xi, idi, h2, eps2, xj, idj, maxlen=1024):
     return [nbl[i, : min(nblen[i], maxlen)].copy() for i in range(ni)], nblen
 
 
+def s2_reff(a, r):
+    """R(r, a) of pg5_table.c:8-22 (or_s2_reff)."""
+    return lib().or_s2_reff(float(a), float(r))
+
+
 def s2_force(eps_pp, eps_pm, r):
     return np.array([lib().or_s2_force(eps_pp, eps_pm, float(x)) for x in np.atleast_1d(r)])
 

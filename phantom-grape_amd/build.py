@@ -37,7 +37,7 @@ HOST_TARGETS = {
 TARGETS = {
     "pg5": (["pg5.hip", "pgamd_common.cpp"], []),
     "pg5_sym": (["pg5.hip", "pgamd_common.cpp"], ["-DPG5_SYMMETRIC"]),
-    "pg5_table": (["pg5_table.hip", "pgamd_common.cpp"], []),
+    "pg5_table": (["pg5_table.hip", "pgamd_common.cpp"], ["-lhipfft"]),
     "pg6": (["pg6.hip", "pgamd_common.cpp"], []),
     "pgtree_dev": (["treebuild_dev.hip", "pgamd_common.cpp"], []),
 }

@@ -14,6 +14,9 @@
 #include "pgamd_api.h"
 #include "table_kernels.h"
 #include "treepm_batch.h"
+#include "pm_kernels.h"
+
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -1130,5 +1133,94 @@ PGAMD_API int pg5t_batch_set_blocks_per_cu(int mode, int bpc) {
     return -1;
   }
   g_batch_bpc[mode] = bpc;
+  return 0;
+}
+
+// ======================================================================
+// Long-range (particle-mesh) side of the TreePM force: the periodic force of
+// the S2 shape R(r, eps_PM) that the table subtracts (pm_kernels.h).  No
+// counterpart in the reference, which ships only the short-range side
+// (PG5/table/libpg5/pppm.h:4-6 names its mesh size).
+namespace {
+
+#define PGAMD_FFT_CHECK(expr)                                                        \
+  do {                                                                              \
+    hipfftResult _r = (expr);                                                       \
+    if (_r != HIPFFT_SUCCESS) {                                                     \
+      std::fprintf(stderr, "pgamd: hipFFT error %d at %s:%d in %s\n", (int)_r,      \
+                   __FILE__, __LINE__, #expr);                                      \
+      std::abort();                                                                 \
+    }                                                                               \
+  } while (0)
+
+struct PmCache {
+  int M = 0;
+  hipfftHandle fwd{}, inv{};
+  DevBuf rho, rhok, fk, f;
+};
+PmCache g_pm;
+std::mutex g_pm_mutex;
+
+void pm_setup(PmCache& c, int M) {
+  if (c.M == M) return;
+  if (c.M) {
+    PGAMD_FFT_CHECK(hipfftDestroy(c.fwd));
+    PGAMD_FFT_CHECK(hipfftDestroy(c.inv));
+  }
+  const size_t nr = (size_t)M * M * M, nk = (size_t)M * M * (M / 2 + 1);
+  c.rho.reserve(sizeof(double) * nr);
+  c.rhok.reserve(sizeof(double2) * nk);
+  c.fk.reserve(sizeof(double2) * nk * 3);
+  c.f.reserve(sizeof(double) * nr * 3);
+  PGAMD_FFT_CHECK(hipfftPlan3d(&c.fwd, M, M, M, HIPFFT_D2Z));
+  PGAMD_FFT_CHECK(hipfftPlan3d(&c.inv, M, M, M, HIPFFT_Z2D));
+  c.M = M;
+}
+
+}  // namespace
+
+// Long-range TreePM accelerations of n particles in the periodic unit box:
+// pos[n][3] in [0, 1) and mass[n] (device, FP64) on an M^3 mesh (M even,
+// 8..1024), with the S2 split at eps_pm (the table's eps_PM, which is also
+// its cut-off radius).  acc[n][3] (device, FP64) is written.  Add the
+// short-range accelerations of the table (pg5t_batch_force or the g5 calls)
+// for the whole TreePM force.  Asynchronous on `stream` (null = the library's).
+PGAMD_API int pg5t_pm_force(int n, const double* pos, const double* mass, int mesh, double eps_pm,
+                            double* acc, void* stream) {
+  Board* b = board(0, "pg5t_pm_force");
+  if (!b) return -1;
+  if (n < 0 || mesh < 8 || mesh > 1024 || (mesh & 1) || !(eps_pm > 0.0) ||
+      (n > 0 && (!pos || !mass || !acc))) {
+    api_error(3, "pg5t_pm_force: n=%d mesh=%d eps_pm=%g (need an even mesh in [8, 1024], "
+                 "eps_pm > 0, non-null arrays)", n, mesh, eps_pm);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_pm_mutex);
+  PmCache& c = g_pm;
+  pm_setup(c, mesh);
+  hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+  PGAMD_FFT_CHECK(hipfftSetStream(c.fwd, st));
+  PGAMD_FFT_CHECK(hipfftSetStream(c.inv, st));
+  const size_t nr = (size_t)mesh * mesh * mesh, nk = (size_t)mesh * mesh * (mesh / 2 + 1);
+  double* rho = c.rho.as<double>();
+  double2* rhok = c.rhok.as<double2>();
+  double2* fk = c.fk.as<double2>();
+  double* f = c.f.as<double>();
+  PGAMD_HIP_CHECK(hipMemsetAsync(rho, 0, sizeof(double) * nr, st));
+  if (n > 0)
+    hipLaunchKernelGGL(pm::pm_assign, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mass,
+                       mesh, rho);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  PGAMD_FFT_CHECK(hipfftExecD2Z(c.fwd, rho, reinterpret_cast<hipfftDoubleComplex*>(rhok)));
+  hipLaunchKernelGGL(pm::pm_green, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, st, mesh,
+                     eps_pm, rhok, fk);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  for (int d = 0; d < 3; ++d)
+    PGAMD_FFT_CHECK(hipfftExecZ2D(c.inv, reinterpret_cast<hipfftDoubleComplex*>(fk + d * nk),
+                                  f + d * nr));
+  if (n > 0)
+    hipLaunchKernelGGL(pm::pm_interp, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mesh, f,
+                       f + nr, f + 2 * nr, acc);
+  PGAMD_HIP_CHECK(hipGetLastError());
   return 0;
 }

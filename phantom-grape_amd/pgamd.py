@@ -420,6 +420,7 @@ class G5Table:
         _sig(L, "pg5t_batch_graph_destroy", C.c_int, _vp)
         _sig(L, "pg5t_batch_autotune", C.c_int, C.c_int, *bt, C.c_int, _ip, _fp)
         _sig(L, "pg5t_batch_set_blocks_per_cu", C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg5t_pm_force", C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_double, _vp, _vp)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
         L.g5_open()

@@ -491,3 +491,42 @@ def test_reference_integrator_conserves_energy():
     o = O.pg5_force(x, (1 / 64) ** 2, x, m)
     e0 = 0.5 * (m * (v * v).sum(1)).sum() - 0.5 * (m * (o[1] - m / (1 / 64))).sum()
     assert abs(e[-1] - e0) / abs(e0) < 1e-4, (e, e0)
+
+
+# ------------------------------------------- long-range (mesh) side of TreePM
+def test_ewald_sum_pins():
+    import pm_oracle as P
+    g = np.array([(i, j, k) for i in (0.25, 0.75) for j in (0.25, 0.75) for k in (0.25, 0.75)])
+    assert np.abs(P.ewald_force(g, np.ones(8))).max() < 1e-9  # lattice: no net force
+    x = np.array([[0.5, 0.5, 0.5], [0.51, 0.5, 0.5]])
+    a = P.ewald_force(x, np.ones(2))
+    np.testing.assert_allclose(a[0], [1e4, 0, 0], rtol=1e-3, atol=1e-9)  # Newtonian up close
+    r = np.linspace(1e-4, 0.2, 64)
+    np.testing.assert_allclose(P.reff(r, 0.05), [O.s2_reff(0.05, v) for v in r], rtol=1e-14)
+
+
+def test_mesh_force_converges_to_the_meshless_long_range_force():
+    import pm_oracle as P
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 1, (48, 3))
+    m = np.full(48, 1 / 48)
+    ex = P.pm_exact(x, m, 3 / 32)
+    e32 = np.median(rel_err(P.pm_force(x, m, 32, 3 / 32), ex))
+    e64 = np.median(rel_err(P.pm_force(x, m, 64, 3 / 32), ex))
+    assert e32 < 2e-2 and e64 < 1e-3 and e64 < e32 / 10
+
+
+def test_short_plus_long_range_is_the_periodic_force():
+    import pgamd
+    import pm_oracle as P
+    mesh = 32
+    pp, pm = 0.1 / mesh, 3.0 / mesh
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 1, (96, 3))
+    m = rng.uniform(0.5, 1.5, 96) / 96
+    L = pgamd.treepm_lists(x, pm, 32)
+    tot = np.zeros((96, 3))
+    tot[L["src"][:96]] = O.table_force_f64(L["pos"][:96], L["pos"], m[L["src"]], pp, pm)
+    tot += P.pm_force(x, m, mesh, pm)
+    e = rel_err(tot, P.pm_exact(x, m, pp))
+    assert np.median(e) < 3e-2 and np.percentile(e, 90) < 8e-2
