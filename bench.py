@@ -495,6 +495,16 @@ def bench_c3(args, stream, cpu):
                                             / FP32_PEAK_TFLOPS, 2),
                      "blocks_per_cu": int(best[0])}
     res["mlp_over_table_speed"] = round(res["table1024"]["ms"] / res["mlp"]["ms"], 3)
+    # long-range side on the 128^3 mesh, and the whole TreePM force step
+    acc_pm = torch.zeros((len(x), 3), dtype=torch.float64, device="cuda")
+    pm_ms = _time(lambda: T.lib.pg5t_pm_force(len(x), d_x.data_ptr(), d_m.data_ptr(), 128, pm,
+                                              acc_pm.data_ptr(), sp), stream, 5)
+    res["pm_mesh128_ms"] = round(pm_ms, 3)
+    res["treepm_step_ms"] = {"gpu_lists": round(1e3 * res["list_build_gpu_s"], 2),
+                             "short_range_table": res["table1024"]["ms"],
+                             "long_range_pm": round(pm_ms, 3),
+                             "total": round(1e3 * res["list_build_gpu_s"]
+                                            + res["table1024"]["ms"] + pm_ms, 2)}
     O = _oracle()
     sel = np.sort(np.random.default_rng(4).choice(len(x), 512, replace=False))
     f = O.table_force_f64(P[sel], P, mm, pp, pm)
