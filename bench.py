@@ -467,7 +467,7 @@ def bench_c3(args, stream, cpu):
     T.lib.pg5_gen_s2_force_table(pp, pm)
     _, xs = T.table()
     t0 = time.time()
-    loss = T.mlp_train(steps=6000, batch=4096, lr=3e-3, seed=7)
+    loss = T.mlp_train(steps=30000, batch=8192, lr=3e-3, seed=7)
     ttrain = time.time() - t0
     P = L["pos"]
     mm = m[L["src"]]
