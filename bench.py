@@ -409,12 +409,17 @@ def bench_c1(args, stream, cpu):
     def run():
         assert g5.lib.pg5_dev_force(0, n, d_xi.data_ptr(), n, d_xj.data_ptr(), None,
                                     acc.data_ptr(), pot.data_ptr(), sp) == 0
+    kernels = {}
+    for variant, name in ((0, "lds_tiles"), (1, "scalar_broadcast")):
+        g5.lib.pg5_dev_set_kernel(variant)
+        kernels[name] = round(_time(run, stream, 20), 4)
+    g5.lib.pg5_dev_set_kernel(1)
     ms = _time(run, stream, 20)
     inter = float(n) * n
     out = {"workload": "pg5_plummer_n16384_eps1/64", "interactions": inter,
            "ms": round(ms, 4), "ginteractions_s": round(inter / ms / 1e6, 1),
            "pct_fp32_peak": round(100 * 38 * inter / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
-           "flops_per_interaction": 38}
+           "flops_per_interaction": 38, "kernel_ms": kernels}
     O = _oracle()
     sel = np.sort(np.random.default_rng(3).choice(n, 1024, replace=False))
     oa, _ = O.pg5_force(x[sel], eps * eps, x, m)

@@ -157,14 +157,20 @@ void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
 }
 
 struct Plan {
-  int bi, nsplit, jper;
+  int bi, nsplit, jper, variant;
 };
 
-Plan plan(int ni, int nj) {
+// Particle kernel: 0 = j-tiles in LDS (pg5_force), 1 = j broadcast through
+// scalar registers (pg5_force_s).  Cells always use the LDS kernel.
+int g_variant = 1;
+
+Plan plan(int ni, int nj, bool cells = false) {
   Plan p;
   p.bi = g_force_bi > 0 ? g_force_bi : (ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64));
+  p.variant = cells ? 0 : g_variant;
   const int nbi = ceil_div(ni, p.bi);
-  const int target = device_cu_count() * g_blocks_per_cu;
+  // the scalar-broadcast kernel runs bi threads per block, the LDS one 256
+  const int target = device_cu_count() * g_blocks_per_cu * (p.variant == 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
   ns = std::min(ns, std::max(1, nj / std::max(1, g_min_jper)));
   int jper = ceil_div(std::max(nj, 1), ns);
@@ -192,9 +198,13 @@ template <int BI>
 void launch_particles_t(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
                         const float* ej2, const Partials5& part, const Outputs5& out,
                         hipStream_t st) {
-  hipLaunchKernelGGL((pg5::pg5_force<BI, kSym>), dim3(ceil_div(ni, BI), p.nsplit),
-                     dim3(pg5::kThreads), 0, st, ni, xi, nj, p.jper, xj, ej2, part, out,
-                     p.nsplit);
+  if (p.variant == 1)
+    hipLaunchKernelGGL((pg5::pg5_force_s<kSym>), dim3(ceil_div(ni, BI), p.nsplit), dim3(BI), 0,
+                       st, ni, xi, nj, p.jper, xj, ej2, part, out, p.nsplit);
+  else
+    hipLaunchKernelGGL((pg5::pg5_force<BI, kSym>), dim3(ceil_div(ni, BI), p.nsplit),
+                       dim3(pg5::kThreads), 0, st, ni, xi, nj, p.jper, xj, ej2, part, out,
+                       p.nsplit);
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
@@ -216,7 +226,7 @@ void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* 
     PGAMD_HIP_CHECK(hipMemsetAsync(out.pot, 0, sizeof(double) * (size_t)ni, st));
     return;
   }
-  const Plan p = plan(ni, nj);
+  const Plan p = plan(ni, nj, cells != nullptr);
   const Partials5 part = partials(scratch, p.nsplit, ni);
   if (cells) {
     switch (p.bi) {
@@ -787,5 +797,16 @@ PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
   g_bh_bpc = bestc;
   if (best) *best = bestc;
+  return 0;
+}
+
+// Particle force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through
+// scalar registers (default).  Same pair arithmetic; speed only.
+PGAMD_API int pg5_dev_set_kernel(int variant) {
+  if (variant != 0 && variant != 1) {
+    api_error(3, "pg5_dev_set_kernel: variant %d not in {0, 1}", variant);
+    return -1;
+  }
+  g_variant = variant;
   return 0;
 }

@@ -125,6 +125,57 @@ __global__ __launch_bounds__(kThreads) void pg5_force(
   pg5_store(i, nsplit, ni, ax, ay, az, pot, part, out);
 }
 
+// Variant of pg5_force that broadcasts the j-particles through scalar
+// registers instead of LDS: every wave of a block walks the same j sequence,
+// so each j is wave-uniform; read through the constant address space it
+// arrives by scalar loads and the VALU instructions take it as an operand.
+// With ~14 VALU instructions per pair the LDS variant issues one LDS read
+// per pair per wave, enough to make LDS issue the limit; this one has none.
+// Block = 64, 128 or 256 threads, one i-particle per lane.  Same pair
+// arithmetic as pg5_force; FP32 sums per 256 j folded into FP64.
+typedef const __attribute__((address_space(4))) float cfloat5;
+
+template <bool SYM>
+__global__ __launch_bounds__(kThreads) void pg5_force_s(
+    int ni, const float4* __restrict__ xi, int nj, int jper, const float4* __restrict__ xj,
+    const float* __restrict__ ej2, Partials5 part, Outputs5 out, int nsplit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ivalid = i < ni;
+  const float4 ip = xi[ivalid ? i : ni - 1];
+  cfloat5* pj = (cfloat5*)xj;
+  cfloat5* pe = (cfloat5*)ej2;
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min(nj, j0 + jper);
+  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
+  for (int jt = j0; jt < j1; jt += kJTile) {
+    const int nt = min(kJTile, j1 - jt);
+    float fax = 0.f, fay = 0.f, faz = 0.f, fpot = 0.f;
+#pragma unroll 8
+    for (int jj = 0; jj < nt; ++jj) {
+      const int j = jt + jj;
+      const float dx = pj[4 * j + 0] - ip.x;
+      const float dy = pj[4 * j + 1] - ip.y;
+      const float dz = pj[4 * j + 2] - ip.z;
+      const float mj = pj[4 * j + 3];
+      const float e2 = SYM ? ip.w + pe[j] : ip.w;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float mr = mj * rinv;
+      const float mr3 = mr * rinv * rinv;
+      fpot += mr;
+      fax = fmaf(mr3, dx, fax);
+      fay = fmaf(mr3, dy, fay);
+      faz = fmaf(mr3, dz, faz);
+    }
+    ax += (double)fax;
+    ay += (double)fay;
+    az += (double)faz;
+    pot += (double)fpot;
+  }
+  if (!ivalid) return;
+  pg5_store(i, nsplit, ni, ax, ay, az, pot, part, out);
+}
+
 // Monopole + quadrupole cell force (quad_avx2.c:35-149):
 //   phi_p = M / r, phi_q = (dx.Q.dx) / (2 r^5),
 //   acc  += (phi_p + 5 phi_q) dx / r^2 - Q.dx / r^5,  pot += phi_p + phi_q.

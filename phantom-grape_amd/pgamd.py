@@ -145,6 +145,7 @@ class G5:
         _sig(L, "pg5_dev_quad", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
         _sig(L, "pg5_dev_stream", _vp, C.c_int)
         _sig(L, "pg5_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg5_dev_set_kernel", C.c_int, C.c_int)
         bh = [C.c_int] * 1 + [_vp] * 10 + [_vp, _vp, C.c_double, _vp, _vp]
         _sig(L, "pg5_bh_batch", C.c_int, C.c_int, *bh, _vp)
         _sig(L, "pg5_bh_graph", _vp, C.c_int, *bh)

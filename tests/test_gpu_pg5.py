@@ -29,9 +29,12 @@ def f32(x):
     return np.asarray(x, np.float32).astype(np.float64)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["pl1k", "pl1k_1", "pl1k_3"])
-def test_reference_fixture(g5, name):
-    """Reference direct.c inputs (N = 1024, 1023, 1021: the tail cases)."""
+def test_reference_fixture(g5, name, variant):
+    """Reference direct.c inputs (N = 1024, 1023, 1021: the tail cases), both
+    particle kernels (LDS tiles, scalar broadcast)."""
+    assert g5.lib.pg5_dev_set_kernel(variant) == 0
     g = golden("pg5_direct.npz")
     m, x, eps = g[f"{name}_m"], g[f"{name}_x"], float(g[f"{name}_eps"])
     g5.set_xmj(0, x, m)
@@ -45,6 +48,21 @@ def test_reference_fixture(g5, name):
     oa, op = O.pg5_force(f32(x), np.float32(eps * eps), f32(x), f32(m))
     assert np.percentile(rel_err(a, oa), 99) < 1e-5
     assert rel_err(p, op).max() < 1e-6
+    g5.lib.pg5_dev_set_kernel(1)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_symmetric_softening_both_kernels(g5s, variant):
+    assert g5s.lib.pg5_dev_set_kernel(variant) == 0
+    g = golden("pg5_symmetric.npz")
+    x, m, e2 = g["x"], g["m"], g["eps2"]
+    g5s.set_xmj(0, x, m, epsj2=e2)
+    g5s.set_n(len(m))
+    a, p = g5s.calculate_force_on_x(x, eps2=e2)
+    oa, op = O.pg5_force(f32(x), f32(e2), f32(x), f32(m), eps2j=f32(e2))
+    assert np.percentile(rel_err(a, oa), 99) < 1e-5
+    assert rel_err(p, op).max() < 1e-6
+    g5s.lib.pg5_dev_set_kernel(1)
 
 
 def test_plummer_16k_vs_fp64(g5):
