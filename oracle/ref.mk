@@ -39,7 +39,27 @@ define so
 endef
 
 ALL := $(OUT)/libref_pg5.so $(OUT)/libref_pg5_omp.so $(OUT)/libref_pg5sym.so $(OUT)/libref_pg5quad.so \
-       $(OUT)/libref_pg5table.so $(OUT)/libref_pg5table_omp.so $(OUT)/libref_pg6.so
+       $(OUT)/libref_pg5table.so $(OUT)/libref_pg5table_omp.so $(OUT)/libref_pg6.so \
+       $(OUT)/direct_ref $(OUT)/direct_pgamd $(OUT)/banana_ref $(OUT)/banana_pgamd
+
+# The reference's own sample programs, compiled from their sources against
+# their own headers, linked twice: with the reference library (*_ref) and with
+# this repository's HIP library (*_pgamd) -- the drop-in check.  They are
+# checkers like the libraries above and run only from tests/.
+PGLIB   := $(abspath $(OUT)/../../phantom-grape_amd/lib)
+DIRECT  := $(REF)/PG5/newton/direct
+BANANA  := $(REF)/PG6/banana
+DIRECT_SRC := $(DIRECT)/direct.c $(DIRECT)/cpu.c
+BANANA_SRC := $(BANANA)/banana01.c $(BANANA)/io.c $(BANANA)/calc_energy.c $(BANANA)/hermite.c \
+              $(BANANA)/timestep.c $(BANANA)/timeprof.c
+
+# $(call prog,output,flags,includes,sources,linkflags)
+define prog
+	rm -rf $(1).objs && mkdir -p $(1).objs
+	cd $(1).objs && for f in $(4); do $(CC) $(2) $(3) -c $$f -o $$(basename $$f .c).o || exit 1; done
+	$(CC) $(1).objs/*.o -o $(1) $(5) -lm
+	rm -rf $(1).objs
+endef
 
 all: $(ALL)
 
@@ -68,5 +88,21 @@ $(OUT)/libref_pg5table_omp.so: $(PG5T)/gravity_kernel_avx2.c $(PG5T)/phantom_g5.
 
 $(OUT)/libref_pg6.so: $(PG6)/phantom_g6.c $(PG6)/gravity_avx2.c $(PG6)/timeprof.c | $(OUT)
 	$(call so,$(abspath $@),$(PG6_FLAGS),-I$(PG6),$^,-fopenmp)
+
+# direct.c as PG5/newton/direct/Makefile builds it (setting.mk: no OpenMP)
+$(OUT)/direct_ref: $(DIRECT_SRC) $(OUT)/libref_pg5.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -Wno-unused-result,-I$(DIRECT),$(DIRECT_SRC),-L$(abspath $(OUT)) -lref_pg5 -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN')
+
+$(OUT)/direct_pgamd: $(DIRECT_SRC) $(PGLIB)/libpg5.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -Wno-unused-result,-I$(DIRECT),$(DIRECT_SRC),-L$(PGLIB) -lpg5 -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN/../../phantom-grape_amd/lib')
+
+# banana01 as PG6/banana/Makefile builds it (PG6/setting.mk: OpenMP, NORMAL)
+$(OUT)/banana_ref: $(BANANA_SRC) $(OUT)/libref_pg6.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -fopenmp -DNORMAL -Wno-unused-result,-I$(PG6) -I$(BANANA),$(BANANA_SRC),-fopenmp -L$(abspath $(OUT)) -lref_pg6 -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN')
+
+$(OUT)/banana_pgamd: $(BANANA_SRC) $(PGLIB)/libpg6.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -fopenmp -DNORMAL -Wno-unused-result,-I$(PG6) -I$(BANANA),$(BANANA_SRC),-fopenmp -L$(PGLIB) -lpg6 -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN/../../phantom-grape_amd/lib')
+
+COMMA := ,
 
 .PHONY: all

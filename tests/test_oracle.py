@@ -4,6 +4,7 @@ made by running the compiled reference on its own input files), against the
 compiled reference live where it is built, and against what the mathematics
 fixes (closed forms, invariants, finite differences, brute force).
 """
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -530,3 +531,21 @@ def test_short_plus_long_range_is_the_periodic_force():
     tot += P.pm_force(x, m, mesh, pm)
     e = rel_err(tot, P.pm_exact(x, m, pp))
     assert np.median(e) < 3e-2 and np.percentile(e, 90) < 8e-2
+
+
+# ------------------------------------------ the reference's own sample programs
+def test_reference_sample_programs_run(tmp_path):
+    """direct.c and banana01 built against the reference library (the other
+    half of the drop-in check runs them against libpg5 / libpg6 on a GPU)."""
+    import dropin
+    if not (os.path.exists(dropin.program("direct_ref")) and
+            os.path.exists(dropin.program("banana_ref"))):
+        pytest.skip("reference sample programs not built (oracle/ref.mk)")
+    m, x, v = ics.plummer(512, 5)
+    dropin.write_direct_input(tmp_path / "pl", m, x, v)
+    de = dropin.direct_energy_errors(dropin.run("direct_ref", ["pl", "pl.out"], tmp_path))
+    assert len(de) == 9 and np.abs(de).max() < 1e-2
+    dropin.write_banana_input(tmp_path, m, x, v, 0.25)
+    out = dropin.run("banana_ref", [], tmp_path)
+    de = dropin.banana_energy_errors(out)
+    assert len(de) == 3 and np.abs(de).max() < 1e-5
