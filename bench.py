@@ -84,7 +84,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", type=int, default=-1,
-                    help="PG6 force kernel: 0 LDS j-tiles, 1 SGPR broadcast (default: library's)")
+                    help="PG6 force kernel: 0 LDS j-tiles, 1 SGPR broadcast, 2 SGPR broadcast "
+                         "with two-float positions (default: library's)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the library's default launch shapes instead of autotuning them")
     ap.add_argument("--launch", default="",
@@ -472,7 +473,7 @@ def bench_c3(args, stream, cpu):
     T.lib.pg5_gen_s2_force_table(pp, pm)
     _, xs = T.table()
     t0 = time.time()
-    loss = T.mlp_train(steps=30000, batch=8192, lr=3e-3, seed=7)
+    loss = T.mlp_train(steps=6000, batch=4096, lr=3e-3, seed=7)
     ttrain = time.time() - t0
     P = L["pos"]
     mm = m[L["src"]]

@@ -145,8 +145,11 @@ void *pg6_dev_stream(void);
  * smallest j-range (0 = 128). */
 int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 /* Force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through scalar
- * registers, -1 = automatic (autotuned or default per i-count).  Same
- * arithmetic and results up to summation order. */
+ * registers, 2 = as 1 with dx formed in FP32 from two-float positions and
+ * the per-pair self test skipped on j-chunks whose id range cannot hold the
+ * self pair, -1 = automatic (autotuned or default per i-count).  0 and 1:
+ * same arithmetic, results equal up to summation order; 2: dx within about
+ * one FP32 ulp of theirs. */
 int pg6_dev_set_kernel(int variant);
 /* Launch shape for (ni, nj): out5 = {variant, bi, blocks_per_cu, nsplit,
  * jper}. */

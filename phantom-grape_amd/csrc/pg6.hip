@@ -41,7 +41,8 @@ struct LaunchPlan {
   int bi;       // i-particles per block
   int nsplit;   // j-ranges
   int jper;     // j-particles per range
-  int variant;  // 0: j-tiles in LDS (pg6_force), 1: j broadcast in SGPRs (pg6_force_s)
+  int variant;  // 0: j-tiles in LDS (pg6_force), 1: j broadcast in SGPRs (pg6_force_s),
+                // 2: as 1 with two-float positions and the self filter (pg6_force_h)
   int bpc;      // target blocks (of 256 threads) per CU
 };
 
@@ -134,7 +135,8 @@ void board_alloc_j(Board& b, int cap) {
   // Grow j-state and prediction arrays, preserving the j-state contents.
   DevBuf njs, njp;
   const size_t jsb = (size_t)cap * (14 * sizeof(double) + sizeof(int));
-  const size_t jpb = (size_t)cap * (3 * sizeof(double) + sizeof(float4) + sizeof(int));
+  const size_t jpb = (size_t)cap * (3 * sizeof(double) + 4 * sizeof(float4) + sizeof(int)) +
+                     (size_t)(cap / pg6::kChunk + 1) * sizeof(int2);
   njs.reserve(jsb);
   njp.reserve(jpb);
   PGAMD_HIP_CHECK(hipMemsetAsync(njs.ptr, 0, jsb, b.stream));
@@ -151,7 +153,9 @@ void board_alloc_j(Board& b, int cap) {
   jp.y = e + cap;
   jp.z = e + 2 * (size_t)cap;
   jp.vm = reinterpret_cast<float4*>(e + 3 * (size_t)cap);
-  jp.id = reinterpret_cast<int*>(jp.vm + cap);
+  jp.rec = jp.vm + cap;
+  jp.crange = reinterpret_cast<int2*>(jp.rec + 3 * (size_t)cap);
+  jp.id = reinterpret_cast<int*>(jp.crange + cap / pg6::kChunk + 1);
   if (b.cap > 0) {
     double** o[14] = {&b.js.x, &b.js.y, &b.js.z, &b.js.vx, &b.js.vy, &b.js.vz, &b.js.ax,
                       &b.js.ay, &b.js.az, &b.js.kx, &b.js.ky, &b.js.kz, &b.js.m, &b.js.t};
@@ -287,11 +291,12 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   p.bpc = b.force_bpc > 0 ? b.force_bpc : c.bpc;
   const int nbi = ceil_div(ni, p.bi);
   // the SGPR-broadcast kernel runs bi threads per block, the LDS one 256
-  const int target = device_cu_count() * p.bpc * (p.variant == 1 ? 256 / p.bi : 1);
+  const int target = device_cu_count() * p.bpc * (p.variant >= 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
   const int maxs = std::max(1, nj / std::max(1, b.min_jper));
   ns = std::min(ns, maxs);
   int jper = ceil_div(nj, ns);
+  static_assert(pg6::kChunk == 64, "j-ranges are aligned to the self-filter chunks");
   jper = (jper + 63) / 64 * 64;
   ns = std::max(1, ceil_div(nj, jper));
   p.nsplit = ns;
@@ -305,7 +310,10 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
                     const Outputs& out, hipStream_t st) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
   g_timer.before(st);
-  if (p.variant == 1)
+  if (p.variant == 2)  // plan_launch keeps jper a multiple of pg6::kChunk
+    hipLaunchKernelGGL((pg6::pg6_force_h<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
+                       nj, p.jper, jp, part, out, p.nsplit);
+  else if (p.variant == 1)
     hipLaunchKernelGGL((pg6::pg6_force_s<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
                        nj, p.jper, jp, part, out, p.nsplit);
   else
@@ -1020,13 +1028,15 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
 }
 
 // Force kernel choice: 0 = j-particles staged in LDS tiles (pg6_force),
-// 1 = j-particles broadcast through scalar registers (pg6_force_s), -1 = the
-// autotuned or default choice for each i-count.  Both kernels compute the
-// same arithmetic; the choice affects speed only.
+// 1 = j-particles broadcast through scalar registers (pg6_force_s), 2 = as 1
+// with two-float positions and the chunked self-pair filter (pg6_force_h),
+// -1 = the autotuned or default choice for each i-count.  0 and 1 compute
+// the same arithmetic; 2 forms dx in FP32 (within about one FP32 ulp of the
+// others).
 PGAMD_API int pg6_dev_set_kernel(int variant) {
   std::lock_guard<std::mutex> lk(g_mutex);
-  if (variant < -1 || variant > 1) {
-    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1}", variant);
+  if (variant < -1 || variant > 2) {
+    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1, 2}", variant);
     return -1;
   }
   g_board.force_variant = variant;
@@ -1099,7 +1109,7 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
     const int ni = 64 << k;
     float best = 1e30f;
     LaunchChoice bc = default_choice(ni);
-    for (int var = 0; var <= 1; ++var)
+    for (int var = 0; var <= 2; ++var)
       for (int bi : {64, 128, 256})
         for (int bpc : kBpc) {
           b.force_variant = var;

@@ -41,11 +41,22 @@ struct JState {
   int* idx;
 };
 
+// j-particles per chunk of the self-pair filter (pg6_force_h).
+constexpr int kChunk = 64;
+
 // Predicted j-particles, SoA.
 struct JPred {
   double *x, *y, *z;
   float4* vm;  // predicted velocity (FP32) and mass (FP32)
   int* id;
+  // One 48-byte record per j for pg6_force_h, three float4:
+  //   {x_hi, y_hi, z_hi, m}, {x_lo, y_lo, z_lo, vx}, {vy, vz, id (int bits), m},
+  // the predicted position as an unevaluated sum of two floats, x = hi + lo
+  // (hi = x rounded to FP32, lo = the remainder rounded to FP32).
+  float4* rec;
+  // Smallest and largest particle id of each chunk of kChunk consecutive
+  // j addresses (chunk c = addresses [c*kChunk, (c+1)*kChunk) below nj).
+  int2* crange;
 };
 
 // Per-j-range partial results, SoA over [split][i].
@@ -91,20 +102,42 @@ __device__ inline double pg6_pred_vel(double dt, double v, double a2, double j6)
   return v + (2.0 * a2) * dt + (3.0 * j6) * dt * dt;
 }
 
-__global__ void pg6_predict(int nj, double ti, JState js, JPred jp) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nj) return;
-  const double dt = ti - js.t[j];
-  jp.x[j] = pg6_pred_pos(dt, js.x[j], js.vx[j], js.ax[j], js.kx[j]);
-  jp.y[j] = pg6_pred_pos(dt, js.y[j], js.vy[j], js.ay[j], js.ky[j]);
-  jp.z[j] = pg6_pred_pos(dt, js.z[j], js.vz[j], js.az[j], js.kz[j]);
-  float4 vm;
-  vm.x = (float)pg6_pred_vel(dt, js.vx[j], js.ax[j], js.kx[j]);
-  vm.y = (float)pg6_pred_vel(dt, js.vy[j], js.ay[j], js.ky[j]);
-  vm.z = (float)pg6_pred_vel(dt, js.vz[j], js.az[j], js.kz[j]);
-  vm.w = (float)js.m[j];
-  jp.vm[j] = vm;
-  jp.id[j] = js.idx[j];
+// Block = 256 threads (whole wavefronts, so that wavefront w of the grid
+// covers exactly j-chunk w).
+__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = j < nj;
+  int id = 0;
+  if (valid) {
+    const double dt = ti - js.t[j];
+    const double x = pg6_pred_pos(dt, js.x[j], js.vx[j], js.ax[j], js.kx[j]);
+    const double y = pg6_pred_pos(dt, js.y[j], js.vy[j], js.ay[j], js.ky[j]);
+    const double z = pg6_pred_pos(dt, js.z[j], js.vz[j], js.az[j], js.kz[j]);
+    jp.x[j] = x;
+    jp.y[j] = y;
+    jp.z[j] = z;
+    float4 vm;
+    vm.x = (float)pg6_pred_vel(dt, js.vx[j], js.ax[j], js.kx[j]);
+    vm.y = (float)pg6_pred_vel(dt, js.vy[j], js.ay[j], js.ky[j]);
+    vm.z = (float)pg6_pred_vel(dt, js.vz[j], js.az[j], js.kz[j]);
+    vm.w = (float)js.m[j];
+    jp.vm[j] = vm;
+    id = js.idx[j];
+    jp.id[j] = id;
+    const float hx = (float)x, hy = (float)y, hz = (float)z;
+    float4* r = jp.rec + 3 * (size_t)j;
+    r[0] = make_float4(hx, hy, hz, vm.w);
+    // x - (double)hx is exact in FP64
+    r[1] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz),
+                       vm.x);
+    r[2] = make_float4(vm.y, vm.z, __int_as_float(id), vm.w);
+  }
+  int lo = valid ? id : INT_MAX, hi = valid ? id : INT_MIN;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x % kChunk) == 0 && valid) jp.crange[j / kChunk] = make_int2(lo, hi);
 }
 
 // Nearest-neighbour key of one pair: the top 24 bits of 1/r (a larger key is
@@ -375,6 +408,152 @@ __global__ __launch_bounds__(kThreads) void pg6_force_s(
     jx += (double)fjx;
     jy += (double)fjy;
     jz += (double)fjz;
+  }
+  if (!ivalid) return;
+  if (nsplit == 1) {
+    out.acc[3 * i + 0] = ax;
+    out.acc[3 * i + 1] = ay;
+    out.acc[3 * i + 2] = az;
+    out.jrk[3 * i + 0] = jx;
+    out.jrk[3 * i + 1] = jy;
+    out.jrk[3 * i + 2] = jz;
+    out.pot[i] = pot;
+    if (NN) {
+      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+    }
+  } else {
+    const size_t o = (size_t)blockIdx.y * ni + i;
+    part.ax[o] = ax;
+    part.ay[o] = ay;
+    part.az[o] = az;
+    part.pot[o] = pot;
+    part.jx[o] = jx;
+    part.jy[o] = jy;
+    part.jz[o] = jz;
+    if (NN) {
+      part.rmax[o] = rmax;
+      part.nn[o] = nn;
+    }
+  }
+}
+
+// Accumulators of one i-particle over one j-tile (FP32), and its pair step
+// for pg6_force_h.  SELF = whether the pair may be the self pair and must
+// be tested by id.
+struct TileAcc {
+  float ax, ay, az, pot, jx, jy, jz;
+  unsigned key;
+};
+
+template <bool SELF, bool NN>
+__device__ __forceinline__ void pg6_pair_h(TileAcc& t, float dx, float dy, float dz, float dvx,
+                                           float dvy, float dvz, float mj, float eps2,
+                                           bool same_id, unsigned slot) {
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+  const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+  const float rinv = (SELF && same_id) ? 0.f : __builtin_amdgcn_rsqf(r2);
+  const float rinv2 = rinv * rinv;
+  const float mr = mj * rinv;
+  const float mr3 = mr * rinv2;
+  const float beta = -3.f * rv * rinv2;
+  t.pot -= mr;
+  t.ax = fmaf(mr3, dx, t.ax);
+  t.ay = fmaf(mr3, dy, t.ay);
+  t.az = fmaf(mr3, dz, t.az);
+  t.jx = fmaf(mr3, fmaf(beta, dx, dvx), t.jx);
+  t.jy = fmaf(mr3, fmaf(beta, dy, dvy), t.jy);
+  t.jz = fmaf(mr3, fmaf(beta, dz, dvz), t.jz);
+  if (NN) t.key = max(t.key, nn_key(rinv, 0u, slot));
+}
+
+// Variant of pg6_force_s with no FP64 arithmetic and no id test in most of
+// the pair loop:
+//  * dx = (x_hi,j - x_hi,i) + (x_lo,j - x_lo,i), three FP32 operations on the
+//    two-float positions (JPred::ph/pl), in place of an FP64 subtraction and
+//    an FP64->FP32 conversion.  It differs from the FP32-rounded FP64
+//    difference by at most about one FP32 ulp of dx (the hi difference is
+//    exact for nearby pairs), which is the size of that rounding itself;
+//  * the j-range is walked in chunks of kChunk addresses; a chunk whose id
+//    range (JPred::crange) is disjoint from the id range of the wavefront's
+//    i-particles cannot hold a self pair, and runs without the per-pair id
+//    test.  The test is exact, so the result does not depend on it.
+// Block = 64, 128 or 256 threads, one i-particle per lane, as pg6_force_s;
+// jper must be a multiple of kChunk.  Same outputs and nearest-neighbour
+// rule (1/r from the dx above).
+template <bool NN>
+__global__ __launch_bounds__(kThreads) void pg6_force_h(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp, Partials part,
+    Outputs out, int nsplit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const float xih = (float)xi, yih = (float)yi, zih = (float)zi;
+  const float xil = (float)(xi - (double)xih);
+  const float yil = (float)(yi - (double)yih);
+  const float zil = (float)(zi - (double)zih);
+  const float vxi = (float)ivel[3 * ic + 0];
+  const float vyi = (float)ivel[3 * ic + 1];
+  const float vzi = (float)ivel[3 * ic + 2];
+  const int idi = iid[ic];
+  // id range of this wavefront's i-particles (padding lanes repeat i = ni-1)
+  int wlo = idi, whi = idi;
+  for (int o = 32; o > 0; o >>= 1) {
+    wlo = min(wlo, __shfl_xor(wlo, o));
+    whi = max(whi, __shfl_xor(whi, o));
+  }
+  wlo = __builtin_amdgcn_readfirstlane(wlo);
+  whi = __builtin_amdgcn_readfirstlane(whi);
+  cint* pcr = (cint*)jp.crange;
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min(nj, j0 + jper);
+  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+  float rmax = 0.f;
+  int nn = -1;
+  const unsigned kmask = 0xFFFFFF00u;
+  for (int jt = j0; jt < j1; jt += kJTile) {
+    const int nt = min(kJTile, j1 - jt);
+    TileAcc t{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u};
+    for (int c = 0; c < nt; c += kChunk) {
+      const int ce = min(nt, c + kChunk);
+      const int ch = (jt + c) / kChunk;
+      const bool clear = pcr[2 * ch + 1] < wlo || pcr[2 * ch] > whi;
+      cfloat_s* q = (cfloat_s*)(jp.rec + 3 * (size_t)(jt + c));
+#define PG6_PAIR_H(SELF)                                                                  \
+  {                                                                                       \
+    const float dx = (q[0] - xih) + (q[4] - xil);                                         \
+    const float dy = (q[1] - yih) + (q[5] - yil);                                         \
+    const float dz = (q[2] - zih) + (q[6] - zil);                                         \
+    pg6_pair_h<SELF, NN>(t, dx, dy, dz, q[7] - vxi, q[8] - vyi, q[9] - vzi, q[3], eps2,   \
+                         SELF && __float_as_int(q[10]) == idi, (unsigned)(255 - jj));     \
+  }
+      if (clear) {
+#pragma unroll 4
+        for (int jj = c; jj < ce; ++jj, q += 12) PG6_PAIR_H(false)
+      } else {
+#pragma unroll 4
+        for (int jj = c; jj < ce; ++jj, q += 12) PG6_PAIR_H(true)
+      }
+#undef PG6_PAIR_H
+    }
+    if (NN) {
+      const float rq = __uint_as_float(t.key & kmask);
+      if (rq > rmax) {
+        rmax = rq;
+        nn = jt + 255 - (int)(t.key & 0xFFu);
+      }
+    }
+    ax += (double)t.ax;
+    ay += (double)t.ay;
+    az += (double)t.az;
+    pot += (double)t.pot;
+    jx += (double)t.jx;
+    jy += (double)t.jy;
+    jz += (double)t.jz;
   }
   if (!ivalid) return;
   if (nsplit == 1) {

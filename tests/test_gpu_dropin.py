@@ -30,10 +30,17 @@ def test_direct_leapfrog_with_libpg5(tmp_path):
     gpu = dropin.direct_energy_errors(dropin.run("direct_pgamd", ["pl", "gpu.out"], tmp_path))
     assert len(gpu) == len(ref) == 9
     assert np.abs(gpu).max() < 2 * np.abs(ref).max() + 1e-6, (gpu, ref)
-    # the final states agree to the chaos of 1000 steps of slightly different forces
+    # 1000 steps of slightly different forces decorrelate individual orbits
+    # (a Plummer sphere is chaotic), so the final states are compared through
+    # the cluster's structure: its Lagrangian radii about the centre of mass
     a = np.loadtxt(tmp_path / "ref.out", skiprows=3 + len(m), max_rows=len(m))
     b = np.loadtxt(tmp_path / "gpu.out", skiprows=3 + len(m), max_rows=len(m))
-    assert np.median(np.linalg.norm(a - b, axis=1)) < 1e-2
+
+    def lagrangian_radii(xf):
+        r = np.sort(np.linalg.norm(xf - (m[:, None] * xf).sum(0) / m.sum(), axis=1))
+        return r[[len(r) // 10, len(r) // 4, len(r) // 2, 3 * len(r) // 4]]
+
+    np.testing.assert_allclose(lagrangian_radii(b), lagrangian_radii(a), rtol=0.05)
 
 
 def test_banana_hermite_with_libpg6(tmp_path):

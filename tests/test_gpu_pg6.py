@@ -70,10 +70,10 @@ def test_matches_reference_fixture(g6):
 
 
 @pytest.mark.parametrize("eps", [1.0 / 256, 0.0])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("ni", [1, 7, 48, 64, 500, 4096])
 def test_force_vs_fp64_oracle(g6, ni, variant, eps):
-    """Both kernels against the FP64 oracle, softened and unsoftened (the
+    """Every kernel against the FP64 oracle, softened and unsoftened (the
     reference's sample integrator runs with eps = 0)."""
     assert g6.lib.pg6_dev_set_kernel(variant) == 0
     n = 4096
@@ -106,6 +106,39 @@ def test_force_vs_fp64_oracle(g6, ni, variant, eps):
     g6.lib.pg6_dev_set_kernel(-1)
 
 
+@pytest.mark.parametrize("eps", [1.0 / 256, 0.0])
+@pytest.mark.parametrize("ids", ["address", "permuted"])
+def test_self_filter(g6, ids, eps):
+    """pg6_force_h skips the per-pair id test on j-chunks whose id range
+    misses the wavefront's: i-particles taken in address order (most chunks
+    skip the test when ids follow addresses, none when they are permuted),
+    with eps = 0 so that a self pair that slipped through would give NaN."""
+    n = 4096
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 5, eps)
+    if ids == "permuted":
+        idx = (1000 + np.random.default_rng(9).permutation(n)).astype(np.int32)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    ti = 1.0 / 512
+    g6.set_ti(ti)
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    vj = vo.astype(np.float32).astype(np.float64)
+    for sel in (np.arange(n), np.arange(1000, 3000), np.arange(77, 141)):
+        assert g6.lib.pg6_dev_set_kernel(2) == 0
+        r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+        assert g6.lib.pg6_dev_set_kernel(1) == 0
+        r1 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+        o = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo, vj, m, idx)
+        for k in ("acc", "jrk", "pot"):
+            assert np.all(np.isfinite(r[k])), k
+        assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 1e-6
+        assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 1e-5
+        assert rel_err(r["pot"], o["pot"]).max() < 1e-6
+        # the two-float dx moves results by about one FP32 ulp of dx
+        assert np.percentile(rel_err(r["acc"], r1["acc"]), 99) < 3e-7
+        assert np.mean(r["nnb"] == r1["nnb"]) > 0.99
+    g6.lib.pg6_dev_set_kernel(-1)
+
+
 def test_autotune_and_launch_shapes(g6):
     n = 4096
     eps = 1.0 / 256
@@ -119,13 +152,14 @@ def test_autotune_and_launch_shapes(g6):
     tuned = g6.autotune(n, eps * eps, reps=1)
     assert sorted(tuned) == [64 << k for k in range(7)]
     for ni, (var, bi, bpc, us) in tuned.items():
-        assert var in (0, 1) and bi in (64, 128, 256) and bpc in (4, 8, 16, 32, 64) and us > 0
+        assert var in (0, 1, 2) and bi in (64, 128, 256) and bpc in (4, 8, 16, 32, 64) and us > 0
         p = g6.plan(ni, n)
         assert (p["variant"], p["bi"], p["bpc"]) == (var, bi, bpc)
         assert p["nsplit"] * p["jper"] >= n
     # every launch shape gives the same result up to the FP32 tile-sum order
     # (a shape changes which j share a tile partial sum)
-    for var, bi, bpc in [(0, 64, 4), (0, 256, 64), (1, 64, 64), (1, 128, 4), (-1, 0, 0)]:
+    for var, bi, bpc in [(0, 64, 4), (0, 256, 64), (1, 64, 64), (1, 128, 4), (2, 64, 64),
+                         (2, 256, 16), (-1, 0, 0)]:
         assert g6.lib.pg6_dev_set_kernel(var) == 0
         assert g6.lib.pg6_dev_set_launch(bi, bpc, 64) == 0
         r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
@@ -136,7 +170,7 @@ def test_autotune_and_launch_shapes(g6):
     assert g6.lib.pg6_dev_autotune_reset() == 0
     g6.lib.pgamd_set_abort_on_error(0)
     try:
-        assert g6.lib.pg6_dev_set_kernel(2) == -1 and g6.lib.pg6_dev_set_launch(96, 0, 0) == -1
+        assert g6.lib.pg6_dev_set_kernel(3) == -1 and g6.lib.pg6_dev_set_launch(96, 0, 0) == -1
         assert g6.lib.pg6_dev_set_launch(0, 1000, 0) == -1
         assert g6.lib.pg6_dev_autotune(32, eps * eps, 1, None) == -1
     finally:
