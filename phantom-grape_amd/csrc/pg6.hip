@@ -59,14 +59,16 @@ inline int ni_bucket(int ni) {
   return k;
 }
 
-// Default choice, from a sweep at nj = 65536 on MI355X: the LDS-tile kernel
-// for small and medium i-sets, the scalar-broadcast kernel for large ones,
-// and many j-ranges (16 blocks of 256 threads per CU) so that the last wave
-// of blocks is a small fraction of the launch.
+// Default choice, from the autotuner's picks at nj = 65536 on MI355X: the
+// LDS-tile kernel for small and medium i-sets, the two-float scalar-broadcast
+// kernel for large ones, and many j-ranges so that the last wave of blocks is
+// a small fraction of the launch.
 inline LaunchChoice default_choice(int ni) {
-  if (ni <= 512) return {0, 64, 8};
-  if (ni <= 8192) return {0, 128, 16};
-  return {1, 256, 16};
+  if (ni <= 128) return {0, 64, 32};
+  if (ni <= 4096) return {0, 64, 8};
+  if (ni <= 8192) return {2, 128, 16};
+  if (ni <= 32768) return {2, 128, 32};
+  return {2, 128, 64};
 }
 
 struct Board {
