@@ -1172,25 +1172,31 @@ struct Hermite {
   int n = 0;
   double eps2 = 0.0;
   pg6h::Params P{};
-  double t_gl = 0.0;   // time inside the current window [0, dtmax]
-  long long t_mod = 0; // completed windows (globaltime.h: t = t_mod * dtmax + t_gl)
-  long long nsteps = 0, nblocks = 0;
   DevBuf state, work;
   pg6h::State S{};
-  // work arrays
+  pg6h::Ctl* ctl = nullptr;  // device
+  // work arrays: active list and its predicted i data, force partials
   int* act = nullptr;
   double *ipos = nullptr, *ivel = nullptr, *acc1 = nullptr, *jrk1 = nullptr, *pot1 = nullptr;
   int* iid = nullptr;
-  int *bcount = nullptr, *boff = nullptr, *scal_i = nullptr;  // scal_i[0] = nact, [1] = bad
-  double *part = nullptr, *scal_d = nullptr;                    // scal_d[0] = t_next
-  HostBuf host;  // pinned {t_next, nact, bad}
+  double* epart = nullptr;  // energy partial sums, 2 per block
+  Partials part{};
+  int target = 0, min_jper = 16;  // force split (pg6h_plan)
+  int min_blocks = 0, force_blocks = 0, finish_blocks = 0;
+  HostBuf host;  // pinned: a Ctl image, then t_end
+  // a captured batch of kGraphSteps steps, and the board arrays it was captured with
+  hipGraphExec_t graph = nullptr;
+  const void* graph_key[2] = {nullptr, nullptr};
 };
+
+constexpr int kGraphSteps = 32;
 
 Hermite g_h;
 
-constexpr int kHermBlocks = 256;  // blocks of the min / energy reductions
+constexpr int kHermBlocks = 256;  // blocks of the energy reduction
 
 void hermite_release(Hermite& h) {
+  if (h.graph) PGAMD_HIP_CHECK(hipGraphExecDestroy(h.graph));
   h.state.release();
   h.work.release();
   h.host.release();
@@ -1202,6 +1208,63 @@ void hermite_release(Hermite& h) {
 void hermite_force(Board& b, Hermite& h, int ni) {
   const Outputs o{h.acc1, h.jrk1, h.pot1, nullptr, nullptr};
   board_force(b, ni, h.n, h.ipos, h.ivel, h.iid, h.eps2, o, b.stream);
+}
+
+// Queues one block step (three kernels, no host synchronisation).
+void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
+  const int n = h.n;
+  hipLaunchKernelGGL(pg6h::pg6h_select, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
+                     h.ctl, b.js, b.jp, h.act, h.ipos, h.ivel, h.iid);
+  hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(256), 0, st, n, (float)h.eps2,
+                     h.P, h.ctl, h.target, h.min_jper, h.ipos, h.ivel, h.iid, b.jp, h.part);
+  hipLaunchKernelGGL(pg6h::pg6h_finish, dim3(h.finish_blocks), dim3(256), 0, st, n, h.P, h.ctl,
+                     h.target, h.min_jper, h.act, h.S, h.ipos, h.ivel, h.part, b.js);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  b.nj_pred = -1;  // the board's predictions now belong to the integrator
+  b.predict_flag = 1;
+}
+
+// Queues kGraphSteps block steps as one captured graph (captured on first
+// use, again when the board's arrays have moved).  The steps are the same
+// kernels with the same arguments as hermite_enqueue's; none clears memory.
+void hermite_enqueue_batch(Board& b, Hermite& h, hipStream_t st) {
+  if (h.graph && (h.graph_key[0] != b.js.x || h.graph_key[1] != b.jp.x)) {
+    PGAMD_HIP_CHECK(hipGraphExecDestroy(h.graph));
+    h.graph = nullptr;
+  }
+  if (!h.graph) {
+    hipGraph_t g = nullptr;
+    PGAMD_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < kGraphSteps; ++k) hermite_enqueue(b, h, st);
+    PGAMD_HIP_CHECK(hipStreamEndCapture(st, &g));
+    PGAMD_HIP_CHECK(hipGraphInstantiate(&h.graph, g, nullptr, nullptr, 0));
+    PGAMD_HIP_CHECK(hipGraphDestroy(g));
+    h.graph_key[0] = b.js.x;
+    h.graph_key[1] = b.jp.x;
+  }
+  PGAMD_HIP_CHECK(hipGraphLaunch(h.graph, st));
+  b.nj_pred = -1;
+  b.predict_flag = 1;
+}
+
+// Sets Ctl::t_end (through the pinned buffer; the caller synchronises before
+// the buffer is written again).
+void hermite_set_end(Hermite& h, double t_end, hipStream_t st) {
+  double* p = reinterpret_cast<double*>(h.host.as<char>() + 256);
+  *p = t_end;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(&h.ctl->t_end, p, sizeof(double), hipMemcpyHostToDevice, st));
+}
+
+// Copies Ctl to the host (synchronous).
+pg6h::Ctl hermite_read(Hermite& h, hipStream_t st) {
+  pg6h::Ctl* c = h.host.as<pg6h::Ctl>();
+  PGAMD_HIP_CHECK(hipMemcpyAsync(c, h.ctl, sizeof(pg6h::Ctl), hipMemcpyDeviceToHost, st));
+  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  return *c;
+}
+
+double hermite_time(const Hermite& h, const pg6h::Ctl& c) {
+  return (double)c.t_mod * h.P.dtmax + c.t_gl;
 }
 
 }  // namespace
@@ -1233,6 +1296,13 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.P.eta = eta;
   h.P.dtmax = dtmax;
   h.P.dtmin = dtmax / std::pow(2.0, 51.0);  // initialize_timestep (timestep.c:18-22)
+  const int cus = device_cu_count();
+  h.target = 16 * cus;  // wavefront items per force launch: 4 per SIMD
+  h.min_blocks = std::min(ceil_div(n, 256), 4 * cus);
+  h.force_blocks = 4 * cus;
+  h.finish_blocks = std::max(1, std::min(ceil_div(n, 64), cus));
+  // largest force split: ns * ntiles <= target + ntiles (pg6h_plan)
+  const size_t nsplit_i = ((size_t)h.target + ceil_div(n, 64)) * 64;
   // state: x, v, a, j (3n each), pot, t, dt, m (n each)
   h.state.reserve(sizeof(double) * (size_t)n * 16);
   double* d = h.state.as<double>();
@@ -1244,11 +1314,11 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.S.t = d + 13 * (size_t)n;
   h.S.dt = d + 14 * (size_t)n;
   h.S.m = d + 15 * (size_t)n;
-  // work: ipos, ivel, acc1, jrk1 (3n each), pot1 (n), part (2 * blocks), scal_d (4);
-  // act, iid (n each), bcount, boff (blocks of 256 + 1), scal_i (4)
-  const int nb = ceil_div(n, 256);
-  const size_t dbytes = sizeof(double) * ((size_t)n * 13 + 2 * kHermBlocks + 4);
-  const size_t ibytes = sizeof(int) * ((size_t)n * 2 + 2 * (size_t)nb + 1 + 4);
+  // work: ipos, ivel, acc1, jrk1 (3n each), pot1 (n), epart (2 * blocks), partials (7 *
+  // nsplit_i), Ctl; act, iid (n each)
+  const size_t dbytes =
+      sizeof(double) * ((size_t)n * 13 + 2 * kHermBlocks + 7 * nsplit_i) + 256;
+  const size_t ibytes = sizeof(int) * ((size_t)n * 2);
   h.work.reserve(dbytes + ibytes);
   double* w = h.work.as<double>();
   h.ipos = w;
@@ -1256,18 +1326,21 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.acc1 = w + 6 * (size_t)n;
   h.jrk1 = w + 9 * (size_t)n;
   h.pot1 = w + 12 * (size_t)n;
-  h.part = w + 13 * (size_t)n;
-  h.scal_d = h.part + 2 * kHermBlocks;
-  int* wi = reinterpret_cast<int*>(h.scal_d + 4);
+  h.epart = w + 13 * (size_t)n;
+  double* pp = h.epart + 2 * kHermBlocks;
+  double** pf[7] = {&h.part.ax, &h.part.ay, &h.part.az, &h.part.pot,
+                    &h.part.jx, &h.part.jy, &h.part.jz};
+  for (int k = 0; k < 7; ++k) *pf[k] = pp + (size_t)k * nsplit_i;
+  h.part.rmax = nullptr;
+  h.part.nn = nullptr;
+  static_assert(sizeof(pg6h::Ctl) <= 256, "Ctl fits its slot");
+  h.ctl = reinterpret_cast<pg6h::Ctl*>(pp + 7 * nsplit_i);
+  int* wi = reinterpret_cast<int*>(reinterpret_cast<char*>(h.ctl) + 256);
   h.act = wi;
   h.iid = wi + n;
-  h.bcount = wi + 2 * (size_t)n;
-  h.boff = h.bcount + nb;
-  h.scal_i = h.boff + nb + 1;
-  h.host.reserve(64);
+  h.host.reserve(512);
   hipStream_t st = b.stream;
   // upload x, v, m; t = 0, a = j = 0
-  std::vector<double> zero((size_t)n * 3, 0.0);
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.x, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.v, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.S.m, m, sizeof(double) * n, hipMemcpyHostToDevice, st));
@@ -1281,7 +1354,15 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   std::vector<int> ids(n);
   for (int i = 0; i < n; ++i) ids[i] = i;
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.iid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemsetAsync(h.scal_i, 0, sizeof(int) * 4, st));
+  // control block: t = 0, n particle steps and 1 block step (the start-up)
+  pg6h::Ctl* c = h.host.as<pg6h::Ctl>();
+  *c = pg6h::Ctl{};
+  c->tmin = pg6h::kInfBits;
+  c->tmin_next = pg6h::kInfBits;
+  c->t_end = HUGE_VAL;
+  c->nsteps = n;
+  c->nblocks = 1;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(h.ctl, c, sizeof(pg6h::Ctl), hipMemcpyHostToDevice, st));
   // j-state of the board from the particles (a = j = 0, t = 0), predicted to 0
   b.npend = 0;
   std::fill(b.slot.begin(), b.slot.end(), -1);
@@ -1292,20 +1373,16 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   board_predict(b, n, st);
   // start-up forces of every particle, then the first steps (startup_sgmotion)
   hermite_force(b, h, n);
-  PGAMD_HIP_CHECK(hipMemsetAsync(h.scal_i, 0, sizeof(int) * 4, st));
   hipLaunchKernelGGL(pg6h::pg6h_startup, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
-                     h.acc1, h.jrk1, h.pot1, b.js, h.scal_i + 1);
+                     h.acc1, h.jrk1, h.pot1, b.js, &h.ctl->bad);
+  hipLaunchKernelGGL(pg6h::pg6h_min_time, dim3(h.min_blocks), dim3(256), 0, st, n, h.S, h.ctl);
   PGAMD_HIP_CHECK(hipGetLastError());
-  int bad = 0;
-  PGAMD_HIP_CHECK(hipMemcpyAsync(&bad, h.scal_i + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  const pg6h::Ctl r = hermite_read(h, st);
   b.predict_flag = 1;
   b.nj_pred = -1;
   h.on = true;
-  h.nsteps = n;
-  h.nblocks = 1;
-  if (bad) {
-    api_error(6, "pg6_hermite_init: %d particles have no admissible first step", bad);
+  if (r.bad) {
+    api_error(6, "pg6_hermite_init: %d particles have no admissible first step", r.bad);
     return -1;
   }
   return 0;
@@ -1325,74 +1402,46 @@ PGAMD_API int pg6_hermite_step(double* t_out, int* nact_out) {
   }
   Board& b = g_board;
   hipStream_t st = b.stream;
-  const int n = h.n;
-  const int nb = ceil_div(n, 256);
-  hipLaunchKernelGGL(pg6h::pg6h_next_time_part, dim3(kHermBlocks), dim3(256), 0, st, n, h.S,
-                     h.part);
-  hipLaunchKernelGGL(pg6h::pg6h_next_time_final, dim3(1), dim3(256), 0, st, kHermBlocks, h.part,
-                     h.scal_d);
-  hipLaunchKernelGGL(pg6h::pg6h_count, dim3(nb), dim3(256), 0, st, n, h.S, h.scal_d, h.bcount);
-  hipLaunchKernelGGL(pg6h::pg6h_scan, dim3(1), dim3(64), 0, st, nb, h.bcount, h.boff, h.scal_i);
-  hipLaunchKernelGGL(pg6h::pg6h_compact, dim3(nb), dim3(256), 0, st, n, h.S, h.scal_d, h.boff,
-                     h.act);
-  PGAMD_HIP_CHECK(hipGetLastError());
-  double* hp = h.host.as<double>();
-  PGAMD_HIP_CHECK(hipMemcpyAsync(hp, h.scal_d, sizeof(double), hipMemcpyDeviceToHost, st));
-  PGAMD_HIP_CHECK(
-      hipMemcpyAsync(hp + 1, h.scal_i, sizeof(int), hipMemcpyDeviceToHost, st));
-  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-  const double tnext = hp[0];
-  int nact = 0;
-  std::memcpy(&nact, hp + 1, sizeof(int));
-  if (nact <= 0 || nact > n) {
-    api_error(6, "pg6_hermite_step: no active particle (t_next = %g)", tnext);
+  hermite_set_end(h, HUGE_VAL, st);
+  hermite_enqueue(b, h, st);
+  const pg6h::Ctl c = hermite_read(h, st);
+  if (c.last_nact <= 0 || c.last_nact > h.n) {
+    api_error(6, "pg6_hermite_step: no active particle");
     return -1;
   }
-  // predict all j to t_next, the active i as predict_exmotion does, force, correct
-  b.ti = tnext;
-  board_predict(b, n, st);
-  b.predict_flag = 0;
-  hipLaunchKernelGGL(pg6h::pg6h_predict_i, dim3(ceil_div(nact, 256)), dim3(256), 0, st, nact,
-                     h.act, h.S, h.scal_d, h.ipos, h.ivel, h.iid);
-  PGAMD_HIP_CHECK(hipGetLastError());
-  hermite_force(b, h, nact);
-  hipLaunchKernelGGL(pg6h::pg6h_correct, dim3(ceil_div(nact, 256)), dim3(256), 0, st, nact,
-                     h.act, h.S, h.P, h.ipos, h.ivel, h.acc1, h.jrk1, h.pot1, b.js,
-                     h.scal_i + 1);
-  PGAMD_HIP_CHECK(hipGetLastError());
-  h.t_gl = tnext;
-  h.nsteps += nact;
-  h.nblocks += 1;
-  if (h.t_gl == h.P.dtmax) {  // end of a window: every particle is synchronised
-    hipLaunchKernelGGL(pg6h::pg6h_reset, dim3(nb), dim3(256), 0, st, n, h.S, b.js);
-    PGAMD_HIP_CHECK(hipGetLastError());
-    h.t_gl = 0.0;
-    h.t_mod += 1;
-  }
-  b.nj_pred = -1;
-  if (t_out) *t_out = (double)h.t_mod * h.P.dtmax + h.t_gl;
-  if (nact_out) *nact_out = nact;
+  if (t_out) *t_out = hermite_time(h, c);
+  if (nact_out) *nact_out = c.last_nact;
   return 0;
 }
 
-// Block steps until t = t_mod * dtmax + t_gl reaches t_end.  Returns the
+// Block steps until t = t_mod * dtmax + t_gl reaches t_end (finite).  Steps
+// are queued in growing batches without waiting for each other; those queued
+// past t_end find it reached on the device and do nothing.  Returns the
 // number of block steps taken, or -1.
 PGAMD_API long long pg6_hermite_run(double t_end) {
-  long long k = 0;
-  for (;;) {
-    double t = 0.0;
-    {
-      std::lock_guard<std::mutex> lk(g_mutex);
-      if (!g_h.on) {
-        api_error(1, "pg6_hermite_run: no integration started (pg6_hermite_init)");
-        return -1;
-      }
-      t = (double)g_h.t_mod * g_h.P.dtmax + g_h.t_gl;
-    }
-    if (t >= t_end) return k;
-    if (pg6_hermite_step(nullptr, nullptr) != 0) return -1;
-    ++k;
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!board_check_open("pg6_hermite_run")) return -1;
+  Hermite& h = g_h;
+  if (!h.on) {
+    api_error(1, "pg6_hermite_run: no integration started (pg6_hermite_init)");
+    return -1;
   }
+  if (!std::isfinite(t_end)) {
+    api_error(3, "pg6_hermite_run: t_end = %g is not finite", t_end);
+    return -1;
+  }
+  Board& b = g_board;
+  hipStream_t st = b.stream;
+  pg6h::Ctl c = hermite_read(h, st);
+  const long long blocks0 = c.nblocks;
+  hermite_set_end(h, t_end, st);
+  int batch = 1;  // graphs of kGraphSteps steps
+  while (hermite_time(h, c) < t_end) {
+    for (int k = 0; k < batch; ++k) hermite_enqueue_batch(b, h, st);
+    c = hermite_read(h, st);
+    batch = std::min(2 * batch, 8);
+  }
+  return c.nblocks - blocks0;
 }
 
 // Copies the integrator state to host arrays (each may be null): x, v, a, j
@@ -1419,11 +1468,11 @@ PGAMD_API int pg6_hermite_state(double* x, double* v, double* a, double* j, doub
   get(pot, h.S.pot, n);
   get(t, h.S.t, n);
   get(dt, h.S.dt, n);
-  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+  const pg6h::Ctl c = hermite_read(h, st);
   if (info) {
-    info[0] = h.t_mod;
-    info[1] = h.nsteps;
-    info[2] = h.nblocks;
+    info[0] = c.t_mod;
+    info[1] = c.nsteps;
+    info[2] = c.nblocks;
     info[3] = h.n;
   }
   return 0;
@@ -1441,10 +1490,10 @@ PGAMD_API int pg6_hermite_energy(double* ek, double* ep) {
   }
   hipStream_t st = g_board.stream;
   hipLaunchKernelGGL(pg6h::pg6h_energy_part, dim3(kHermBlocks), dim3(256), 0, st, h.n, h.S,
-                     h.part);
+                     h.epart);
   PGAMD_HIP_CHECK(hipGetLastError());
   std::vector<double> p(2 * kHermBlocks);
-  PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), h.part, sizeof(double) * p.size(),
+  PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), h.epart, sizeof(double) * p.size(),
                                  hipMemcpyDeviceToHost, st));
   PGAMD_HIP_CHECK(hipStreamSynchronize(st));
   double k = 0.0, w = 0.0;

@@ -102,11 +102,11 @@ __device__ inline double pg6_pred_vel(double dt, double v, double a2, double j6)
   return v + (2.0 * a2) * dt + (3.0 * j6) * dt * dt;
 }
 
-// Block = 256 threads (whole wavefronts, so that wavefront w of the grid
-// covers exactly j-chunk w).
-__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = j < nj;
+// Predicts j-particle j (valid = j < nj) to time ti into every JPred array.
+// Called by whole wavefronts whose lane l holds j = 64 w + l, so that the
+// wavefront fills in the id range of j-chunk w.
+__device__ inline void pg6_predict_j(int j, bool valid, double ti, const JState& js,
+                                     const JPred& jp) {
   int id = 0;
   if (valid) {
     const double dt = ti - js.t[j];
@@ -137,7 +137,13 @@ __global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js,
     lo = min(lo, __shfl_xor(lo, o));
     hi = max(hi, __shfl_xor(hi, o));
   }
-  if ((threadIdx.x % kChunk) == 0 && valid) jp.crange[j / kChunk] = make_int2(lo, hi);
+  if ((j % kChunk) == 0 && valid) jp.crange[j / kChunk] = make_int2(lo, hi);
+}
+
+// Block = 256 threads, one j per thread.
+__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  pg6_predict_j(j, j < nj, ti, js, jp);
 }
 
 // Nearest-neighbour key of one pair: the top 24 bits of 1/r (a larger key is
@@ -467,40 +473,38 @@ __device__ __forceinline__ void pg6_pair_h(TileAcc& t, float dx, float dy, float
   if (NN) t.key = max(t.key, nn_key(rinv, 0u, slot));
 }
 
-// Variant of pg6_force_s with no FP64 arithmetic and no id test in most of
-// the pair loop:
+// Sums of one i-particle over one j-range (FP64 over FP32 tile sums), and the
+// quantised 1/r and j address of its nearest neighbour (NN only).
+struct RangeSums {
+  double ax, ay, az, pot, jx, jy, jz;
+  float rmax;
+  int nn;
+};
+
+// Pair loop of pg6_force_h for the i-particle of this lane against the
+// j-addresses [j0, j1):
 //  * dx = (x_hi,j - x_hi,i) + (x_lo,j - x_lo,i), three FP32 operations on the
-//    two-float positions (JPred::ph/pl), in place of an FP64 subtraction and
-//    an FP64->FP32 conversion.  It differs from the FP32-rounded FP64
-//    difference by at most about one FP32 ulp of dx (the hi difference is
-//    exact for nearby pairs), which is the size of that rounding itself;
-//  * the j-range is walked in chunks of kChunk addresses; a chunk whose id
-//    range (JPred::crange) is disjoint from the id range of the wavefront's
-//    i-particles cannot hold a self pair, and runs without the per-pair id
-//    test.  The test is exact, so the result does not depend on it.
-// Block = 64, 128 or 256 threads, one i-particle per lane, as pg6_force_s;
-// jper must be a multiple of kChunk.  Same outputs and nearest-neighbour
-// rule (1/r from the dx above).
+//    two-float positions (JPred::rec), in place of an FP64 subtraction and an
+//    FP64->FP32 conversion.  It differs from the FP32-rounded FP64 difference
+//    by at most about one FP32 ulp of dx (the hi difference is exact for
+//    nearby pairs), which is the size of that rounding itself;
+//  * the j-range is walked in pieces that stay inside one chunk of kChunk
+//    addresses; a piece whose chunk's id range (JPred::crange) is disjoint
+//    from the id range of the wavefront's i-particles cannot hold a self
+//    pair, and runs without the per-pair id test.  The test is exact, so the
+//    result does not depend on it.
+// FP32 sums run over tiles of kJTile addresses from j0 and are folded into
+// FP64 after each tile.  Every lane of the wavefront calls it with the same
+// j0, j1 (the j data are wave-uniform and arrive through scalar loads).
 template <bool NN>
-__global__ __launch_bounds__(kThreads) void pg6_force_h(
-    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
-    const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp, Partials part,
-    Outputs out, int nsplit) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ivalid = i < ni;
-  const int ic = ivalid ? i : ni - 1;
-  const double xi = ipos[3 * ic + 0];
-  const double yi = ipos[3 * ic + 1];
-  const double zi = ipos[3 * ic + 2];
+__device__ __forceinline__ RangeSums pg6_range_h(double xi, double yi, double zi, float vxi,
+                                                 float vyi, float vzi, int idi, float eps2,
+                                                 int j0, int j1, const JPred& jp) {
   const float xih = (float)xi, yih = (float)yi, zih = (float)zi;
   const float xil = (float)(xi - (double)xih);
   const float yil = (float)(yi - (double)yih);
   const float zil = (float)(zi - (double)zih);
-  const float vxi = (float)ivel[3 * ic + 0];
-  const float vyi = (float)ivel[3 * ic + 1];
-  const float vzi = (float)ivel[3 * ic + 2];
-  const int idi = iid[ic];
-  // id range of this wavefront's i-particles (padding lanes repeat i = ni-1)
+  // id range of this wavefront's i-particles
   int wlo = idi, whi = idi;
   for (int o = 32; o > 0; o >>= 1) {
     wlo = min(wlo, __shfl_xor(wlo, o));
@@ -509,18 +513,14 @@ __global__ __launch_bounds__(kThreads) void pg6_force_h(
   wlo = __builtin_amdgcn_readfirstlane(wlo);
   whi = __builtin_amdgcn_readfirstlane(whi);
   cint* pcr = (cint*)jp.crange;
-  const int j0 = blockIdx.y * jper;
-  const int j1 = min(nj, j0 + jper);
-  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
-  float rmax = 0.f;
-  int nn = -1;
+  RangeSums r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f, -1};
   const unsigned kmask = 0xFFFFFF00u;
   for (int jt = j0; jt < j1; jt += kJTile) {
     const int nt = min(kJTile, j1 - jt);
     TileAcc t{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u};
-    for (int c = 0; c < nt; c += kChunk) {
-      const int ce = min(nt, c + kChunk);
+    for (int c = 0, ce; c < nt; c = ce) {
       const int ch = (jt + c) / kChunk;
+      ce = min(nt, (ch + 1) * kChunk - jt);  // to the end of the chunk or the tile
       const bool clear = pcr[2 * ch + 1] < wlo || pcr[2 * ch] > whi;
       cfloat_s* q = (cfloat_s*)(jp.rec + 3 * (size_t)(jt + c));
 #define PG6_PAIR_H(SELF)                                                                  \
@@ -542,44 +542,66 @@ __global__ __launch_bounds__(kThreads) void pg6_force_h(
     }
     if (NN) {
       const float rq = __uint_as_float(t.key & kmask);
-      if (rq > rmax) {
-        rmax = rq;
-        nn = jt + 255 - (int)(t.key & 0xFFu);
+      if (rq > r.rmax) {
+        r.rmax = rq;
+        r.nn = jt + 255 - (int)(t.key & 0xFFu);
       }
     }
-    ax += (double)t.ax;
-    ay += (double)t.ay;
-    az += (double)t.az;
-    pot += (double)t.pot;
-    jx += (double)t.jx;
-    jy += (double)t.jy;
-    jz += (double)t.jz;
+    r.ax += (double)t.ax;
+    r.ay += (double)t.ay;
+    r.az += (double)t.az;
+    r.pot += (double)t.pot;
+    r.jx += (double)t.jx;
+    r.jy += (double)t.jy;
+    r.jz += (double)t.jz;
   }
+  return r;
+}
+
+// Variant of pg6_force_s with no FP64 arithmetic and no id test in most of
+// the pair loop (pg6_range_h).  Block = 64, 128 or 256 threads, one
+// i-particle per lane, as pg6_force_s.
+// Same outputs and nearest-neighbour rule (1/r from the two-float dx).
+template <bool NN>
+__global__ __launch_bounds__(kThreads) void pg6_force_h(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp, Partials part,
+    Outputs out, int nsplit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;  // padding lanes repeat i = ni-1
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const int j0 = blockIdx.y * jper;
+  const RangeSums r = pg6_range_h<NN>(xi, yi, zi, (float)ivel[3 * ic + 0],
+                                      (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], iid[ic],
+                                      eps2, j0, min(nj, j0 + jper), jp);
   if (!ivalid) return;
   if (nsplit == 1) {
-    out.acc[3 * i + 0] = ax;
-    out.acc[3 * i + 1] = ay;
-    out.acc[3 * i + 2] = az;
-    out.jrk[3 * i + 0] = jx;
-    out.jrk[3 * i + 1] = jy;
-    out.jrk[3 * i + 2] = jz;
-    out.pot[i] = pot;
+    out.acc[3 * i + 0] = r.ax;
+    out.acc[3 * i + 1] = r.ay;
+    out.acc[3 * i + 2] = r.az;
+    out.jrk[3 * i + 0] = r.jx;
+    out.jrk[3 * i + 1] = r.jy;
+    out.jrk[3 * i + 2] = r.jz;
+    out.pot[i] = r.pot;
     if (NN) {
-      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
-      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+      out.nnb[i] = r.nn >= 0 ? jp.id[r.nn] : -1;
+      out.rnnb[i] = r.nn >= 0 ? nn_distance(jp, r.nn, xi, yi, zi, eps2) : 0.f;
     }
   } else {
     const size_t o = (size_t)blockIdx.y * ni + i;
-    part.ax[o] = ax;
-    part.ay[o] = ay;
-    part.az[o] = az;
-    part.pot[o] = pot;
-    part.jx[o] = jx;
-    part.jy[o] = jy;
-    part.jz[o] = jz;
+    part.ax[o] = r.ax;
+    part.ay[o] = r.ay;
+    part.az[o] = r.az;
+    part.pot[o] = r.pot;
+    part.jx[o] = r.jx;
+    part.jy[o] = r.jy;
+    part.jz[o] = r.jz;
     if (NN) {
-      part.rmax[o] = rmax;
-      part.nn[o] = nn;
+      part.rmax[o] = r.rmax;
+      part.nn[o] = r.nn;
     }
   }
 }

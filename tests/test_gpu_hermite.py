@@ -102,6 +102,32 @@ def test_integration_conserves_energy_like_the_reference(g6):
         assert abs(info["blocks"] - (blocks + 1)) < 0.2 * info["blocks"]
 
 
+def test_run_in_graphs_equals_single_steps(g6):
+    """pg6_hermite_run queues captured graphs of steps (replayed many times,
+    with host reads between batches, and steps past t_end skipped on the
+    device); stepping the same integration one call at a time ends in the
+    same state, bit for bit."""
+    n, eps2 = 1024, (1 / 64) ** 2
+    m, x, v = ics.plummer(n, 34)
+    t_end = 3.0 / 16
+    g6.hermite_init(x, v, m, eps2, ETA_S, ETA, DTMAX)
+    blocks = g6.hermite_run(t_end)
+    a = g6.hermite_state()
+    assert blocks > 100  # several graphs of steps
+    g6.hermite_init(x, v, m, eps2, ETA_S, ETA, DTMAX)
+    t, k = 0.0, 0
+    while t < t_end:
+        t, _ = g6.hermite_step()
+        k += 1
+    b = g6.hermite_state()
+    assert k == blocks
+    for key in ("x", "v", "a", "j", "pot", "t", "dt", "t_mod", "steps", "blocks"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    # a second run continues where the first stopped
+    assert g6.hermite_run(t_end) == 0
+    assert g6.hermite_run(t_end + 1.0 / 64) > 0
+
+
 def test_errors_are_reported(g6):
     import pgamd
     g6.lib.pgamd_set_abort_on_error(0)
