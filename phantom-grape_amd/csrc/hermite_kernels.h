@@ -105,9 +105,13 @@ __device__ inline bool pg6h_skip(const Ctl* c, double dtmax) {
 }
 
 // Split of the force work of `nact` active particles over all nj
-// j-particles: tiles of 64 i (one wavefront each) times ns j-ranges of jper
-// addresses (a multiple of 16, at least min_jper), about `target` items in
-// all.  Partials of j-range s for active slot k sit at s * stride + k.
+// j-particles: tiles of 64 i times ns j-ranges of jper addresses, one block
+// of kForceWaves wavefronts per (tile, range) item; wavefront w of the block
+// takes the w-th of kForceWaves equal pieces of the range (a multiple of 16
+// addresses, at least min_jper).  About `target` wavefronts of work in all.
+// Partials of range s for active slot k sit at s * stride + k.
+constexpr int kForceWaves = 8;
+
 struct Plan {
   int ntiles, ns, jper, stride;
 };
@@ -115,10 +119,12 @@ struct Plan {
 __host__ __device__ inline Plan pg6h_plan(int nact, int nj, int target, int min_jper) {
   Plan p;
   p.ntiles = (nact + 63) / 64;
-  int ns = p.ntiles > 0 ? (target + p.ntiles - 1) / p.ntiles : 1;
-  if (ns > nj / min_jper) ns = nj / min_jper;
+  const int blocks = target / kForceWaves;
+  int ns = p.ntiles > 0 ? (blocks + p.ntiles - 1) / p.ntiles : 1;
+  if (ns > nj / (kForceWaves * min_jper)) ns = nj / (kForceWaves * min_jper);
   if (ns < 1) ns = 1;
-  p.jper = ((nj + ns - 1) / ns + 15) / 16 * 16;
+  const int q = 16 * kForceWaves;
+  p.jper = ((nj + ns - 1) / ns + q - 1) / q * q;
   p.ns = (nj + p.jper - 1) / p.jper;
   p.stride = p.ntiles * 64;
   return p;
@@ -188,41 +194,48 @@ __global__ __launch_bounds__(256) void pg6h_select(int n, State S, Params P, Ctl
 
 // ---------------------------------------------------------------- force
 // Acc, jerk and pot of the active list against all n j-particles with the
-// two-float pair loop (pg6::pg6_range_h): persistent wavefronts walk the
-// (i-tile, j-range) items of pg6h_plan and write FP64 partials.
-__global__ __launch_bounds__(256) void pg6h_force(int n, float eps2, Params P, const Ctl* ctl,
-                                                  int target, int min_jper,
-                                                  const double* __restrict__ ipos,
-                                                  const double* __restrict__ ivel,
-                                                  const int* __restrict__ iid, pg6::JPred jp,
-                                                  pg6::Partials part) {
+// two-float pair loop (pg6::pg6_range_h).  Block = kForceWaves wavefronts;
+// blocks walk the (i-tile, j-range) items of pg6h_plan.  The wavefronts'
+// sums over their pieces of a range are added in LDS in wavefront order and
+// written as one FP64 partial per range.
+__global__ __launch_bounds__(64 * kForceWaves) void pg6h_force(
+    int n, float eps2, Params P, const Ctl* ctl, int target, int min_jper,
+    const double* __restrict__ ipos, const double* __restrict__ ivel, const int* __restrict__ iid,
+    pg6::JPred jp, pg6::Partials part) {
   if (pg6h_skip(ctl, P.dtmax)) return;
   const int nact = ctl->nact;
   const Plan pl = pg6h_plan(nact, n, target, min_jper);
   const int items = pl.ntiles * pl.ns;
   const int lane = threadIdx.x & 63;
-  const int wpb = blockDim.x / 64;
-  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / 64);
-  for (int it = w0; it < items; it += gridDim.x * wpb) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int piece = pl.jper / kForceWaves;
+  double* dst[7] = {part.ax, part.ay, part.az, part.pot, part.jx, part.jy, part.jz};
+  __shared__ double red[7][kForceWaves][64];
+  for (int it = blockIdx.x; it < items; it += gridDim.x) {
     const int tile = it % pl.ntiles, sp = it / pl.ntiles;
     const int i = tile * 64 + lane;
-    const bool ivalid = i < nact;
-    const int ic = ivalid ? i : nact - 1;  // padding lanes repeat the last slot
-    const int j0 = sp * pl.jper;
+    const int ic = i < nact ? i : nact - 1;  // padding lanes repeat the last slot
+    const int j0 = min(n, sp * pl.jper + w * piece);
     const pg6::RangeSums r = pg6::pg6_range_h<false>(
         ipos[3 * ic], ipos[3 * ic + 1], ipos[3 * ic + 2], (float)ivel[3 * ic],
-        (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], iid[ic], eps2, j0, min(n, j0 + pl.jper),
+        (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], iid[ic], eps2, j0, min(n, j0 + piece),
         jp);
-    if (ivalid) {
-      const size_t o = (size_t)sp * pl.stride + i;
-      part.ax[o] = r.ax;
-      part.ay[o] = r.ay;
-      part.az[o] = r.az;
-      part.pot[o] = r.pot;
-      part.jx[o] = r.jx;
-      part.jy[o] = r.jy;
-      part.jz[o] = r.jz;
+    red[0][w][lane] = r.ax;
+    red[1][w][lane] = r.ay;
+    red[2][w][lane] = r.az;
+    red[3][w][lane] = r.pot;
+    red[4][w][lane] = r.jx;
+    red[5][w][lane] = r.jy;
+    red[6][w][lane] = r.jz;
+    __syncthreads();
+    if (threadIdx.x < 7 * 64) {
+      const int c = threadIdx.x / 64;
+      double sum = 0.0;
+      for (int q = 0; q < kForceWaves; ++q) sum += red[c][q][lane];
+      const int k = tile * 64 + lane;
+      if (k < nact) dst[c][(size_t)sp * pl.stride + k] = sum;
     }
+    __syncthreads();
   }
 }
 

@@ -1215,7 +1215,8 @@ void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
   const int n = h.n;
   hipLaunchKernelGGL(pg6h::pg6h_select, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
                      h.ctl, b.js, b.jp, h.act, h.ipos, h.ivel, h.iid);
-  hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(256), 0, st, n, (float)h.eps2,
+  hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(64 * pg6h::kForceWaves), 0, st,
+                     n, (float)h.eps2,
                      h.P, h.ctl, h.target, h.min_jper, h.ipos, h.ivel, h.iid, b.jp, h.part);
   hipLaunchKernelGGL(pg6h::pg6h_finish, dim3(h.finish_blocks), dim3(256), 0, st, n, h.P, h.ctl,
                      h.target, h.min_jper, h.act, h.S, h.ipos, h.ivel, h.part, b.js);
@@ -1299,10 +1300,10 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   const int cus = device_cu_count();
   h.target = 16 * cus;  // wavefront items per force launch: 4 per SIMD
   h.min_blocks = std::min(ceil_div(n, 256), 4 * cus);
-  h.force_blocks = 4 * cus;
+  h.force_blocks = 4 * cus * 4 / pg6h::kForceWaves;  // 4 wavefronts per SIMD
   h.finish_blocks = std::max(1, std::min(ceil_div(n, 64), cus));
-  // largest force split: ns * ntiles <= target + ntiles (pg6h_plan)
-  const size_t nsplit_i = ((size_t)h.target + ceil_div(n, 64)) * 64;
+  // largest force split: ns * ntiles <= target / kForceWaves + ntiles (pg6h_plan)
+  const size_t nsplit_i = ((size_t)h.target / pg6h::kForceWaves + ceil_div(n, 64)) * 64;
   // state: x, v, a, j (3n each), pot, t, dt, m (n each)
   h.state.reserve(sizeof(double) * (size_t)n * 16);
   double* d = h.state.as<double>();
