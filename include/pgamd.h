@@ -168,7 +168,8 @@ int pg6_dev_autotune_reset(void);
  * v[n][3], m[n]; start-up forces and steps (eta_s); eta = corrector step
  * accuracy; dtmax = largest step and time window.  step: one block step;
  * *t = time after it, *nact = particles advanced.  run: block steps until
- * t >= t_end, returns their number.  state: host copies (each may be null),
+ * t >= t_end (finite), queued without a host round trip per step; returns
+ * their number.  state: host copies (each may be null),
  * t inside the current window; info[4] = {windows done, particle steps, block
  * steps, n}.  energy: 0.5 sum m v^2, 0.5 sum m pot. */
 int pg6_hermite_init(int n, const double *x, const double *v, const double *m, double eps2,
