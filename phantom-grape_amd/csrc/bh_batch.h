@@ -20,8 +20,9 @@
 
 namespace pgbh {
 
-constexpr int kThreads = 256;
-constexpr int kTile = 256;
+constexpr int kThreads = 1024;  // block
+constexpr int kPTile = kThreads;     // particles staged per LDS tile
+constexpr int kCTile = kThreads / 2; // cells staged per LDS tile
 
 __device__ inline int take_item(int* counter, int* s_item) {
   __syncthreads();
@@ -30,6 +31,11 @@ __device__ inline int take_item(int* counter, int* s_item) {
   return *s_item;
 }
 
+// Lanes are packed by phases: a group of n <= kThreads i-particles takes
+// p = kThreads / n copies of the i-set; lane t holds i-particle t % n and
+// walks the particles and cells of each LDS tile with jj = t / n, t / n + p,
+// ...; the p phase sums of each i are added in phase order at the end.
+// Groups larger than kThreads run in passes.
 __global__ __launch_bounds__(kThreads) void bh_batch(
     int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
     const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ poff,
@@ -37,8 +43,13 @@ __global__ __launch_bounds__(kThreads) void bh_batch(
     const int* __restrict__ ccnt, const int* __restrict__ clist, const float4* __restrict__ pos,
     const pg5::Cell* __restrict__ cells, float eps2, double* __restrict__ acc,
     double* __restrict__ pot, int* counter) {
-  __shared__ float4 s_p[kTile];
-  __shared__ pg5::Cell s_c[kTile];
+  // one LDS buffer holds either a particle tile or a cell tile
+  constexpr size_t kStage = sizeof(float4) * kPTile > sizeof(pg5::Cell) * kCTile
+                                ? sizeof(float4) * kPTile
+                                : sizeof(pg5::Cell) * kCTile;
+  __shared__ __align__(16) char s_stage[kStage];
+  float4* s_p = reinterpret_cast<float4*>(s_stage);
+  pg5::Cell* s_c = reinterpret_cast<pg5::Cell*>(s_stage);
   __shared__ double s_red[4][kThreads];
   __shared__ int s_item;
   const int tid = threadIdx.x;
@@ -51,35 +62,36 @@ __global__ __launch_bounds__(kThreads) void bh_batch(
     const int c0 = coff[grp], nc = ccnt[grp];
     for (int ib = 0; ib < ni; ib += kThreads) {
       const int nib = min(kThreads, ni - ib);
-      const int bi = nib <= 64 ? 64 : (nib <= 128 ? 128 : 256);
-      const int G = kThreads / bi;
-      const int il = tid % bi;
-      const int g = __builtin_amdgcn_readfirstlane(tid / bi);  // uniform per wave
-      const bool iv = il < nib;
-      const int ii = ilist[i0 + ib + (iv ? il : 0)];
+      const int nph = kThreads / nib;  // phases
+      const int il = tid % nib;
+      const int phase = tid / nib;
+      const bool live = phase < nph;
+      const int ii = ilist[i0 + ib + il];
       const float4 ip = pos[ii];
       double ax = 0.0, ay = 0.0, az = 0.0, pp = 0.0;
       // particles: the GRAPE-5 pair force (gravity_avx2.c:95-154)
-      for (int jt = 0; jt < np; jt += kTile) {
-        const int nt = min(kTile, np - jt);
+      for (int jt = 0; jt < np; jt += kPTile) {
+        const int nt = min(kPTile, np - jt);
         __syncthreads();
         if (tid < nt) s_p[tid] = pos[plist[p0 + jt + tid]];
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f, fp = 0.f;
+        if (live) {
 #pragma unroll 4
-        for (int jj = g; jj < nt; jj += G) {
-          const float4 q = s_p[jj];
-          const float dx = q.x - ip.x;
-          const float dy = q.y - ip.y;
-          const float dz = q.z - ip.z;
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-          const float rinv = __builtin_amdgcn_rsqf(r2);
-          const float mr = q.w * rinv;
-          const float mr3 = mr * rinv * rinv;
-          fp += mr;
-          fax = fmaf(mr3, dx, fax);
-          fay = fmaf(mr3, dy, fay);
-          faz = fmaf(mr3, dz, faz);
+          for (int jj = phase; jj < nt; jj += nph) {
+            const float4 q = s_p[jj];
+            const float dx = q.x - ip.x;
+            const float dy = q.y - ip.y;
+            const float dz = q.z - ip.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float mr = q.w * rinv;
+            const float mr3 = mr * rinv * rinv;
+            fp += mr;
+            fax = fmaf(mr3, dx, fax);
+            fay = fmaf(mr3, dy, fay);
+            faz = fmaf(mr3, dz, faz);
+          }
         }
         ax += (double)fax;
         ay += (double)fay;
@@ -87,59 +99,59 @@ __global__ __launch_bounds__(kThreads) void bh_batch(
         pp += (double)fp;
       }
       // cells: monopole + quadrupole (quad_avx2.c:35-149)
-      for (int ct = 0; ct < nc; ct += kTile) {
-        const int nt = min(kTile, nc - ct);
+      for (int ct = 0; ct < nc; ct += kCTile) {
+        const int nt = min(kCTile, nc - ct);
         __syncthreads();
         if (tid < nt) s_c[tid] = cells[clist[c0 + ct + tid]];
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f, fp = 0.f;
+        if (live) {
 #pragma unroll 2
-        for (int jj = g; jj < nt; jj += G) {
-          const pg5::Cell c = s_c[jj];
-          const float dx = c.xm.x - ip.x;
-          const float dy = c.xm.y - ip.y;
-          const float dz = c.xm.z - ip.z;
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-          const float rinv = __builtin_amdgcn_rsqf(r2);
-          const float rinv2 = rinv * rinv;
-          const float phip = c.xm.w * rinv;
-          const float qx = fmaf(c.qa.z, dz, fmaf(c.qa.y, dy, c.qa.x * dx));
-          const float qy = fmaf(c.qb.x, dz, fmaf(c.qa.w, dy, c.qa.y * dx));
-          const float qz = fmaf(c.qb.y, dz, fmaf(c.qb.x, dy, c.qa.z * dx));
-          const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
-          const float rinv5 = rinv2 * rinv2 * rinv;
-          const float phiq = 0.5f * dqd * rinv5;
-          fp += phip + phiq;
-          const float f = fmaf(5.f, phiq, phip) * rinv2;
-          fax = fmaf(f, dx, fmaf(-rinv5, qx, fax));
-          fay = fmaf(f, dy, fmaf(-rinv5, qy, fay));
-          faz = fmaf(f, dz, fmaf(-rinv5, qz, faz));
+          for (int jj = phase; jj < nt; jj += nph) {
+            const pg5::Cell c = s_c[jj];
+            const float dx = c.xm.x - ip.x;
+            const float dy = c.xm.y - ip.y;
+            const float dz = c.xm.z - ip.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float rinv2 = rinv * rinv;
+            const float phip = c.xm.w * rinv;
+            const float qx = fmaf(c.qa.z, dz, fmaf(c.qa.y, dy, c.qa.x * dx));
+            const float qy = fmaf(c.qb.x, dz, fmaf(c.qa.w, dy, c.qa.y * dx));
+            const float qz = fmaf(c.qb.y, dz, fmaf(c.qb.x, dy, c.qa.z * dx));
+            const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
+            const float rinv5 = rinv2 * rinv2 * rinv;
+            const float phiq = 0.5f * dqd * rinv5;
+            fp += phip + phiq;
+            const float f = fmaf(5.f, phiq, phip) * rinv2;
+            fax = fmaf(f, dx, fmaf(-rinv5, qx, fax));
+            fay = fmaf(f, dy, fmaf(-rinv5, qy, fay));
+            faz = fmaf(f, dz, fmaf(-rinv5, qz, faz));
+          }
         }
         ax += (double)fax;
         ay += (double)fay;
         az += (double)faz;
         pp += (double)fp;
       }
-      if (G > 1) {
-        __syncthreads();
-        s_red[0][tid] = ax;
-        s_red[1][tid] = ay;
-        s_red[2][tid] = az;
-        s_red[3][tid] = pp;
-        __syncthreads();
-        if (g == 0)
-          for (int q = 1; q < G; ++q) {
-            ax += s_red[0][q * bi + il];
-            ay += s_red[1][q * bi + il];
-            az += s_red[2][q * bi + il];
-            pp += s_red[3][q * bi + il];
-          }
-      }
-      if (g == 0 && iv) {
-        acc[3 * (size_t)ii + 0] = ax;
-        acc[3 * (size_t)ii + 1] = ay;
-        acc[3 * (size_t)ii + 2] = az;
-        pot[ii] = pp;
+      __syncthreads();
+      s_red[0][tid] = ax;
+      s_red[1][tid] = ay;
+      s_red[2][tid] = az;
+      s_red[3][tid] = pp;
+      __syncthreads();
+      if (tid < nib) {
+        double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
+        for (int q = 0; q < nph; ++q) {
+          sx += s_red[0][q * nib + tid];
+          sy += s_red[1][q * nib + tid];
+          sz += s_red[2][q * nib + tid];
+          sp += s_red[3][q * nib + tid];
+        }
+        acc[3 * (size_t)ii + 0] = sx;
+        acc[3 * (size_t)ii + 1] = sy;
+        acc[3 * (size_t)ii + 2] = sz;
+        pot[ii] = sp;
       }
     }
   }

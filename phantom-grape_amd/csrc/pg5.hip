@@ -669,7 +669,7 @@ void launch_bh(const BhArgs& a, int* counter, int bpc, hipStream_t st) {
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
-int bh_bpc() { return g_bh_bpc > 0 ? g_bh_bpc : 4; }
+int bh_bpc() { return g_bh_bpc > 0 ? g_bh_bpc : 2; }
 
 struct BhGraph {
   hipGraph_t graph = nullptr;

@@ -387,22 +387,20 @@ void launch_batch(int mode, int ngroups, const int* order, const int* ioff, cons
   pgb::TableCut tc{table_dev(st), g_tab.ntab, g_tab.frc_bit};
   if (mode == 1) {
     pgb::MlpCut mc = mlp_cut(st);
-    hipLaunchKernelGGL((pgb::treepm_batch<true>), dim3(blocks), dim3(pgb::kThreads), 0, st,
-                       ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, pos, tc, mc, ascale,
-                       acc, counter);
+    hipLaunchKernelGGL(pgb::treepm_batch_mlp, dim3(blocks), dim3(pgb::kThreads), 0, st, ngroups,
+                       order, ioff, icnt, ilist, joff, jcnt, jlist, pos, mc, ascale, acc, counter);
   } else {
-    pgb::MlpCut mc{};
     const size_t lds = sizeof(float2) * (size_t)g_tab.ntab;
-    hipLaunchKernelGGL((pgb::treepm_batch<false>), dim3(blocks), dim3(pgb::kThreads), lds, st,
-                       ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, pos, tc, mc, ascale,
-                       acc, counter);
+    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock>), dim3(blocks),
+                       dim3(pgb::kTableBlock), lds, st, ngroups, order, ioff, icnt, ilist, joff,
+                       jcnt, jlist, pos, tc, ascale, acc, counter);
   }
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
 int default_bpc(int mode) {
   if (g_batch_bpc[mode] > 0) return g_batch_bpc[mode];
-  return mode == 1 ? 4 : 8;
+  return mode == 1 ? 4 : 2;
 }
 
 // Drop-in force through the batch kernel (used in MLP mode): device i and j

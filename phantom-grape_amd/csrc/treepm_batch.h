@@ -9,12 +9,11 @@
 // and a j-list jlist[joff[g] .. +jcnt[g]) (groups may share list ranges), an
 // optional processing order (groups by decreasing cost), and a work counter
 // that must be 0 at launch.
-// Each block repeatedly takes the next group from the counter; a group's
-// i-particles (up to 256 at a time) are spread over the block's lanes and the
-// j-list is staged in LDS tiles of 256 (gathered through jlist).  The cut-off
-// shape comes from the evaluator: the LDS table (bit-identical per pair with
-// the drop-in kernel) or the learned MLP.  Results: acc[i][3] FP64 (written,
-// not accumulated) for every listed i.
+// Each block repeatedly takes the next group from the counter and stages the
+// group's j-list in LDS tiles (gathered through jlist).  The cut-off shape
+// comes from the evaluator: the LDS table (treepm_batch_table, bit-identical
+// per pair with the drop-in kernel) or the learned MLP (treepm_batch_mlp).
+// Results: acc[i][3] FP64 (written, not accumulated) for every listed i.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,8 +22,9 @@
 
 namespace pgb {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;    // treepm_batch_mlp block
 constexpr int kJTile = 256;
+constexpr int kTableBlock = 1024;  // treepm_batch_table block
 
 // Table evaluator: s = 2 + |dx_s|^2, f from the LDS table.
 struct TableCut {
@@ -49,26 +49,19 @@ __device__ inline int take_item(int* counter, int* s_item) {
   return *s_item;
 }
 
-template <bool MLP>
-__global__ __launch_bounds__(kThreads) void treepm_batch(
+// Learned cut-off: a group's i-particles (up to 256 at a time) over 64, 128
+// or 256 lanes, the block's G = 256 / that wave groups splitting each j-tile
+// (the network's skip test wants every lane of a wavefront on the same j).
+__global__ __launch_bounds__(kThreads) void treepm_batch_mlp(
     int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
     const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ joff,
     const int* __restrict__ jcnt, const int* __restrict__ jlist,
-    const float4* __restrict__ pos, TableCut tc, MlpCut mc,
-    double ascale, double* __restrict__ acc, int* counter) {
-  extern __shared__ float2 s_tab[];  // table entries (table evaluator only)
+    const float4* __restrict__ pos, MlpCut mc, double ascale, double* __restrict__ acc,
+    int* counter) {
   __shared__ float4 s_j[kJTile];
   __shared__ double s_red[3][kThreads];
   __shared__ int s_item;
   const int tid = threadIdx.x;
-  int shift = 0, base = 0;
-  float scut = 0.f;
-  if (!MLP) {
-    for (int k = tid; k < tc.ntab; k += kThreads) s_tab[k] = tc.gtab[k];
-    shift = 23 - tc.frc_bit;
-    base = 1 << (30 - shift);
-    scut = __uint_as_float((1u << 30) | ((unsigned)(tc.ntab - 1) << shift));
-  }
   for (;;) {
     const int it = take_item(counter, &s_item);
     if (it >= ngroups) break;
@@ -91,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void treepm_batch(
         if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f;
-        if (MLP) {
+        {
           // This wave's j are jj = g + G t.  Pass 1 marks (64 t at a time,
           // bit t of a wave-uniform mask) the j that some lane has inside
           // the cut-off; pass 2 evaluates the network only for those, four
@@ -144,19 +137,6 @@ __global__ __launch_bounds__(kThreads) void treepm_batch(
               }
             }
           }
-        } else {
-#pragma unroll 2
-          for (int jj = g; jj < nt; jj += G) {
-            const float4 q = s_j[jj];
-            const float dx = q.x - ip.x;
-            const float dy = q.y - ip.y;
-            const float dz = q.z - ip.z;
-            const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
-            const float f = pgt::table_f(s_tab, s, scut, shift, base);
-            fax = fmaf(f, q.w * dx, fax);
-            fay = fmaf(f, q.w * dy, fay);
-            faz = fmaf(f, q.w * dz, faz);
-          }
         }
         ax += (double)fax;
         ay += (double)fay;
@@ -179,6 +159,89 @@ __global__ __launch_bounds__(kThreads) void treepm_batch(
         acc[3 * (size_t)ii + 0] = ax * ascale;
         acc[3 * (size_t)ii + 1] = ay * ascale;
         acc[3 * (size_t)ii + 2] = az * ascale;
+      }
+    }
+  }
+}
+
+// Table cut-off with lanes packed by phases.  Block = BLOCK threads.  A
+// group of n <= BLOCK i-particles takes p = BLOCK / n copies of the i-set:
+// lane t holds i-particle t % n and walks the j of each LDS tile with
+// jj = t / n, t / n + p, ...; the p phase sums of each i are added in phase
+// order at the end.  Lanes in use: n p of BLOCK (at least half, usually far
+// more), where whole-wavefront i-blocks of 64, 128 or 256 lanes leave the
+// rest of a wavefront idle for every group that is not a multiple of 64.
+// Groups larger than BLOCK run in passes of BLOCK.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void treepm_batch_table(
+    int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
+    const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ joff,
+    const int* __restrict__ jcnt, const int* __restrict__ jlist,
+    const float4* __restrict__ pos, TableCut tc, double ascale, double* __restrict__ acc,
+    int* counter) {
+  extern __shared__ float2 s_tab[];
+  __shared__ float4 s_j[BLOCK];
+  __shared__ double s_red[3][BLOCK];
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < tc.ntab; k += BLOCK) s_tab[k] = tc.gtab[k];
+  const int shift = 23 - tc.frc_bit;
+  const int base = 1 << (30 - shift);
+  const float scut = __uint_as_float((1u << 30) | ((unsigned)(tc.ntab - 1) << shift));
+  for (;;) {
+    const int it = take_item(counter, &s_item);
+    if (it >= ngroups) break;
+    const int grp = order ? order[it] : it;
+    const int i0 = ioff[grp], ni = icnt[grp];
+    const int j0 = joff[grp], nj = jcnt[grp];
+    for (int ib = 0; ib < ni; ib += BLOCK) {
+      const int nib = min(BLOCK, ni - ib);
+      const int p = BLOCK / nib;  // phases
+      const int il = tid % nib;
+      const int phase = tid / nib;
+      const bool live = phase < p;
+      const int ii = ilist[i0 + ib + il];
+      const float4 ip = pos[ii];
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int jt = 0; jt < nj; jt += BLOCK) {
+        const int nt = min(BLOCK, nj - jt);
+        __syncthreads();
+        if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
+        __syncthreads();
+        float fax = 0.f, fay = 0.f, faz = 0.f;
+        if (live) {
+#pragma unroll 4
+          for (int jj = phase; jj < nt; jj += p) {
+            const float4 q = s_j[jj];
+            const float dx = q.x - ip.x;
+            const float dy = q.y - ip.y;
+            const float dz = q.z - ip.z;
+            const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
+            const float f = pgt::table_f(s_tab, s, scut, shift, base);
+            fax = fmaf(f, q.w * dx, fax);
+            fay = fmaf(f, q.w * dy, fay);
+            faz = fmaf(f, q.w * dz, faz);
+          }
+        }
+        ax += (double)fax;
+        ay += (double)fay;
+        az += (double)faz;
+      }
+      __syncthreads();
+      s_red[0][tid] = ax;
+      s_red[1][tid] = ay;
+      s_red[2][tid] = az;
+      __syncthreads();
+      if (tid < nib) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int q = 0; q < p; ++q) {
+          sx += s_red[0][q * nib + tid];
+          sy += s_red[1][q * nib + tid];
+          sz += s_red[2][q * nib + tid];
+        }
+        acc[3 * (size_t)ii + 0] = sx * ascale;
+        acc[3 * (size_t)ii + 1] = sy * ascale;
+        acc[3 * (size_t)ii + 2] = sz * ascale;
       }
     }
   }
