@@ -414,7 +414,7 @@ def bench_c1(args, stream, cpu):
     for variant, name in ((0, "lds_tiles"), (1, "scalar_broadcast")):
         g5.lib.pg5_dev_set_kernel(variant)
         kernels[name] = round(_time(run, stream, 20), 4)
-    g5.lib.pg5_dev_set_kernel(1)
+    g5.lib.pg5_dev_set_kernel(0)
     ms = _time(run, stream, 20)
     inter = float(n) * n
     out = {"workload": "pg5_plummer_n16384_eps1/64", "interactions": inter,

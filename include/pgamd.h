@@ -39,8 +39,8 @@ void *pg5_dev_stream(int devid);
 /* bi in {0 = automatic, 64, 128, 256} i-particles per block; blocks_per_cu and
  * min_jper (> 0) set the j-range split. */
 int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
-/* Particle force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through
- * scalar registers (default).  Same pair arithmetic; speed only. */
+/* Particle force kernel: 0 = j-tiles staged in LDS (default), 1 = j
+ * broadcast through scalar registers.  Same pair arithmetic; speed only. */
 int pg5_dev_set_kernel(int variant);
 /* Batched tree-list force (the work of the per-group g5_calculate_force_on_xMC
  * + g5c_calculate_force_on_xMC pair of PG5/newton_quad/source.cpp:845-867 for

@@ -162,7 +162,7 @@ struct Plan {
 
 // Particle kernel: 0 = j-tiles in LDS (pg5_force), 1 = j broadcast through
 // scalar registers (pg5_force_s).  Cells always use the LDS kernel.
-int g_variant = 1;
+int g_variant = 0;
 
 Plan plan(int ni, int nj, bool cells = false) {
   Plan p;
@@ -800,8 +800,9 @@ PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int
   return 0;
 }
 
-// Particle force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through
-// scalar registers (default).  Same pair arithmetic; speed only.
+// Particle force kernel: 0 = j-tiles staged in LDS (default, measured the
+// faster on MI355X), 1 = j broadcast through scalar registers.  Same pair
+// arithmetic; speed only.
 PGAMD_API int pg5_dev_set_kernel(int variant) {
   if (variant != 0 && variant != 1) {
     api_error(3, "pg5_dev_set_kernel: variant %d not in {0, 1}", variant);

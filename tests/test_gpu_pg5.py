@@ -48,7 +48,7 @@ def test_reference_fixture(g5, name, variant):
     oa, op = O.pg5_force(f32(x), np.float32(eps * eps), f32(x), f32(m))
     assert np.percentile(rel_err(a, oa), 99) < 1e-5
     assert rel_err(p, op).max() < 1e-6
-    g5.lib.pg5_dev_set_kernel(1)
+    g5.lib.pg5_dev_set_kernel(0)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -62,7 +62,7 @@ def test_symmetric_softening_both_kernels(g5s, variant):
     oa, op = O.pg5_force(f32(x), f32(e2), f32(x), f32(m), eps2j=f32(e2))
     assert np.percentile(rel_err(a, oa), 99) < 1e-5
     assert rel_err(p, op).max() < 1e-6
-    g5s.lib.pg5_dev_set_kernel(1)
+    g5s.lib.pg5_dev_set_kernel(0)
 
 
 def test_plummer_16k_vs_fp64(g5):
