@@ -719,7 +719,35 @@ def bench_c2i(args, stream, cpu):
             "particle_steps": csteps,
             "energy_error": float(f"{abs(en[-1].sum() - ek0 - ep0) / abs(ek0 + ep0):.3e}"),
             "gpu_speedup": round(t / wall, 1)}
+    if cpu:
+        res["dropin_banana"] = _banana_dropin(m, x, v, t_end)
     return res
+
+
+def _banana_dropin(m, x, v, t_end):
+    """The reference's own Hermite program (PG6/banana, built from its sources
+    by oracle/ref.mk) on the same initial conditions, linked against the
+    reference library and against libpg6.so: its own reported integration
+    time (tcalc).  libpg6 runs once with the reference's 48 i per
+    g6calc_lasthalf call and once with g6_npipes() raised to 128, the size of
+    banana's own i buffers (PG6/banana/hermite.c:14,218-219)."""
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import dropin
+    if not all(os.path.exists(dropin.program(p)) for p in ("banana_ref", "banana_pgamd")):
+        return None
+    out = {"setup": "pl016k-style Plummer N = 16384, eps = 0, eta = 0.2, eta_s = 0.01, "
+                    "dtmax = 1/8, t = 0 -> 1/8", "threads": _threads()}
+    for key, prog, env in (("reference_library_s", "banana_ref", None),
+                           ("libpg6_s", "banana_pgamd", None),
+                           ("libpg6_npipes128_s", "banana_pgamd", {"PGAMD_G6_NPIPES": "128"})):
+        with tempfile.TemporaryDirectory() as d:
+            dropin.write_banana_input(d, m, x, v, t_end)
+            t = dropin.banana_tcalc(dropin.run(prog, [], d, timeout=600, threads=_threads(),
+                                               env_extra=env))
+            out[key] = None if t is None else round(t, 3)
+    return out
 
 
 EXTRA = {"c1": bench_c1, "c2i": bench_c2i, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}

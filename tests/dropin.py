@@ -36,8 +36,8 @@ def write_banana_input(dirname, m, x, v, tend, epsinv=0.0, eta=0.2, eta_s=0.01, 
         f.write(f"0 {tend} 100.0\n{epsinv}\n{eta} {eta_s} {dtinv}\npl.init out\n")
 
 
-def run(name, args, cwd, timeout=300, threads=4):
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+def run(name, args, cwd, timeout=300, threads=4, env_extra=None):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), **(env_extra or {}))
     r = subprocess.run([program(name), *args], cwd=cwd, capture_output=True, text=True,
                        timeout=timeout, env=env)
     assert r.returncode == 0, (name, r.returncode, r.stderr[-2000:])
@@ -58,3 +58,9 @@ def banana_energy_errors(out):
 
 def banana_gflops(out):
     return np.array([float(a) for a in re.findall(r"tcalc: \S+ (\S+) Gflops", out)])
+
+
+def banana_tcalc(out):
+    """The integration time banana01.c reports at its last output (io.c:82)."""
+    t = re.findall(r"tcalc: (\S+)", out)
+    return float(t[-1]) if t else None
