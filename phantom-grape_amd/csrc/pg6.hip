@@ -17,6 +17,7 @@
 #include "pg6_kernels.h"
 #include "pg6_nblist.h"
 #include "cluster_batch.h"
+#include "isa_probe.h"
 #include "hermite_kernels.h"
 
 #include <algorithm>
@@ -1158,6 +1159,47 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
 PGAMD_API int pg6_dev_autotune_reset(void) {
   std::lock_guard<std::mutex> lk(g_mutex);
   for (int k = 0; k < kBuckets; ++k) g_board.tuned_ok[k] = false;
+  return 0;
+}
+
+// Issue rates of the pair kernels' building blocks on this device
+// (isa_probe.h): ns_out[k] = nanoseconds per wavefront instruction per SIMD
+// for k = 0: v_fma_f32, 1: v_rsq_f32 + v_add_f32 (per pair of
+// instructions), 2: v_rsq_f32; eight wavefronts per SIMD, `iters` rounds of
+// eight independent chains.  Synchronous.  A measurement aid for the force
+// kernels' roofline; no counterpart in the reference.
+PGAMD_API int pg6_dev_isa_rates(int iters, double* ns_out) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (iters <= 0 || !ns_out) {
+    api_error(3, "pg6_dev_isa_rates: iters=%d out=%p", iters, (void*)ns_out);
+    return -1;
+  }
+  ensure_device();
+  const int simds = 4 * device_cu_count();
+  const int blocks = 2 * simds;  // 256 threads = 4 wavefronts: 8 per SIMD
+  DevBuf out;
+  out.reserve(sizeof(float) * 256 * (size_t)blocks);
+  hipStream_t st = nullptr;
+  PGAMD_HIP_CHECK(hipStreamCreate(&st));
+  hipEvent_t e0, e1;
+  PGAMD_HIP_CHECK(hipEventCreate(&e0));
+  PGAMD_HIP_CHECK(hipEventCreate(&e1));
+  void (*k[3])(int, float, float*) = {pgprobe::chains<0>, pgprobe::chains<1>, pgprobe::chains<2>};
+  const double wave_ops = (double)blocks * 4 / simds * iters * pgprobe::kChains;
+  for (int m = 0; m < 3; ++m) {
+    hipLaunchKernelGGL(k[m], dim3(blocks), dim3(256), 0, st, iters, 1.f, out.as<float>());
+    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k[m], dim3(blocks), dim3(256), 0, st, iters, 1.f, out.as<float>());
+    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PGAMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    ns_out[m] = 1e6 * ms / wave_ops;
+  }
+  PGAMD_HIP_CHECK(hipEventDestroy(e0));
+  PGAMD_HIP_CHECK(hipEventDestroy(e1));
+  PGAMD_HIP_CHECK(hipStreamDestroy(st));
+  out.release();
   return 0;
 }
 

@@ -263,3 +263,12 @@ def test_empty_and_error_paths(g6):
         assert rc == 1 and code != 0 and "ipipe" in msg
     finally:
         g6.lib.pgamd_set_abort_on_error(1)
+
+
+def test_isa_rate_probe(g6):
+    """The issue-rate probe runs and orders its operations as the hardware
+    does: a transcendental costs more than an FMA, rsq + add more than rsq."""
+    r = g6.isa_rates(4096)
+    assert all(v > 0 for v in r.values())
+    assert r["v_rsq_f32"] > r["v_fma_f32"]
+    assert r["v_rsq_f32+v_add_f32"] > r["v_rsq_f32"]
