@@ -128,6 +128,33 @@ def test_run_in_graphs_equals_single_steps(g6):
     assert g6.hermite_run(t_end + 1.0 / 64) > 0
 
 
+@pytest.mark.parametrize("n", [2, 3, 100])
+def test_small_systems(g6, n):
+    """Particle counts below one wavefront and not a multiple of 64: the
+    graph-driven run and single steps agree bit for bit, the two-body orbit
+    keeps its energy."""
+    m, x, v = ics.plummer(n, 40 + n)
+    eps2 = (1 / 64) ** 2
+    t_end = 1.0 / 16
+    g6.hermite_init(x, v, m, eps2, ETA_S, ETA, DTMAX)
+    ek0, ep0 = g6.hermite_energy()
+    blocks = g6.hermite_run(t_end)
+    a = g6.hermite_state()
+    g6.hermite_init(x, v, m, eps2, ETA_S, ETA, DTMAX)
+    t, k = 0.0, 0
+    while t < t_end:
+        t, nact = g6.hermite_step()
+        assert 1 <= nact <= n
+        k += 1
+    b = g6.hermite_state()
+    assert k == blocks > 0
+    for key in ("x", "v", "a", "j", "pot", "t", "dt", "t_mod", "steps", "blocks"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    if t_end % DTMAX == 0 or np.all(b["t"] == b["t"][0]):
+        ek, ep = g6.hermite_energy()
+        assert abs(ek + ep - ek0 - ep0) < 1e-3 * abs(ek0 + ep0)
+
+
 def test_errors_are_reported(g6):
     import pgamd
     g6.lib.pgamd_set_abort_on_error(0)
