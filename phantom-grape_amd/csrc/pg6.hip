@@ -90,7 +90,7 @@ struct Board {
   int nj_pred = -1;  // j count of the last prediction
   // i-set and outputs of the host API
   HostBuf h_in, h_out;
-  DevBuf d_in, d_out, d_part;
+  DevBuf d_in, d_part;
   // neighbour lists of the last lasthalf*n call
   DevBuf d_nb;
   std::vector<int> nbl, nblen;
@@ -222,7 +222,6 @@ void board_close(Board& b) {
   b.jp_buf.release();
   b.pend_dev.release();
   b.d_in.release();
-  b.d_out.release();
   b.d_part.release();
   b.d_nb.release();
   b.pend.release();
@@ -421,11 +420,14 @@ int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double 
   const int* did = reinterpret_cast<const int*>(dvel + 3 * (size_t)ni);
   const float* dh2 = reinterpret_cast<const float*>(did + ni);
 
+  // The kernels write the results straight into pinned host memory (no copy
+  // back to queue behind them); the synchronisation below makes them visible.
   const size_t out_bytes = (size_t)ni * (7 * sizeof(double) + sizeof(int) + sizeof(float));
-  b.d_out.reserve(out_bytes);
   b.h_out.reserve(out_bytes);
+  void* dout = nullptr;
+  PGAMD_HIP_CHECK(hipHostGetDevicePointer(&dout, b.h_out.ptr, 0));
   Outputs o;
-  o.acc = b.d_out.as<double>();
+  o.acc = static_cast<double*>(dout);
   o.jrk = o.acc + 3 * (size_t)ni;
   o.pot = o.jrk + 3 * (size_t)ni;
   o.nnb = want_nn ? reinterpret_cast<int*>(o.pot + ni) : nullptr;
@@ -446,8 +448,6 @@ int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double 
     PGAMD_HIP_CHECK(hipMemcpyAsync(b.nbl.data(), dnbl, sizeof(int) * (size_t)ni * kMaxLen,
                                    hipMemcpyDeviceToHost, b.stream));
   }
-  PGAMD_HIP_CHECK(hipMemcpyAsync(b.h_out.ptr, b.d_out.ptr, out_bytes, hipMemcpyDeviceToHost,
-                                 b.stream));
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   const double* ha = b.h_out.as<double>();
   const double* hj = ha + 3 * (size_t)ni;
