@@ -16,6 +16,7 @@
 #include "treepm_batch.h"
 #include "pm_kernels.h"
 
+#include <hipcub/device/device_radix_sort.hpp>
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -1155,6 +1156,7 @@ struct PmCache {
   int M = 0;
   hipfftHandle fwd{}, inv{};
   DevBuf rho, rhok, fk, f;
+  DevBuf keys, order, sort_tmp;  // particle sort by mesh cell (pm_keys)
 };
 PmCache g_pm;
 std::mutex g_pm_mutex;
@@ -1205,10 +1207,27 @@ PGAMD_API int pg5t_pm_force(int n, const double* pos, const double* mass, int me
   double2* fk = c.fk.as<double2>();
   double* f = c.f.as<double>();
   PGAMD_HIP_CHECK(hipMemsetAsync(rho, 0, sizeof(double) * nr, st));
-  if (n > 0)
-    hipLaunchKernelGGL(pm::pm_assign, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mass,
-                       mesh, rho);
-  PGAMD_HIP_CHECK(hipGetLastError());
+  // particles sorted by the Morton code of their nearest mesh point
+  const int* order = nullptr;
+  if (n > 0) {
+    c.keys.reserve(2 * sizeof(unsigned) * (size_t)n);
+    c.order.reserve(2 * sizeof(int) * (size_t)n);
+    unsigned* kin = c.keys.as<unsigned>();
+    int* vin = c.order.as<int>();
+    hipLaunchKernelGGL(pm::pm_keys, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mesh, kin,
+                       vin);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    size_t tmp = 0;
+    PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kin + n, vin, vin + n,
+                                                       n, 0, 30, st));
+    c.sort_tmp.reserve(tmp);
+    PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.ptr, tmp, kin, kin + n, vin,
+                                                       vin + n, n, 0, 30, st));
+    order = vin + n;
+    hipLaunchKernelGGL(pm::pm_assign_sorted, dim3(ceil_div(n, pm::kAssignRun)),
+                       dim3(pm::kAssignThreads), 0, st, n, order, pos, mass, mesh, rho);
+    PGAMD_HIP_CHECK(hipGetLastError());
+  }
   PGAMD_FFT_CHECK(hipfftExecD2Z(c.fwd, rho, reinterpret_cast<hipfftDoubleComplex*>(rhok)));
   hipLaunchKernelGGL(pm::pm_green, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, st, mesh,
                      eps_pm, rhok, fk);
@@ -1217,8 +1236,8 @@ PGAMD_API int pg5t_pm_force(int n, const double* pos, const double* mass, int me
     PGAMD_FFT_CHECK(hipfftExecZ2D(c.inv, reinterpret_cast<hipfftDoubleComplex*>(fk + d * nk),
                                   f + d * nr));
   if (n > 0)
-    hipLaunchKernelGGL(pm::pm_interp, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mesh, f,
-                       f + nr, f + 2 * nr, acc);
+    hipLaunchKernelGGL(pm::pm_interp, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, order, pos,
+                       mesh, f, f + nr, f + 2 * nr, acc);
   PGAMD_HIP_CHECK(hipGetLastError());
   return 0;
 }
