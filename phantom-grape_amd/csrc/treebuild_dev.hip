@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
+#include <unordered_map>
 #include <vector>
 
 using namespace pgamd;
@@ -34,20 +36,51 @@ using u64 = unsigned long long;
 constexpr int kLevels = 21;  // bits per axis of the Morton key
 
 // ------------------------------------------------------------ allocations
+// Device memory of the builds comes from a cache: blocks rounded up to powers
+// of two and kept for reuse once released, so that repeated builds make no
+// hipMalloc / hipFree calls (hipFree also waits for the whole device).  A
+// build releases its temporaries after its stream has drained, lists release
+// theirs in pgt_dev_free after a device synchronisation; both run under the
+// library's mutex.  The cache holds its peak size until the process ends.
+struct Cache {
+  std::map<size_t, std::vector<void*>> free;
+  std::unordered_map<void*, size_t> size;
+  void* get(size_t bytes) {
+    size_t b = 256;
+    while (b < bytes) b <<= 1;
+    std::vector<void*>& v = free[b];
+    if (!v.empty()) {
+      void* p = v.back();
+      v.pop_back();
+      return p;
+    }
+    void* p = nullptr;
+    PGAMD_HIP_CHECK(hipMalloc(&p, b));
+    size[p] = b;
+    return p;
+  }
+  void put(void* p) { free[size.at(p)].push_back(p); }
+};
+Cache g_cache;
+
 struct Mem {
   std::vector<void*> ptrs;
   template <class T>
   T* get(size_t n) {
-    void* p = nullptr;
-    PGAMD_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    void* p = g_cache.get(std::max<size_t>(n, 1) * sizeof(T));
     ptrs.push_back(p);
     return static_cast<T*>(p);
   }
   void free_all() {
-    for (void* p : ptrs) PGAMD_HIP_CHECK(hipFree(p));
+    for (void* p : ptrs) g_cache.put(p);
     ptrs.clear();
   }
 };
+
+// out[n] = out[n - 1] + in[n - 1] (0 for n = 0): the total of an exclusive scan.
+__global__ void k_scan_total(const int* __restrict__ in, int* __restrict__ out, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = n > 0 ? out[n - 1] + in[n - 1] : 0;
+}
 
 // Result of one build: device arrays in the host builder's layout.
 struct Lists {
@@ -71,14 +104,14 @@ struct Build {
   explicit Build(hipStream_t s) : st(s) {}
   ~Build() {
     PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-    if (cub) PGAMD_HIP_CHECK(hipFree(cub));
+    if (cub) g_cache.put(cub);
     tmp.free_all();
   }
   void* cub_tmp(size_t bytes) {
     if (bytes > cub_bytes) {
-      PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-      if (cub) PGAMD_HIP_CHECK(hipFree(cub));
-      PGAMD_HIP_CHECK(hipMalloc(&cub, bytes));
+      PGAMD_HIP_CHECK(hipStreamSynchronize(st));  // the old block may still be in use
+      if (cub) g_cache.put(cub);
+      cub = g_cache.get(bytes);
       cub_bytes = bytes;
     }
     return cub;
@@ -95,18 +128,9 @@ struct Build {
     size_t b = 0;
     PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
     PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub_tmp(b), b, in, out, n, st));
-    int last[2] = {0, 0};
-    if (n > 0) {
-      PGAMD_HIP_CHECK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(int), hipMemcpyDeviceToHost,
-                                     st));
-      PGAMD_HIP_CHECK(hipMemcpyAsync(&last[1], in + n - 1, sizeof(int), hipMemcpyDeviceToHost,
-                                     st));
-    }
-    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-    const int tot = last[0] + last[1];
-    PGAMD_HIP_CHECK(hipMemcpyAsync(out + n, &tot, sizeof(int), hipMemcpyHostToDevice, st));
-    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-    return tot;
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, in, out, n);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    return read(out + n);
   }
   // Stable radix sort of (key, value) pairs; int keys must be >= 0.
   void sort_pairs(const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit) {
@@ -1064,6 +1088,7 @@ PGAMD_API int pgt_dev_pack(void* h, const double* mass, double scale, void* pos4
 
 PGAMD_API void pgt_dev_free(void* h) {
   if (!h) return;
+  std::lock_guard<std::mutex> lk(g_mutex);
   Lists* r = static_cast<Lists*>(h);
   PGAMD_HIP_CHECK(hipDeviceSynchronize());
   r->mem.free_all();
