@@ -578,17 +578,79 @@ __device__ inline double bh_box_d2(const double* __restrict__ gc, double li,
   return d2;
 }
 
-// One thread per group: the depth-first walk of the host builder (children
-// visited in octant order 7..0 onto the stack), counting (FILL == false) or
-// writing the particle and cell lists.
+// Node data the Barnes-Hut walk reads for one child, in one 64-byte record:
+// centre of mass, particle range, cell index (nodes of more than one
+// particle) and the children as the first child's index plus an octant mask
+// (the children of a node are numbered consecutively in octant order).
+struct WalkNode {
+  int lo, hi, cell, first, mask, pad[3];
+  double com[3], pad2;
+};
+static_assert(sizeof(WalkNode) == 64, "one record per 64 bytes");
+
+__global__ void k_walk_nodes(int nn, Nodes nd, WalkNode* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  WalkNode w;
+  for (int d = 0; d < 3; ++d) w.com[d] = nd.com[3 * k + d];
+  w.lo = nd.lo[k];
+  w.hi = nd.hi[k];
+  w.cell = w.hi - w.lo > 1 ? nd.cell[k] : -1;
+  w.first = -1;
+  w.mask = 0;
+  for (int o = 7; o >= 0; --o) {
+    const int ch = nd.child[8 * k + o];
+    if (ch >= 0) {
+      w.first = ch;
+      w.mask |= 1 << o;
+    }
+  }
+  w.pad[0] = w.pad[1] = w.pad[2] = 0;
+  w.pad2 = 0.0;
+  out[k] = w;
+}
+
+// Depth of the walk stack: the depth-first walk keeps at most 7 pending
+// siblings per level below the root, plus the 8 children of the node being
+// expanded (internal nodes sit above level kLevels).
+constexpr int kWalkDepth = 7 * (kLevels - 1) + 8;
+constexpr int kWalkWaves = 4;  // groups (one wavefront each) per block
+
+// Block b of a launch of nb blocks runs on XCD b % 8; renumber the blocks so
+// that each XCD takes one contiguous range of groups, whose walks visit
+// nearby nodes through that XCD's L2.
+__device__ inline int xcd_block(int b, int nb) {
+  const int x = b % 8, i = b / 8, q = nb / 8, r = nb % 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// One wavefront per group: the depth-first walk of the host builder
+// (source.cpp:832-867 order: the children of an opened node are examined in
+// octant order 7..0, emitting singletons and accepted cells in that order and
+// pushing the rest, so octant 0 is expanded first), counting (FILL == false)
+// or writing the particle and cell lists.  Each step examines the children
+// of the top 8 stack entries at once, one lane per (entry, octant), and keeps
+// the work of the entries the serial walk would finish before the first one
+// that pushes children: a run of entries whose children are all listed,
+// ending at one that opens some, so the lists and the stack come out exactly
+// as in the serial walk.  Ballots rank the emissions and pushes in the serial
+// order.  A pushed entry holds what its expansion needs (the node's children
+// and level, or the particle range of a leaf), so a step costs one round of
+// global loads: the children's records.
 template <bool FILL>
-__global__ void k_bh_walk(int ng, const int* __restrict__ gnode, Nodes nd, double L, double th2,
-                          int nleaf, int* __restrict__ pcnt, int* __restrict__ ccnt,
-                          const int* __restrict__ poff, const int* __restrict__ coff,
-                          int* __restrict__ plist, int* __restrict__ clist) {
+__global__ __launch_bounds__(64 * kWalkWaves) void k_bh_walk(
+    int ng, const int* __restrict__ gnode, Nodes nd, const WalkNode* __restrict__ wn, double L,
+    double th2, int nleaf, int* __restrict__ pcnt, int* __restrict__ ccnt,
+    const int* __restrict__ poff, const int* __restrict__ coff, int* __restrict__ plist,
+    int* __restrict__ clist) {
 #pragma clang fp contract(off)
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+  // entry: {first child, octant mask, level} of a node to expand, or
+  // {lo, hi, -1} of a leaf whose particles are listed
+  __shared__ int3 s_stack[kWalkWaves][kWalkDepth];
+  const int lane = threadIdx.x & 63;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * kWalkWaves + (threadIdx.x >> 6);
+  if (g >= ng) return;  // uniform per wavefront
+  int3* stk = s_stack[threadIdx.x >> 6];
   const int gk = gnode[g];
   const double gc[3] = {nd.c[3 * gk], nd.c[3 * gk + 1], nd.c[3 * gk + 2]};
   const double li = 0.5 * ldexp(L, -nd.lev[gk]);
@@ -596,41 +658,80 @@ __global__ void k_bh_walk(int ng, const int* __restrict__ gnode, Nodes nd, doubl
   int* pl = FILL ? plist + poff[g] : nullptr;
   int* cl = FILL ? clist + coff[g] : nullptr;
   // the whole tree as one cell (source.cpp:832-840)
-  if (bh_box_d2(gc, li, nd.com) * th2 >= L * L) {
-    if (FILL) cl[0] = nd.cell[0];
+  if (bh_box_d2(gc, li, wn[0].com) * th2 >= L * L) {
+    if (FILL && lane == 0) cl[0] = wn[0].cell;
     nc = 1;
   } else {
-    int stack[8 * kLevels + 8];
-    int sp = 0;
-    stack[sp++] = 0;
+    if (lane == 0)
+      stk[0] = wn[0].hi - wn[0].lo <= nleaf ? make_int3(wn[0].lo, wn[0].hi, -1)
+                                            : make_int3(wn[0].first, wn[0].mask, 0);
+    int sp = 1;
     while (sp > 0) {
-      const int k = stack[--sp];
-      const int lo = nd.lo[k], hi = nd.hi[k];
-      if (hi - lo <= nleaf) {
-        for (int p = lo; p < hi; ++p) {
-          if (FILL) pl[np] = p;
-          ++np;
-        }
-        continue;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // lanes 8e..8e+7 take entry e from the top (e = 0 is the top), octant
+      // o = lane % 8 of it
+      const int npop = sp < 8 ? sp : 8;
+      const int e = lane >> 3, o = lane & 7;
+      int3 ent = make_int3(0, 0, 0);
+      if (e < npop) ent = stk[sp - 1 - e];
+      const bool leaf = e < npop && ent.z < 0;
+      int kind = 0;  // 1 particle, 2 cell, 3 push
+      int lo = 0, hi = 0, cell = 0, first = 0, mask = 0;
+      if (e < npop && !leaf && ((ent.y >> o) & 1)) {
+        const double l = ldexp(L, -ent.z);
+        const double s2 = 0.25 * l * l;  // child side squared
+        const WalkNode* w = wn + ent.x + __popc(ent.y & ((1 << o) - 1));
+        const int4 r1 = *reinterpret_cast<const int4*>(w);
+        lo = r1.x;
+        hi = r1.y;
+        cell = r1.z;
+        first = r1.w;
+        mask = w->mask;
+        if (hi - lo == 1)
+          kind = 1;
+        else if (bh_box_d2(gc, li, w->com) * th2 > s2)
+          kind = 2;
+        else
+          kind = 3;
       }
-      const double l = ldexp(L, -nd.lev[k]);
-      const double s2 = 0.25 * l * l;  // child side squared
-      for (int o = 7; o >= 0; --o) {
-        const int ck = nd.child[8 * k + o];
-        if (ck < 0) continue;
-        if (nd.hi[ck] - nd.lo[ck] == 1) {
-          if (FILL) pl[np] = nd.lo[ck];
-          ++np;
-        } else if (bh_box_d2(gc, li, nd.com + 3 * ck) * th2 > s2) {
-          if (FILL) cl[nc] = nd.cell[ck];
-          ++nc;
-        } else {
-          stack[sp++] = ck;
-        }
+      // The serial walk would expand the entries from the top until one
+      // pushes children; those entries (ncommit of them) are done here, the
+      // rest stay on the stack.
+      const u64 ms = __ballot(kind == 3);
+      const int ncommit = ms ? (__builtin_ctzll(ms) >> 3) + 1 : npop;
+      const bool commit = e < ncommit;
+      const u64 mp = __ballot(commit && kind == 1), mc = __ballot(commit && kind == 2);
+      const u64 mine = 0xffULL << (8 * e);           // this entry's lanes
+      const u64 before = (1ULL << (8 * e)) - 1ULL;   // entries above this one
+      const u64 above = ~((2ULL << lane) - 1ULL);    // lanes > this one
+      // emission order: entries from the top; octants 7..0 inside an entry;
+      // a leaf entry lists its particles lo..hi-1
+      const int leafcnt = commit && leaf && o == 0 ? ent.y - ent.x : 0;
+      int leafpre = 0, leaftot = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = __builtin_amdgcn_readlane(leafcnt, 8 * q);
+        leafpre += q < e ? c : 0;
+        leaftot += c;
       }
+      const int pbase = np + leafpre + __popcll(mp & before);
+      if (FILL && commit && kind == 1) pl[pbase + __popcll(mp & mine & above)] = lo;
+      if (FILL && commit && leaf)
+        for (int p = ent.x + o; p < ent.y; p += 8) pl[pbase + (p - ent.x)] = p;
+      if (FILL && commit && kind == 2) cl[nc + __popcll(mc & (before | (mine & above)))] = cell;
+      const u64 mpush = ms & (0xffULL << (8 * (ncommit - 1)));  // the last committed entry
+      sp -= ncommit;
+      if (kind == 3 && commit)
+        stk[sp + __popcll(mpush & above)] =
+            hi - lo <= nleaf ? make_int3(lo, hi, -1) : make_int3(first, mask, ent.z + 1);
+      np += __popcll(mp) + leaftot;
+      nc += __popcll(mc);
+      sp += __popcll(mpush);
     }
   }
-  if (!FILL) {
+  if (!FILL && lane == 0) {
     pcnt[g] = np;
     ccnt[g] = nc;
   }
@@ -942,15 +1043,19 @@ Lists* bh_build(int n, const double* x, const double* m, double theta, int ncrit
   R->joff = R->mem.get<int>(ng + 1);
   R->ccnt = R->mem.get<int>(ng);
   R->coff = R->mem.get<int>(ng + 1);
-  hipLaunchKernelGGL((k_bh_walk<false>), grid_for(ng, 64), dim3(64), 0, st, ng, gnode, T.nd, L,
-                     th2, nleaf, R->jcnt, R->ccnt, nullptr, nullptr, nullptr, nullptr);
+  WalkNode* wn = B.tmp.get<WalkNode>(T.nn);
+  hipLaunchKernelGGL(k_walk_nodes, grid_for(T.nn), dim3(256), 0, st, T.nn, T.nd, wn);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const dim3 wgrid((ng + kWalkWaves - 1) / kWalkWaves), wblock(64 * kWalkWaves);
+  hipLaunchKernelGGL((k_bh_walk<false>), wgrid, wblock, 0, st, ng, gnode, T.nd, wn, L, th2, nleaf,
+                     R->jcnt, R->ccnt, nullptr, nullptr, nullptr, nullptr);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->nj = B.scan(R->jcnt, R->joff, ng);
   R->nc = B.scan(R->ccnt, R->coff, ng);
   R->jlist = R->mem.get<int>(R->nj);
   R->clist = R->mem.get<int>(R->nc);
-  hipLaunchKernelGGL((k_bh_walk<true>), grid_for(ng, 64), dim3(64), 0, st, ng, gnode, T.nd, L,
-                     th2, nleaf, nullptr, nullptr, R->joff, R->coff, R->jlist, R->clist);
+  hipLaunchKernelGGL((k_bh_walk<true>), wgrid, wblock, 0, st, ng, gnode, T.nd, wn, L, th2, nleaf,
+                     nullptr, nullptr, R->joff, R->coff, R->jlist, R->clist);
   PGAMD_HIP_CHECK(hipGetLastError());
   finish(B, *R, true);
   return R;
