@@ -7,7 +7,9 @@
 //  * pgt_treepm_build: periodic unit box; ghost copies of the particles within
 //    r_cut of a face (so the GPU kernel needs no minimum-image logic);
 //    i-groups = octree nodes of <= ncrit particles; j-list of a group = every
-//    particle or ghost within r_cut of the group's bounding box.
+//    particle or ghost within r_cut of the group's bounding box, in the order
+//    of a uniform grid of cells of side >= r_cut (cells by x, y, z index, a
+//    cell's particles by index).
 //  * pgt_bh_build: open domain; i-groups = octree nodes of <= ncrit particles;
 //    the tree walk of source.cpp:660-707 with its opening test (a child is
 //    accepted as a cell when theta^2 * d^2 > s^2, d = distance from the child's
@@ -328,7 +330,6 @@ PGT_API void* pgt_treepm_build(int n, const double* x, double rcut, int ncrit) {
             if (d2 < rc2) jl[g].push_back(p);
           }
         }
-    std::sort(jl[g].begin(), jl[g].end());
   }
   concat(il, r->ioff, r->icnt, r->ilist);
   concat(jl, r->joff, r->jcnt, r->jlist);

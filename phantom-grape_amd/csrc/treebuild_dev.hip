@@ -5,7 +5,7 @@
 // (treebuild.cpp), array for array: Morton keys (21 bits per axis), a stable
 // radix sort, the octree split level by level, bottom-up moments, i-groups of
 // <= ncrit particles in Morton order, TreePM ghosts and grid-based j-lists
-// sorted by particle, the Barnes-Hut walk with the opening test of
+// in grid-cell order, the Barnes-Hut walk with the opening test of
 // PG5/newton_quad/source.cpp:660-707, and groups ordered by decreasing cost.
 // Every FP64 operation that decides or produces a value is evaluated in the
 // host builder's order with contraction off, so the two agree bit for bit;
@@ -19,7 +19,6 @@
 
 #include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_scan.hpp>
-#include <hipcub/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -497,7 +496,10 @@ __device__ inline bool in_reach(const double* __restrict__ box, const double* __
 }
 
 // One wavefront per group walks the grid cells around the group box and
-// counts (fill == false) or writes (fill == true) the particles in reach.
+// counts (fill == false) or writes (fill == true) the particles in reach, in
+// cell order (cells by x, y, z index; a cell's particles by index).  The cells
+// of one (x, y) row are consecutive, so a row is one range of the cell-sorted
+// particles.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __restrict__ box,
                                                       const double* __restrict__ pos, Grid gr,
@@ -520,26 +522,25 @@ __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __re
   int count = 0;
   const int base = FILL ? joff[g] : 0;
   for (int a = c0[0]; a <= c1[0]; ++a)
-    for (int b = c0[1]; b <= c1[1]; ++b)
-      for (int c = c0[2]; c <= c1[2]; ++c) {
-        const int cc = (a * gr.nc + b) * gr.nc + c;
-        const int k0 = cstart[cc], k1 = cstart[cc + 1];
-        for (int k = k0; k < k1; k += 64) {
-          const int kk = k + lane;
-          int p = -1;
-          bool in = false;
-          if (kk < k1) {
-            p = cpart[kk];
-            in = in_reach(bx, pos + 3 * p, rc2);
-          }
-          const u64 mask = __ballot(in);
-          if (FILL && in) {
-            const int below = __popcll(mask & ((1ULL << lane) - 1ULL));
-            jlist[base + count + below] = p;
-          }
-          count += __popcll(mask);
+    for (int b = c0[1]; b <= c1[1]; ++b) {
+      const int row = (a * gr.nc + b) * gr.nc;
+      const int k0 = cstart[row + c0[2]], k1 = cstart[row + c1[2] + 1];
+      for (int k = k0; k < k1; k += 64) {
+        const int kk = k + lane;
+        int p = -1;
+        bool in = false;
+        if (kk < k1) {
+          p = cpart[kk];
+          in = in_reach(bx, pos + 3 * p, rc2);
         }
+        const u64 mask = __ballot(in);
+        if (FILL && in) {
+          const int below = __popcll(mask & ((1ULL << lane) - 1ULL));
+          jlist[base + count + below] = p;
+        }
+        count += __popcll(mask);
       }
+    }
   if (!FILL && lane == 0) jcnt[g] = count;
 }
 
@@ -960,7 +961,7 @@ Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t 
   hipLaunchKernelGGL(k_cell_start, grid_for(ncell + 1), dim3(256), 0, st, ncell, np, scell,
                      cstart);
   PGAMD_HIP_CHECK(hipGetLastError());
-  // j-lists: count, offsets, fill (cell order), sort each list by particle
+  // j-lists: count, offsets, fill (cell order)
   double* box = B.tmp.get<double>(6 * (size_t)std::max(ng, 1));
   hipLaunchKernelGGL(k_group_box, grid_for(ng), dim3(256), 0, st, ng, gnode, T.nd, R->pos, box);
   R->jcnt = R->mem.get<int>(ng);
@@ -969,20 +970,10 @@ Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t 
                      gr, cstart, cpart, rcut, R->jcnt, nullptr, nullptr);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->nj = B.scan(R->jcnt, R->joff, ng);
-  int* jl = B.tmp.get<int>(R->nj);
   R->jlist = R->mem.get<int>(R->nj);
   hipLaunchKernelGGL((k_treepm_jlist<true>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
-                     gr, cstart, cpart, rcut, R->jcnt, R->joff, jl);
+                     gr, cstart, cpart, rcut, R->jcnt, R->joff, R->jlist);
   PGAMD_HIP_CHECK(hipGetLastError());
-  if (R->nj > 0) {
-    size_t b = 0;
-    const unsigned* kin = reinterpret_cast<const unsigned*>(jl);
-    unsigned* kout = reinterpret_cast<unsigned*>(R->jlist);
-    PGAMD_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(
-        nullptr, b, kin, kout, (int)R->nj, ng, R->joff, R->joff + 1, 0, bits_for(np), st));
-    PGAMD_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(
-        B.cub_tmp(b), b, kin, kout, (int)R->nj, ng, R->joff, R->joff + 1, 0, bits_for(np), st));
-  }
   R->coff = R->mem.get<int>(ng + 1);
   R->ccnt = R->mem.get<int>(ng);
   PGAMD_HIP_CHECK(hipMemsetAsync(R->coff, 0, sizeof(int) * (size_t)(ng + 1), st));

@@ -28,13 +28,21 @@ def test_treepm_jlists_are_the_particles_in_reach():
     m, x = ics.clumpy_box(3000, 14)
     L = pgamd.treepm_lists(x, rcut, 64)
     P = L["pos"]
+    glo, gL = -rcut, 1.0 + 2 * rcut
+    nc = max(1, int(np.floor(gL / rcut)))
+    h = gL / nc
     for g in range(0, len(L["ioff"]), 7):
         ii = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
         jj = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
         bmin, bmax = P[ii].min(0), P[ii].max(0)
         e = np.maximum(np.maximum(bmin - P, P - bmax), 0.0)
         want = np.nonzero((e * e).sum(1) < rcut * rcut)[0]
-        np.testing.assert_array_equal(jj, want)
+        np.testing.assert_array_equal(np.sort(jj), want)
+        # listed in grid-cell order (cells of side >= rcut over [-rcut, 1 + rcut),
+        # by x, y, z index), a cell's particles by index
+        c = np.clip(np.floor((P[jj] - glo) / h).astype(np.int64), 0, nc - 1)
+        key = ((c[:, 0] * nc + c[:, 1]) * nc + c[:, 2]) * len(P) + jj
+        assert np.all(np.diff(key) > 0)
     # groups partition the particles in Morton order
     np.testing.assert_array_equal(np.sort(L["ilist"]), np.arange(len(x)))
 
