@@ -85,6 +85,10 @@ struct Board {
   int npend = 0;
   std::vector<int> slot;  // address -> queue slot, -1 if none
   DevBuf pend_dev;
+  // recorded after the last flush's copy and scatter: the queue is written
+  // again only once they are done
+  hipEvent_t pend_ev = nullptr;
+  bool pend_busy = false;
   double ti = 0.0;
   int predict_flag = 0;
   int nj_pred = -1;  // j count of the last prediction
@@ -210,6 +214,7 @@ void board_open(Board& b) {
     int v = std::atoi(s);
     if (v > 0) b.npipes = v;
   }
+  PGAMD_HIP_CHECK(hipEventCreateWithFlags(&b.pend_ev, hipEventDisableTiming));
   b.open = true;
   board_alloc_j(b, kInitialJCap);
   b.pend.reserve(sizeof(JRecord) * 1024);
@@ -218,6 +223,7 @@ void board_open(Board& b) {
 void board_close(Board& b) {
   if (!b.open) return;
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  PGAMD_HIP_CHECK(hipEventDestroy(b.pend_ev));
   b.js_buf.release();
   b.jp_buf.release();
   b.pend_dev.release();
@@ -241,6 +247,10 @@ bool board_check_open(const char* fn) {
 
 void board_queue_j(Board& b, int padr, int pidx, double t, double m, const double* acc2,
                    const double* jrk6, const double* vel, const double* pos) {
+  if (b.pend_busy) {  // the previous flush may still be reading the queue
+    PGAMD_HIP_CHECK(hipEventSynchronize(b.pend_ev));
+    b.pend_busy = false;
+  }
   int s = b.slot[padr];
   if (s < 0) {
     b.pend.reserve(sizeof(JRecord) * (size_t)(b.npend + 1), true);
@@ -269,8 +279,10 @@ void board_flush_j(Board& b) {
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
                      b.npend, b.pend_dev.as<JRecord>(), b.js);
   PGAMD_HIP_CHECK(hipGetLastError());
-  // The pinned queue is reused by the next set_j_particle: wait for the copy.
-  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  // No wait here: the next set_j_particle waits for this event before it
+  // writes the pinned queue again (by then lasthalf has usually synchronised).
+  PGAMD_HIP_CHECK(hipEventRecord(b.pend_ev, b.stream));
+  b.pend_busy = true;
   const JRecord* r = b.pend.as<JRecord>();
   for (int k = 0; k < b.npend; ++k) b.slot[r[k].addr] = -1;
   b.npend = 0;

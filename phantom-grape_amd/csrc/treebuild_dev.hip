@@ -208,13 +208,22 @@ __global__ void k_bbox_part(int n, const double* __restrict__ x, double* __restr
     for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = s[k][0];
 }
 
+// One wavefront over the per-block results (minima and maxima do not depend
+// on the order).
 __global__ void k_bbox_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int lane = threadIdx.x;
+  if (blockIdx.x != 0 || lane >= 64) return;
   double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-  for (int b = 0; b < nb; ++b)
+  for (int b = lane; b < nb; b += 64)
     for (int k = 0; k < 6; ++k)
       v[k] = k < 3 ? fmin(v[k], part[6 * b + k]) : fmax(v[k], part[6 * b + k]);
-  for (int k = 0; k < 6; ++k) out[k] = v[k];
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 6; ++k) {
+      const double o = __shfl_xor(v[k], off);
+      v[k] = k < 3 ? fmin(v[k], o) : fmax(v[k], o);
+    }
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) out[k] = v[k];
 }
 
 // Particles in Morton order: pos[p] = x[perm[p]], mass likewise (optional).
@@ -467,20 +476,29 @@ __global__ void k_cell_start(int ncell, int np, const int* __restrict__ scell,
   cstart[c] = a;
 }
 
-// Group bounding boxes: [g][6] = min xyz, max xyz of the group's particles.
-__global__ void k_group_box(int ng, const int* __restrict__ gnode, Nodes nd,
-                            const double* __restrict__ pos, double* __restrict__ box) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+// Group bounding boxes: [g][6] = min xyz, max xyz of the group's particles;
+// one wavefront per group (minima and maxima do not depend on the order).
+__global__ __launch_bounds__(256) void k_group_box(int ng, const int* __restrict__ gnode, Nodes nd,
+                                                   const double* __restrict__ pos,
+                                                   double* __restrict__ box) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;  // uniform per wavefront
+  const int k = gnode[g], lo = nd.lo[k], hi = nd.hi[k];
   double bmin[3] = {1e30, 1e30, 1e30}, bmax[3] = {-1e30, -1e30, -1e30};
-  for (int p = nd.lo[gnode[g]]; p < nd.hi[gnode[g]]; ++p)
+  for (int p = lo + lane; p < hi; p += 64)
     for (int d = 0; d < 3; ++d) {
       bmin[d] = fmin(bmin[d], pos[3 * p + d]);
       bmax[d] = fmax(bmax[d], pos[3 * p + d]);
     }
-  for (int d = 0; d < 3; ++d) {
-    box[6 * g + d] = bmin[d];
-    box[6 * g + 3 + d] = bmax[d];
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = fmin(bmin[d], __shfl_xor(bmin[d], off));
+      bmax[d] = fmax(bmax[d], __shfl_xor(bmax[d], off));
+    }
+  if (lane < 3) {
+    box[6 * g + lane] = lane == 0 ? bmin[0] : (lane == 1 ? bmin[1] : bmin[2]);
+    box[6 * g + 3 + lane] = lane == 0 ? bmax[0] : (lane == 1 ? bmax[1] : bmax[2]);
   }
 }
 
@@ -963,7 +981,7 @@ Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t 
   PGAMD_HIP_CHECK(hipGetLastError());
   // j-lists: count, offsets, fill (cell order)
   double* box = B.tmp.get<double>(6 * (size_t)std::max(ng, 1));
-  hipLaunchKernelGGL(k_group_box, grid_for(ng), dim3(256), 0, st, ng, gnode, T.nd, R->pos, box);
+  hipLaunchKernelGGL(k_group_box, grid_for(ng, 4), dim3(256), 0, st, ng, gnode, T.nd, R->pos, box);
   R->jcnt = R->mem.get<int>(ng);
   R->joff = R->mem.get<int>(ng + 1);
   hipLaunchKernelGGL((k_treepm_jlist<false>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
