@@ -133,12 +133,30 @@ __host__ __device__ inline Plan pg6h_plan(int nact, int nj, int target, int min_
 // ---------------------------------------------------------------- next time
 // Folds the wavefront minimum of v (lanes without a value pass 1e300) into
 // *dst.  The atomic's result is consumed so that it has completed before the
-// wavefront goes on (pg6h_finish's last block relies on that).
+// wavefront goes on.
 __device__ inline void fold_min(double v, unsigned long long* dst) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
   if ((threadIdx.x & 63) == 0 && v < 1e300) {
     const unsigned long long old = atomicMin(dst, (unsigned long long)__double_as_longlong(v));
     asm volatile("" ::"v"(old));
+  }
+}
+
+// The same for the whole block, with one atomic per block (all the block's
+// atomics on one address would queue behind each other).  Every thread of
+// the block calls it; thread 0 waits for the atomic's result (pg6h_finish's
+// last block relies on that).  Blocks of at most 1024 threads.
+__device__ inline void fold_min_block(double v, unsigned long long* dst) {
+  __shared__ double s_min[16];
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) v = fmin(v, s_min[w]);
+    if (v < 1e300) {
+      const unsigned long long old = atomicMin(dst, (unsigned long long)__double_as_longlong(v));
+      asm volatile("" ::"v"(old));
+    }
   }
 }
 
@@ -171,7 +189,7 @@ __global__ __launch_bounds__(256) void pg6h_select(int n, State S, Params P, Ctl
   pg6::pg6_predict_j(i, valid, tnext, js, jp);
   const double tn = valid ? S.t[i] + S.dt[i] : 1e300;
   const bool a = valid && tn == tnext;
-  fold_min(a ? 1e300 : tn, &ctl->tmin_next);  // the particles that do not move
+  fold_min_block(a ? 1e300 : tn, &ctl->tmin_next);  // the particles that do not move
   const unsigned long long mask = __ballot(a);
   const int lane = threadIdx.x & 63;
   int base = 0;
@@ -361,7 +379,7 @@ __global__ __launch_bounds__(256) void pg6h_finish(int n, Params P, Ctl* ctl, in
         tmin = fmin(tmin, correct_one(k, act[k], S, P, ipos, ivel, f, js, &ctl->bad));
     }
   }
-  fold_min(tmin, &ctl->tmin_next);
+  fold_min_block(tmin, &ctl->tmin_next);
   // The last block to get here closes the step.  It reads nothing that other
   // blocks wrote in this kernel (only nact and t_next, read by every block
   // before it counted itself done), so no memory fence is needed: a
@@ -381,7 +399,8 @@ __global__ __launch_bounds__(256) void pg6h_finish(int n, Params P, Ctl* ctl, in
       ctl->t_gl = tnext;
     }
     ctl->nact = 0;
-    // every block's fold into tmin_next has completed (fold_min waits for it)
+    // every block's fold into tmin_next has completed (fold_min_block waits
+    // for it)
     ctl->tmin = atomicExch(&ctl->tmin_next, kInfBits);
     ctl->done = 0;
   }

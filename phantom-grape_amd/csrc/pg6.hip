@@ -1355,7 +1355,9 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.target = 16 * cus;  // wavefront items per force launch: 4 per SIMD
   h.min_blocks = std::min(ceil_div(n, 256), 4 * cus);
   h.force_blocks = 4 * cus * 4 / pg6h::kForceWaves;  // 4 wavefronts per SIMD
-  h.finish_blocks = std::max(1, std::min(ceil_div(n, 64), cus));
+  // blocks of 256 threads: a wavefront per active particle of a typical block
+  // step (a few hundred), and few blocks to count in Ctl::done
+  h.finish_blocks = std::max(1, std::min(ceil_div(n, 256), cus));
   // largest force split: ns * ntiles <= target / kForceWaves + ntiles (pg6h_plan)
   const size_t nsplit_i = ((size_t)h.target / pg6h::kForceWaves + ceil_div(n, 64)) * 64;
   // state: x, v, a, j (3n each), pot, t, dt, m (n each)
