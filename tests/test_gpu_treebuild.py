@@ -71,6 +71,27 @@ def test_coincident_particles_behave_as_host_builder():
     assert d["icnt"].sum() == 2000
 
 
+@pytest.mark.parametrize("case", ["coincident", "n2", "n9"])
+def test_bh_edge_cases_behave_as_host_builder(case):
+    """Barnes-Hut walk on a node that the key cannot split (more than nleaf
+    coincident particles: a walk that opens it finds no children and lists
+    nothing, in both builders) and on tiny systems (the root a leaf, or one
+    level of children)."""
+    import pgamd
+    rng = np.random.default_rng(6)
+    if case == "coincident":
+        x = np.vstack([rng.uniform(-1, 1, (2000, 3)), np.full((40, 3), 0.25)])
+    else:
+        x = rng.uniform(-1, 1, (int(case[1:]), 3))
+    m = rng.uniform(0.5, 1.5, len(x)) / len(x)
+    h = pgamd.bh_lists(x, m, 0.5, 64, 4)
+    d_x, d_m = dev(x), dev(m)
+    L = pgamd.DevLists.bh(d_x.data_ptr(), d_m.data_ptr(), len(x), 0.5, 64, 4)
+    d = L.host()
+    L.free()
+    same_lists(h, d, cells=True)
+
+
 def test_gpu_lists_give_identical_batch_forces():
     import torch
     import pgamd
