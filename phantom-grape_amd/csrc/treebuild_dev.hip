@@ -262,52 +262,60 @@ __device__ inline int lower_digit(const u64* key, int a, int b, int shift, int o
   return a;
 }
 
+// Octant ranges of the nodes [a, b) of one level: eight threads per node,
+// thread o finding where octant o ends (the first index of the node's range
+// whose octant digit exceeds o), so the eight binary searches run side by
+// side.  end[8 * (k - a) + o] keeps the ends for k_split_fill; nchild[k - a]
+// = the number of non-empty octants (0 for a node that is not split).
 __global__ void k_split_count(int a, int b, Nodes nd, const u64* __restrict__ key, int split,
-                              int* __restrict__ nchild) {
-  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= b) return;
-  const int lo = nd.lo[k], hi = nd.hi[k], lev = nd.lev[k];
-  int cnt = 0;
-  if (hi - lo > split && lev < kLevels) {
-    const int shift = 3 * (kLevels - 1 - lev);
-    int s0 = lo;
-    for (int o = 0; o < 8; ++o) {
-      const int s1 = o == 7 ? hi : lower_digit(key, s0, hi, shift, o + 1);
-      cnt += s1 > s0;
-      s0 = s1;
-    }
+                              int* __restrict__ end, int* __restrict__ nchild) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = a + (int)(t >> 3), o = (int)(t & 7);
+  int lo = 0, e = 0;
+  bool splits = false;
+  if (k < b) {
+    const int hi = nd.hi[k], lev = nd.lev[k];
+    lo = nd.lo[k];
+    e = lo;
+    splits = hi - lo > split && lev < kLevels;
+    if (splits) e = o == 7 ? hi : lower_digit(key, lo, hi, 3 * (kLevels - 1 - lev), o + 1);
+    end[8 * (size_t)(k - a) + o] = e;
   }
-  nchild[k - a] = cnt;
+  // a node's eight threads are one aligned group of eight lanes: octant o
+  // starts where octant o - 1 ends
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl(e, (lane + 63) & 63);
+  const unsigned long long m = __ballot(splits && e > (o == 0 ? lo : prev));
+  if (k < b && o == 0) nchild[k - a] = __popcll((m >> lane) & 0xffULL);
 }
 
-__global__ void k_split_fill(int a, int b, Nodes nd, const u64* __restrict__ key, int split,
+// Children of the nodes [a, b) from the octant ends of k_split_count: eight
+// threads per node, thread o writing child o (numbered after the level, in
+// node order, then octant order).
+__global__ void k_split_fill(int a, int b, Nodes nd, const int* __restrict__ end,
                              const int* __restrict__ off, double L) {
 #pragma clang fp contract(off)
-  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = a + (int)(t >> 3), o = (int)(t & 7);
   if (k >= b) return;
-  const int lo = nd.lo[k], hi = nd.hi[k], lev = nd.lev[k];
-  if (!(hi - lo > split && lev < kLevels)) return;
-  const int shift = 3 * (kLevels - 1 - lev);
+  const int* e = end + 8 * (size_t)(k - a);
+  const int lo = nd.lo[k], lev = nd.lev[k];
+  const int s0 = o == 0 ? lo : e[o - 1], s1 = e[o];
+  if (s1 <= s0) return;
+  int rank = 0;  // non-empty octants before o
+  for (int q = 0; q < o; ++q) rank += e[q] > (q == 0 ? lo : e[q - 1]);
+  const int id = b + off[k - a] + rank;
   const double l = ldexp(L, -lev);  // L halved lev times, exactly
-  int id = b + off[k - a];
-  int s0 = lo;
-  for (int o = 0; o < 8; ++o) {
-    const int s1 = o == 7 ? hi : lower_digit(key, s0, hi, shift, o + 1);
-    if (s1 > s0) {
-      nd.lo[id] = s0;
-      nd.hi[id] = s1;
-      nd.par[id] = k;
-      nd.lev[id] = lev + 1;
-      for (int t = 0; t < 8; ++t) nd.child[8 * id + t] = -1;
-      // octant bits: x is bit 2, y bit 1, z bit 0
-      nd.c[3 * id + 0] = nd.c[3 * k + 0] + ((o & 4) ? 0.25 : -0.25) * l;
-      nd.c[3 * id + 1] = nd.c[3 * k + 1] + ((o & 2) ? 0.25 : -0.25) * l;
-      nd.c[3 * id + 2] = nd.c[3 * k + 2] + ((o & 1) ? 0.25 : -0.25) * l;
-      nd.child[8 * k + o] = id;
-      ++id;
-    }
-    s0 = s1;
-  }
+  nd.lo[id] = s0;
+  nd.hi[id] = s1;
+  nd.par[id] = k;
+  nd.lev[id] = lev + 1;
+  for (int q = 0; q < 8; ++q) nd.child[8 * id + q] = -1;
+  // octant bits: x is bit 2, y bit 1, z bit 0
+  nd.c[3 * id + 0] = nd.c[3 * k + 0] + ((o & 4) ? 0.25 : -0.25) * l;
+  nd.c[3 * id + 1] = nd.c[3 * k + 1] + ((o & 2) ? 0.25 : -0.25) * l;
+  nd.c[3 * id + 2] = nd.c[3 * k + 2] + ((o & 1) ? 0.25 : -0.25) * l;
+  nd.child[8 * k + o] = id;
 }
 
 // Moments of the nodes [a, b) of one level, children already done.
@@ -851,8 +859,9 @@ void build_octree(Build& B, Lists& R, int n, const double* x, const double* m,
     if (b == a) break;
     int* nch = B.tmp.get<int>(b - a);
     int* off = B.tmp.get<int>(b - a + 1);
-    hipLaunchKernelGGL(k_split_count, grid_for(b - a), dim3(256), 0, st, a, b, T.nd, skey, split,
-                       nch);
+    int* end = B.tmp.get<int>(8 * (size_t)(b - a));
+    hipLaunchKernelGGL(k_split_count, grid_for(8LL * (b - a)), dim3(256), 0, st, a, b, T.nd, skey,
+                       split, end, nch);
     PGAMD_HIP_CHECK(hipGetLastError());
     const int add = B.scan(nch, off, b - a);
     if (add == 0) {
@@ -874,7 +883,7 @@ void build_octree(Build& B, Lists& R, int n, const double* x, const double* m,
       T.nd = g;
       cap = ncap;
     }
-    hipLaunchKernelGGL(k_split_fill, grid_for(b - a), dim3(256), 0, st, a, b, T.nd, skey, split,
+    hipLaunchKernelGGL(k_split_fill, grid_for(8LL * (b - a)), dim3(256), 0, st, a, b, T.nd, end,
                        off, L);
     PGAMD_HIP_CHECK(hipGetLastError());
     T.level_start.push_back(b + add);
