@@ -76,9 +76,48 @@ struct Mem {
   }
 };
 
-// out[n] = out[n - 1] + in[n - 1] (0 for n = 0): the total of an exclusive scan.
-__global__ void k_scan_total(const int* __restrict__ in, int* __restrict__ out, int n) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = n > 0 ? out[n - 1] + in[n - 1] : 0;
+// out[n] = out[n - 1] + in[n - 1] (0 for n = 0): the total of an exclusive
+// scan, also written to *host (mapped pinned memory).
+__global__ void k_scan_total(const int* __restrict__ in, int* __restrict__ out, int n,
+                             int* __restrict__ host) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int t = n > 0 ? out[n - 1] + in[n - 1] : 0;
+    out[n] = t;
+    *host = t;
+  }
+}
+
+// Exclusive scan of n <= kSmallScan ints in one block: thread q sums its
+// consecutive chunk, the block scans the chunk sums, thread q writes its
+// chunk's prefixes.  out[n] = total, also written to *host.
+constexpr int kSmallScanThreads = 1024;
+constexpr int kSmallScan = 64 * kSmallScanThreads;
+__global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(const int* __restrict__ in,
+                                                                  int* __restrict__ out, int n,
+                                                                  int* __restrict__ host) {
+  __shared__ int s[kSmallScanThreads];
+  const int q = threadIdx.x;
+  const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
+  const int a = min(n, q * per), b = min(n, a + per);
+  int sum = 0;
+  for (int i = a; i < b; ++i) sum += in[i];
+  s[q] = sum;
+  __syncthreads();
+  for (int d = 1; d < kSmallScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = q >= d ? s[q - d] : 0;
+    __syncthreads();
+    s[q] += v;
+    __syncthreads();
+  }
+  int run = s[q] - sum;  // exclusive prefix of this chunk
+  for (int i = a; i < b; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (q == kSmallScanThreads - 1) {
+    out[n] = s[q];
+    *host = s[q];
+  }
 }
 
 // Result of one build: device arrays in the host builder's layout.
@@ -93,6 +132,14 @@ struct Lists {
   int* order = nullptr;
   double* cells = nullptr;  // [ncells][10]
 };
+
+// One pinned int per process for scan totals; builds run one at a time under
+// the library's mutex.
+int* scan_total_host() {
+  static int* p = nullptr;
+  if (!p) PGAMD_HIP_CHECK(hipHostMalloc((void**)&p, sizeof(int), hipHostMallocMapped));
+  return p;
+}
 
 // Temporary storage for one build, released at its end.
 struct Build {
@@ -122,14 +169,24 @@ struct Build {
     PGAMD_HIP_CHECK(hipStreamSynchronize(st));
     return v;
   }
-  // out[k] = sum in[0..k), out[n] = total (out has n + 1 entries); returns total.
+  // out[k] = sum in[0..k), out[n] = total (out has n + 1 entries); returns
+  // total.  The total comes back through mapped pinned memory: one kernel and
+  // a synchronisation for small n, the hipCUB scan for large n.
   int scan(const int* in, int* out, int n) {
-    size_t b = 0;
-    PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
-    PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub_tmp(b), b, in, out, n, st));
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, in, out, n);
+    int* host = scan_total_host();
+    int* dev = nullptr;
+    PGAMD_HIP_CHECK(hipHostGetDevicePointer((void**)&dev, host, 0));
+    if (n <= kSmallScan) {
+      hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallScanThreads), 0, st, in, out, n, dev);
+    } else {
+      size_t b = 0;
+      PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, out, n, st));
+      PGAMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cub_tmp(b), b, in, out, n, st));
+      hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, in, out, n, dev);
+    }
     PGAMD_HIP_CHECK(hipGetLastError());
-    return read(out + n);
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    return *static_cast<volatile int*>(host);
   }
   // Stable radix sort of (key, value) pairs; int keys must be >= 0.
   void sort_pairs(const int* kin, int* kout, const int* vin, int* vout, int n, int end_bit) {
