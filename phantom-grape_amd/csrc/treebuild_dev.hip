@@ -525,6 +525,16 @@ __global__ void k_cell_of(int np, const double* __restrict__ pos, Grid g, int* _
   idx[p] = p;
 }
 
+// Positions in cell order: cpos[k] = pos[cpart[k]], so the list walk reads
+// each row of cells as one contiguous range.
+__global__ void k_cell_pos(int np, const int* __restrict__ cpart, const double* __restrict__ pos,
+                           double* __restrict__ cpos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int p = cpart[k];
+  for (int d = 0; d < 3; ++d) cpos[3 * k + d] = pos[3 * p + d];
+}
+
 // cstart[c] = first position of cell c in the sorted cell array (c <= ncell).
 __global__ void k_cell_start(int ncell, int np, const int* __restrict__ scell,
                              int* __restrict__ cstart) {
@@ -582,10 +592,10 @@ __device__ inline bool in_reach(const double* __restrict__ box, const double* __
 // counts (fill == false) or writes (fill == true) the particles in reach, in
 // cell order (cells by x, y, z index; a cell's particles by index).  The cells
 // of one (x, y) row are consecutive, so a row is one range of the cell-sorted
-// particles.
+// particles (cpos: positions in that order).
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __restrict__ box,
-                                                      const double* __restrict__ pos, Grid gr,
+                                                      const double* __restrict__ cpos, Grid gr,
                                                       const int* __restrict__ cstart,
                                                       const int* __restrict__ cpart, double rcut,
                                                       int* __restrict__ jcnt,
@@ -610,16 +620,11 @@ __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __re
       const int k0 = cstart[row + c0[2]], k1 = cstart[row + c1[2] + 1];
       for (int k = k0; k < k1; k += 64) {
         const int kk = k + lane;
-        int p = -1;
-        bool in = false;
-        if (kk < k1) {
-          p = cpart[kk];
-          in = in_reach(bx, pos + 3 * p, rc2);
-        }
+        const bool in = kk < k1 && in_reach(bx, cpos + 3 * (size_t)kk, rc2);
         const u64 mask = __ballot(in);
         if (FILL && in) {
           const int below = __popcll(mask & ((1ULL << lane) - 1ULL));
-          jlist[base + count + below] = p;
+          jlist[base + count + below] = cpart[kk];
         }
         count += __popcll(mask);
       }
@@ -1050,12 +1055,14 @@ Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t 
   hipLaunchKernelGGL(k_group_box, grid_for(ng, 4), dim3(256), 0, st, ng, gnode, T.nd, R->pos, box);
   R->jcnt = R->mem.get<int>(ng);
   R->joff = R->mem.get<int>(ng + 1);
-  hipLaunchKernelGGL((k_treepm_jlist<false>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
+  double* cpos = B.tmp.get<double>(3 * (size_t)np);
+  hipLaunchKernelGGL(k_cell_pos, grid_for(np), dim3(256), 0, st, np, cpart, R->pos, cpos);
+  hipLaunchKernelGGL((k_treepm_jlist<false>), grid_for(ng, 4), dim3(256), 0, st, ng, box, cpos,
                      gr, cstart, cpart, rcut, R->jcnt, nullptr, nullptr);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->nj = B.scan(R->jcnt, R->joff, ng);
   R->jlist = R->mem.get<int>(R->nj);
-  hipLaunchKernelGGL((k_treepm_jlist<true>), grid_for(ng, 4), dim3(256), 0, st, ng, box, R->pos,
+  hipLaunchKernelGGL((k_treepm_jlist<true>), grid_for(ng, 4), dim3(256), 0, st, ng, box, cpos,
                      gr, cstart, cpart, rcut, R->jcnt, R->joff, R->jlist);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->coff = R->mem.get<int>(ng + 1);
