@@ -445,6 +445,8 @@ def bench_c1(args, stream, cpu):
                                "kind": "reference", "sample": f"{len(sel)} i x {n} j",
                                "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
                                "accuracy_vs_fp64": _errs(ra, oa)}
+    if cpu:
+        out["dropin_direct"] = _direct_dropin()
     return out
 
 
@@ -730,6 +732,36 @@ def bench_c2i(args, stream, cpu):
     if cpu:
         res["dropin_banana"] = _banana_dropin(m, x, v, t_end)
     return res
+
+
+def _direct_dropin():
+    """The reference's own PG5 benchmark (user guide 5.1, Listing 6: direct
+    with init/pl4k, 2.06e9 interactions/s on its CPU): PG5/newton/direct, built
+    from its sources by oracle/ref.mk, on a pl4k-style Plummer N = 4096
+    (eps = 0.02 and 1000 leapfrog steps, as direct.c fixes them), linked
+    against the reference library and against libpg5.so.  Its own report,
+    interactions per second over each 100 steps (direct.c:306-316): the
+    median of the nine reports."""
+    import re
+    import tempfile
+
+    import numpy as np
+
+    import ics
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import dropin
+    if not all(os.path.exists(dropin.program(p)) for p in ("direct_ref", "direct_pgamd")):
+        return None
+    m, x, v = ics.plummer(4096, 4096)
+    out = {"setup": "pl4k-style Plummer N = 4096, eps = 0.02, dt = 0.01, 1000 steps",
+           "threads": _threads(), "published_reference_interactions_s": 2.060791e9}
+    for key, prog in (("reference_library", "direct_ref"), ("libpg5", "direct_pgamd")):
+        with tempfile.TemporaryDirectory() as d:
+            dropin.write_direct_input(os.path.join(d, "pl"), m, x, v)
+            o = dropin.run(prog, ["pl", "out"], d, timeout=600, threads=_threads())
+        r = [float(a) for a in re.findall(r"^(\S+) interaction per sec", o, re.M)]
+        out[key + "_interactions_s"] = float(f"{np.median(r):.4e}") if r else None
+    return out
 
 
 def _banana_dropin(m, x, v, t_end):
