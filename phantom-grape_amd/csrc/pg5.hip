@@ -79,7 +79,7 @@ struct Board {
   HostBuf opipe;       // double acc[3], pot per pipeline
   // staging of the calculate_force calls
   HostBuf h_in, h_out;
-  DevBuf d_in, d_out, d_part, d_counter;
+  DevBuf d_in, d_part, d_counter;
 };
 
 Board g_boards[kMaxDev];
@@ -132,6 +132,39 @@ void jmem_reserve_host(JMem& m, int need, size_t rec_bytes, bool eps) {
   m.hcap = cap;
 }
 
+// Copies of small uploads from pinned host memory, read by a kernel through
+// the memory's device mapping: a DMA copy costs several microseconds of
+// latency, a per-call cost the drop-in API pays on every force call.
+__global__ void copy_in16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+__global__ void copy_in4(const unsigned* __restrict__ src, unsigned* __restrict__ dst, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+// dst (device) = bytes [off, off + bytes) of the pinned buffer h; bytes is a
+// multiple of 4.  Asynchronous on st: h must not change until it has run.
+void upload(void* dst, const HostBuf& h, size_t off, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return;
+  char* base = nullptr;
+  PGAMD_HIP_CHECK(hipHostGetDevicePointer((void**)&base, h.ptr, 0));
+  const char* src = base + off;
+  const bool wide = (((uintptr_t)src | (uintptr_t)dst | bytes) & 15) == 0;
+  const size_t n = wide ? bytes / 16 : bytes / 4;
+  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+  if (wide)
+    hipLaunchKernelGGL(copy_in16, dim3(blocks), dim3(256), 0, st,
+                       reinterpret_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  else
+    hipLaunchKernelGGL(copy_in4, dim3(blocks), dim3(256), 0, st,
+                       reinterpret_cast<const unsigned*>(src), static_cast<unsigned*>(dst), n);
+  PGAMD_HIP_CHECK(hipGetLastError());
+}
+
 // Upload the dirty range of a j-memory before a kernel reads [0, nread).
 void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
   const size_t need = rec_bytes * (size_t)std::max(nread, m.hcap);
@@ -144,14 +177,10 @@ void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
   }
   if (m.lo < m.hi) {
     const size_t off = rec_bytes * (size_t)m.lo;
-    PGAMD_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(m.d.ptr) + off,
-                                   static_cast<char*>(m.h.ptr) + off,
-                                   rec_bytes * (size_t)(m.hi - m.lo), hipMemcpyHostToDevice,
-                                   st));
+    upload(static_cast<char*>(m.d.ptr) + off, m.h, off, rec_bytes * (size_t)(m.hi - m.lo), st);
     if (eps)
-      PGAMD_HIP_CHECK(hipMemcpyAsync(m.de.as<float>() + m.lo, m.he.as<float>() + m.lo,
-                                     sizeof(float) * (size_t)(m.hi - m.lo),
-                                     hipMemcpyHostToDevice, st));
+      upload(m.de.as<float>() + m.lo, m.he, sizeof(float) * (size_t)m.lo,
+             sizeof(float) * (size_t)(m.hi - m.lo), st);
     m.lo = m.hi = 0;
   }
 }
@@ -242,7 +271,7 @@ void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* 
     }
   }
   if (p.nsplit > 1) {
-    hipLaunchKernelGGL(pg5::pg5_reduce, dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni,
+    hipLaunchKernelGGL(pg5::pg5_reduce, dim3(ceil_div(ni, 64)), dim3(256), 0, st, ni,
                        p.nsplit, part, out);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
@@ -255,6 +284,12 @@ void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* 
 void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4* pre, int ni,
                 double (*a)[3], double* p, int cellmem) {
   if (ni <= 0) return;
+  const int nread = cellmem < 0 ? b.n : b.nc[cellmem];
+  if (nread <= 0) {  // nothing to sum over
+    std::memset(a, 0, sizeof(double) * 3 * (size_t)ni);
+    std::memset(p, 0, sizeof(double) * (size_t)ni);
+    return;
+  }
   const size_t in_bytes = sizeof(float4) * (size_t)ni;
   b.h_in.reserve(in_bytes);
   b.d_in.reserve(in_bytes);
@@ -270,25 +305,23 @@ void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4
       hin[i].w = eps2 ? (float)eps2[i] : e2all;
     }
   }
-  PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
-                                 b.stream));
+  upload(b.d_in.ptr, b.h_in, 0, in_bytes, b.stream);
+  // The kernels write the results straight into pinned host memory through
+  // its device mapping; the synchronisation below makes them visible.
   const size_t out_bytes = sizeof(double) * 4 * (size_t)ni;
-  b.d_out.reserve(out_bytes);
   b.h_out.reserve(out_bytes);
-  Outputs5 o{b.d_out.as<double>(), b.d_out.as<double>() + 3 * (size_t)ni};
+  double* dout = nullptr;
+  PGAMD_HIP_CHECK(hipHostGetDevicePointer((void**)&dout, b.h_out.ptr, 0));
+  Outputs5 o{dout, dout + 3 * (size_t)ni};
   if (cellmem < 0) {
-    const int nj = b.n;
-    jmem_sync(b.jm, sizeof(float4), kSym, nj, b.stream);
-    run_force(b.d_part, ni, b.d_in.as<float4>(), nj, b.jm.d.as<float4>(),
+    jmem_sync(b.jm, sizeof(float4), kSym, nread, b.stream);
+    run_force(b.d_part, ni, b.d_in.as<float4>(), nread, b.jm.d.as<float4>(),
               kSym ? b.jm.de.as<float>() : nullptr, nullptr, o, b.stream);
   } else {
-    const int nc = b.nc[cellmem];
-    jmem_sync(b.cm[cellmem], sizeof(Cell), false, nc, b.stream);
-    run_force(b.d_part, ni, b.d_in.as<float4>(), nc, nullptr, nullptr,
+    jmem_sync(b.cm[cellmem], sizeof(Cell), false, nread, b.stream);
+    run_force(b.d_part, ni, b.d_in.as<float4>(), nread, nullptr, nullptr,
               b.cm[cellmem].d.as<Cell>(), o, b.stream);
   }
-  PGAMD_HIP_CHECK(hipMemcpyAsync(b.h_out.ptr, b.d_out.ptr, out_bytes, hipMemcpyDeviceToHost,
-                                 b.stream));
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   const double* ha = b.h_out.as<double>();
   const double* hp = ha + 3 * (size_t)ni;

@@ -232,21 +232,33 @@ __global__ __launch_bounds__(kThreads) void pg5_quad(
   pg5_store(i, nsplit, ni, ax, ay, az, pot, part, out);
 }
 
-__global__ void pg5_reduce(int ni, int nsplit, Partials5 part, Outputs5 out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ni) return;
-  double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
-  for (int s = 0; s < nsplit; ++s) {
-    const size_t o = (size_t)s * ni + i;
-    ax += part.ax[o];
-    ay += part.ay[o];
-    az += part.az[o];
-    pot += part.pot[o];
-  }
-  out.acc[3 * i + 0] = ax;
-  out.acc[3 * i + 1] = ay;
-  out.acc[3 * i + 2] = az;
-  out.pot[i] = pot;
+// Sums the nsplit partials of each i: four threads per i, thread q taking
+// splits q, q + 4, ... in order (a wavefront holds 64 consecutive i of one q,
+// so every load is coalesced), then the four sums added in q order.
+// Deterministic.  Block = 256 threads = 64 i.
+__global__ __launch_bounds__(256) void pg5_reduce(int ni, int nsplit, Partials5 part,
+                                                  Outputs5 out) {
+  __shared__ double s[4][4][64];
+  const int il = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + il;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < ni)
+    for (int sp = q; sp < nsplit; sp += 4) {
+      const size_t o = (size_t)sp * ni + i;
+      v[0] += part.ax[o];
+      v[1] += part.ay[o];
+      v[2] += part.az[o];
+      v[3] += part.pot[o];
+    }
+  for (int c = 0; c < 4; ++c) s[c][q][il] = v[c];
+  __syncthreads();
+  if (q != 0 || i >= ni) return;
+  double t[4];
+  for (int c = 0; c < 4; ++c) t[c] = ((s[c][0][il] + s[c][1][il]) + s[c][2][il]) + s[c][3][il];
+  out.acc[3 * i + 0] = t[0];
+  out.acc[3 * i + 1] = t[1];
+  out.acc[3 * i + 2] = t[2];
+  out.pot[i] = t[3];
 }
 
 }  // namespace pg5
