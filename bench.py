@@ -539,6 +539,8 @@ def bench_c3(args, stream, cpu):
                       "OpenMP over groups with devid = thread (reference 512-entry table)",
             "threads": min(thr, 24), "seconds": round(t, 3),
             "accuracy_vs_closed_form_s2": _errs(ra[sel], f)}
+    if cpu:
+        res["dropin_force_prof"] = _force_prof_dropin()
     return res
 
 
@@ -761,6 +763,30 @@ def _direct_dropin():
             o = dropin.run(prog, ["pl", "out"], d, timeout=600, threads=_threads())
         r = [float(a) for a in re.findall(r"^(\S+) interaction per sec", o, re.M)]
         out[key + "_interactions_s"] = float(f"{np.median(r):.4e}") if r else None
+    return out
+
+
+def _force_prof_dropin():
+    """The reference's own table benchmark (user guide 5.2, Listing 11:
+    force_prof 32768 0.1, 8.5e8 interactions/s on its CPU): PG5/table/
+    force_prof, built from its sources by oracle/ref.mk, linked against the
+    reference's serial table library (its setting.mk default) and against
+    libpg5_table.so.  It times one g5_calculate_force_on_xMC over all N and
+    prints interactions per second on stderr."""
+    import re
+    import subprocess
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import dropin
+    if not all(os.path.exists(dropin.program(p)) for p in ("force_prof_ref", "force_prof_pgamd")):
+        return None
+    out = {"setup": "force_prof 32768 0.1 (uniform cube + one heavy particle, one call)",
+           "published_reference_interactions_s": 8.508592e8}
+    for key, prog in (("reference_library", "force_prof_ref"), ("libpg5_table", "force_prof_pgamd")):
+        r = subprocess.run([dropin.program(prog), "32768", "0.1"], capture_output=True, text=True,
+                           timeout=600, env=dict(os.environ, OMP_NUM_THREADS=str(_threads())))
+        v = re.findall(r"(\S+) interactions / sec", r.stderr) if r.returncode == 0 else []
+        out[key + "_interactions_s"] = float(v[-1]) if v else None
     return out
 
 

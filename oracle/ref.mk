@@ -40,7 +40,8 @@ endef
 
 ALL := $(OUT)/libref_pg5.so $(OUT)/libref_pg5_omp.so $(OUT)/libref_pg5sym.so $(OUT)/libref_pg5quad.so \
        $(OUT)/libref_pg5table.so $(OUT)/libref_pg5table_omp.so $(OUT)/libref_pg6.so \
-       $(OUT)/direct_ref $(OUT)/direct_pgamd $(OUT)/banana_ref $(OUT)/banana_pgamd
+       $(OUT)/direct_ref $(OUT)/direct_pgamd $(OUT)/banana_ref $(OUT)/banana_pgamd \
+       $(OUT)/force_prof_ref $(OUT)/force_prof_pgamd
 
 # The reference's own sample programs, compiled from their sources against
 # their own headers, linked twice: with the reference library (*_ref) and with
@@ -52,6 +53,8 @@ BANANA  := $(REF)/PG6/banana
 DIRECT_SRC := $(DIRECT)/direct.c $(DIRECT)/cpu.c
 BANANA_SRC := $(BANANA)/banana01.c $(BANANA)/io.c $(BANANA)/calc_energy.c $(BANANA)/hermite.c \
               $(BANANA)/timestep.c $(BANANA)/timeprof.c
+FPROF   := $(REF)/PG5/table/force_prof
+FPROF_SRC := $(FPROF)/force_prof.c $(FPROF)/cpu.c
 
 # $(call prog,output,flags,includes,sources,linkflags)
 define prog
@@ -102,6 +105,14 @@ $(OUT)/banana_ref: $(BANANA_SRC) $(OUT)/libref_pg6.so | $(OUT)
 
 $(OUT)/banana_pgamd: $(BANANA_SRC) $(PGLIB)/libpg6.so | $(OUT)
 	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -fopenmp -DNORMAL -Wno-unused-result,-I$(PG6) -I$(BANANA),$(BANANA_SRC),-fopenmp -L$(PGLIB) -lpg6 -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN/../../phantom-grape_amd/lib')
+
+# force_prof as PG5/table/force_prof/Makefile builds it (PG5/table/setting.mk:
+# enable_openmp=no, so the serial table library)
+$(OUT)/force_prof_ref: $(FPROF_SRC) $(OUT)/libref_pg5table.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -std=c99 -Wno-unused-result,-I$(FPROF),$(FPROF_SRC),-L$(abspath $(OUT)) -lref_pg5table -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN')
+
+$(OUT)/force_prof_pgamd: $(FPROF_SRC) $(PGLIB)/libpg5_table.so | $(OUT)
+	$(call prog,$(abspath $@),-O3 -ffast-math -funroll-loops -std=c99 -Wno-unused-result,-I$(FPROF),$(FPROF_SRC),-L$(PGLIB) -lpg5_table -Wl$(COMMA)-rpath$(COMMA)'$$ORIGIN/../../phantom-grape_amd/lib')
 
 COMMA := ,
 

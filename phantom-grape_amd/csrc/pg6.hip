@@ -206,6 +206,10 @@ bool board_reserve(Board& b, int need) {
   return true;
 }
 
+__global__ void open_warm(int* flag) {
+  if (flag) *flag = 0;
+}
+
 void board_open(Board& b) {
   if (b.open) return;
   ensure_device();
@@ -218,6 +222,10 @@ void board_open(Board& b) {
   b.open = true;
   board_alloc_j(b, kInitialJCap);
   b.pend.reserve(sizeof(JRecord) * 1024);
+  // one empty launch loads the library's kernels now, not in the first call
+  hipLaunchKernelGGL(open_warm, dim3(1), dim3(64), 0, b.stream, nullptr);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
 }
 
 void board_close(Board& b) {

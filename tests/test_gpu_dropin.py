@@ -54,3 +54,22 @@ def test_banana_hermite_with_libpg6(tmp_path):
     gpu = dropin.banana_energy_errors(out)
     assert len(gpu) == len(ref) == 3
     assert np.abs(gpu).max() < max(3 * np.abs(ref).max(), 1e-7), (gpu, ref)
+
+
+def test_table_force_prof_with_libpg5_table(tmp_path):
+    """force_prof.c, the reference's table benchmark: N = 32768 in a cube of
+    side 0.1 (libc rand) with one heavy particle at its centre, one
+    g5_calculate_force_on_xMC over all of them with the library's default
+    table.  Each pair's table force is bit-identical to the reference's, so
+    |acc| differs only by the FP32 summation order (~sqrt(N) ulp); pot is 0
+    as the reference's table path returns."""
+    need("force_prof_ref", "force_prof_pgamd")
+    out = {p: np.loadtxt(dropin.run(p, ["32768", "0.1"], tmp_path).splitlines())
+           for p in ("force_prof_ref", "force_prof_pgamd")}
+    a, b = out["force_prof_ref"], out["force_prof_pgamd"]
+    assert a.shape == b.shape == (32767, 3)
+    np.testing.assert_array_equal(b[:, 0], a[:, 0])  # the same particles
+    e = np.abs(b[:, 1] - a[:, 1]) / np.abs(a[:, 1])
+    assert np.median(e) < 1e-5 and e.max() < 1e-4, (np.median(e), e.max())
+    assert np.all(b[:, 2] == 0.0) and np.all(a[:, 2] == 0.0)
+
