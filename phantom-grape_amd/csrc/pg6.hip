@@ -15,6 +15,7 @@
 #include "pgamd_common.h"
 #include "pgamd_api.h"
 #include "pg6_kernels.h"
+#include "host_upload.h"
 #include "pg6_nblist.h"
 #include "cluster_batch.h"
 #include "isa_probe.h"
@@ -84,9 +85,8 @@ struct Board {
   HostBuf pend;
   int npend = 0;
   std::vector<int> slot;  // address -> queue slot, -1 if none
-  DevBuf pend_dev;
-  // recorded after the last flush's copy and scatter: the queue is written
-  // again only once they are done
+  // recorded after the last flush's scatter: the queue is written again
+  // only once it is done
   hipEvent_t pend_ev = nullptr;
   bool pend_busy = false;
   double ti = 0.0;
@@ -234,7 +234,6 @@ void board_close(Board& b) {
   PGAMD_HIP_CHECK(hipEventDestroy(b.pend_ev));
   b.js_buf.release();
   b.jp_buf.release();
-  b.pend_dev.release();
   b.d_in.release();
   b.d_part.release();
   b.d_nb.release();
@@ -280,12 +279,9 @@ void board_queue_j(Board& b, int padr, int pidx, double t, double m, const doubl
 
 void board_flush_j(Board& b) {
   if (b.npend == 0) return;
-  const size_t bytes = sizeof(JRecord) * (size_t)b.npend;
-  b.pend_dev.reserve(bytes);
-  PGAMD_HIP_CHECK(hipMemcpyAsync(b.pend_dev.ptr, b.pend.ptr, bytes, hipMemcpyHostToDevice,
-                                 b.stream));
+  // the scatter kernel reads the pinned queue through its device mapping
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
-                     b.npend, b.pend_dev.as<JRecord>(), b.js);
+                     b.npend, mapped<JRecord>(b.pend), b.js);
   PGAMD_HIP_CHECK(hipGetLastError());
   // No wait here: the next set_j_particle waits for this event before it
   // writes the pinned queue again (by then lasthalf has usually synchronised).
@@ -433,8 +429,7 @@ int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double 
   std::memcpy(hid, pidx, sizeof(int) * (size_t)ni);
   float* hh2 = reinterpret_cast<float*>(hid + ni);
   for (int i = 0; i < ni; ++i) hh2[i] = want_nblist && h2 ? (float)h2[i] : 0.f;
-  PGAMD_HIP_CHECK(hipMemcpyAsync(b.d_in.ptr, b.h_in.ptr, in_bytes, hipMemcpyHostToDevice,
-                                 b.stream));
+  upload(b.d_in.ptr, b.h_in, 0, in_bytes, b.stream);
   const double* dpos = b.d_in.as<double>();
   const double* dvel = dpos + 3 * (size_t)ni;
   const int* did = reinterpret_cast<const int*>(dvel + 3 * (size_t)ni);
