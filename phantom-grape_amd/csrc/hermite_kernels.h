@@ -111,6 +111,8 @@ __device__ inline bool pg6h_skip(const Ctl* c, double dtmax) {
 // addresses, at least min_jper).  About `target` wavefronts of work in all.
 // Partials of range s for active slot k sit at s * stride + k.
 constexpr int kForceWaves = 8;
+// pg6h_force sums the seven components with one wavefront each
+static_assert(kForceWaves >= 7, "a block needs a wavefront per force component");
 
 struct Plan {
   int ntiles, ns, jper, stride;
