@@ -12,6 +12,7 @@
 #include "pgamd_common.h"
 
 namespace pgamd {
+namespace {  // kernels with internal linkage: no exported launch stubs
 
 __global__ void copy_in16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
@@ -24,6 +25,8 @@ __global__ void copy_in4(const unsigned* __restrict__ src, unsigned* __restrict_
        k += (size_t)gridDim.x * blockDim.x)
     dst[k] = src[k];
 }
+
+}  // namespace
 
 // dst (device) = bytes [off, off + bytes) of the pinned buffer h; bytes is a
 // multiple of 4.  Asynchronous on st: h must not change until it has run.
