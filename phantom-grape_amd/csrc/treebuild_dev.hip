@@ -91,7 +91,7 @@ __global__ void k_scan_total(const int* __restrict__ in, int* __restrict__ out, 
 // consecutive chunk, the block scans the chunk sums, thread q writes its
 // chunk's prefixes.  out[n] = total, also written to *host.
 constexpr int kSmallScanThreads = 1024;
-constexpr int kSmallScan = 64 * kSmallScanThreads;
+constexpr int kSmallScan = 4 * kSmallScanThreads;
 __global__ __launch_bounds__(kSmallScanThreads) void k_scan_small(const int* __restrict__ in,
                                                                   int* __restrict__ out, int n,
                                                                   int* __restrict__ host) {
