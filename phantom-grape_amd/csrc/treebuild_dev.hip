@@ -589,11 +589,11 @@ __device__ inline bool in_reach(const double* __restrict__ box, const double* __
 }
 
 // One wavefront per group walks the grid cells around the group box and
-// counts (fill == false) or writes (fill == true) the particles in reach, in
-// cell order (cells by x, y, z index; a cell's particles by index).  The cells
+// writes the particles in reach to jlist[joff[g]...] and their number to
+// jcnt[g], in cell order (cells by x, y, z index; a cell's particles by
+// index).  The cells
 // of one (x, y) row are consecutive, so a row is one range of the cell-sorted
 // particles (cpos: positions in that order).
-template <bool FILL>
 __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __restrict__ box,
                                                       const double* __restrict__ cpos, Grid gr,
                                                       const int* __restrict__ cstart,
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __re
     c1[d] = cidx(gr, bx[3 + d] + rcut);
   }
   int count = 0;
-  const int base = FILL ? joff[g] : 0;
+  const int base = joff[g];
   for (int a = c0[0]; a <= c1[0]; ++a)
     for (int b = c0[1]; b <= c1[1]; ++b) {
       const int row = (a * gr.nc + b) * gr.nc;
@@ -622,14 +622,54 @@ __global__ __launch_bounds__(256) void k_treepm_jlist(int ng, const double* __re
         const int kk = k + lane;
         const bool in = kk < k1 && in_reach(bx, cpos + 3 * (size_t)kk, rc2);
         const u64 mask = __ballot(in);
-        if (FILL && in) {
+        if (in) {
           const int below = __popcll(mask & ((1ULL << lane) - 1ULL));
           jlist[base + count + below] = cpart[kk];
         }
         count += __popcll(mask);
       }
     }
-  if (!FILL && lane == 0) jcnt[g] = count;
+  if (lane == 0) jcnt[g] = count;
+}
+
+// Candidates of each group: the particles in the rows of grid cells its walk
+// visits (an upper bound of its list length).  One wavefront per group.
+__global__ __launch_bounds__(256) void k_treepm_cand(int ng, const double* __restrict__ box,
+                                                     Grid gr, const int* __restrict__ cstart,
+                                                     double rcut, int* __restrict__ ccnt) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;  // uniform per wavefront
+  const double* bx = box + 6 * g;
+  int c0[3], c1[3];
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = cidx(gr, bx[d] - rcut);
+    c1[d] = cidx(gr, bx[3 + d] + rcut);
+  }
+  const int ny = c1[1] - c0[1] + 1, nrows = (c1[0] - c0[0] + 1) * ny;
+  int sum = 0;
+  for (int r = lane; r < nrows; r += 64) {
+    const int row = ((c0[0] + r / ny) * gr.nc + c0[1] + r % ny) * gr.nc;
+    sum += cstart[row + c1[2] + 1] - cstart[row + c0[2]];
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0) ccnt[g] = sum;
+}
+
+// jlist[joff[g] + k] = scratch[soff[g] + k] for k < jcnt[g]: the lists
+// written at their candidate offsets, packed.  One wavefront per group.
+__global__ __launch_bounds__(256) void k_pack_lists(int ng, const int* __restrict__ soff,
+                                                    const int* __restrict__ jcnt,
+                                                    const int* __restrict__ joff,
+                                                    const int* __restrict__ scratch,
+                                                    int* __restrict__ jlist) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  const int* src = scratch + soff[g];
+  int* dst = jlist + joff[g];
+  for (int k = lane; k < jcnt[g]; k += 64) dst[k] = src[k];
 }
 
 // ------------------------------------------------------------ Barnes-Hut
@@ -1057,13 +1097,22 @@ Lists* treepm_build(int n, const double* x, double rcut, int ncrit, hipStream_t 
   R->joff = R->mem.get<int>(ng + 1);
   double* cpos = B.tmp.get<double>(3 * (size_t)np);
   hipLaunchKernelGGL(k_cell_pos, grid_for(np), dim3(256), 0, st, np, cpart, R->pos, cpos);
-  hipLaunchKernelGGL((k_treepm_jlist<false>), grid_for(ng, 4), dim3(256), 0, st, ng, box, cpos,
-                     gr, cstart, cpart, rcut, R->jcnt, nullptr, nullptr);
+  // One walk: each group writes its list at its candidate offset (the
+  // candidate count bounds the list length), then the lists are packed.
+  int* ccand = B.tmp.get<int>(ng);
+  int* soff = B.tmp.get<int>(ng + 1);
+  hipLaunchKernelGGL(k_treepm_cand, grid_for(ng, 4), dim3(256), 0, st, ng, box, gr, cstart, rcut,
+                     ccand);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  const int ncand = B.scan(ccand, soff, ng);
+  int* scratch = B.tmp.get<int>(std::max(ncand, 1));
+  hipLaunchKernelGGL(k_treepm_jlist, grid_for(ng, 4), dim3(256), 0, st, ng, box, cpos,
+                     gr, cstart, cpart, rcut, R->jcnt, soff, scratch);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->nj = B.scan(R->jcnt, R->joff, ng);
   R->jlist = R->mem.get<int>(R->nj);
-  hipLaunchKernelGGL((k_treepm_jlist<true>), grid_for(ng, 4), dim3(256), 0, st, ng, box, cpos,
-                     gr, cstart, cpart, rcut, R->jcnt, R->joff, R->jlist);
+  hipLaunchKernelGGL(k_pack_lists, grid_for(ng, 4), dim3(256), 0, st, ng, soff, R->jcnt, R->joff,
+                     scratch, R->jlist);
   PGAMD_HIP_CHECK(hipGetLastError());
   R->coff = R->mem.get<int>(ng + 1);
   R->ccnt = R->mem.get<int>(ng);
