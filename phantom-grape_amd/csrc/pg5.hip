@@ -134,7 +134,8 @@ void jmem_reserve_host(JMem& m, int need, size_t rec_bytes, bool eps) {
 }
 
 // Upload the dirty range of a j-memory before a kernel reads [0, nread).
-void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
+void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, UploadBatch& ub,
+               hipStream_t st) {
   const size_t need = rec_bytes * (size_t)std::max(nread, m.hcap);
   if (need > m.d.bytes) {
     // A grown device buffer starts empty: upload everything written so far.
@@ -145,9 +146,9 @@ void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, hipStream_t st) {
   }
   if (m.lo < m.hi) {
     const size_t off = rec_bytes * (size_t)m.lo;
-    upload(static_cast<char*>(m.d.ptr) + off, m.h, off, rec_bytes * (size_t)(m.hi - m.lo), st);
+    ub.add(static_cast<char*>(m.d.ptr) + off, m.h, off, rec_bytes * (size_t)(m.hi - m.lo), st);
     if (eps)
-      upload(m.de.as<float>() + m.lo, m.he, sizeof(float) * (size_t)m.lo,
+      ub.add(m.de.as<float>() + m.lo, m.he, sizeof(float) * (size_t)m.lo,
              sizeof(float) * (size_t)(m.hi - m.lo), st);
     m.lo = m.hi = 0;
   }
@@ -273,7 +274,8 @@ void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4
       hin[i].w = eps2 ? (float)eps2[i] : e2all;
     }
   }
-  upload(b.d_in.ptr, b.h_in, 0, in_bytes, b.stream);
+  UploadBatch ub;
+  ub.add(b.d_in.ptr, b.h_in, 0, in_bytes, b.stream);
   // The kernels write the results straight into pinned host memory through
   // its device mapping; the synchronisation below makes them visible.
   const size_t out_bytes = sizeof(double) * 4 * (size_t)ni;
@@ -282,11 +284,13 @@ void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4
   PGAMD_HIP_CHECK(hipHostGetDevicePointer((void**)&dout, b.h_out.ptr, 0));
   Outputs5 o{dout, dout + 3 * (size_t)ni};
   if (cellmem < 0) {
-    jmem_sync(b.jm, sizeof(float4), kSym, nread, b.stream);
+    jmem_sync(b.jm, sizeof(float4), kSym, nread, ub, b.stream);
+    ub.flush(b.stream);
     run_force(b.d_part, ni, b.d_in.as<float4>(), nread, b.jm.d.as<float4>(),
               kSym ? b.jm.de.as<float>() : nullptr, nullptr, o, b.stream);
   } else {
-    jmem_sync(b.cm[cellmem], sizeof(Cell), false, nread, b.stream);
+    jmem_sync(b.cm[cellmem], sizeof(Cell), false, nread, ub, b.stream);
+    ub.flush(b.stream);
     run_force(b.d_part, ni, b.d_in.as<float4>(), nread, nullptr, nullptr,
               b.cm[cellmem].d.as<Cell>(), o, b.stream);
   }
