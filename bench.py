@@ -271,7 +271,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "pg6_hermite_plummer", "n": n, "eps": EPS,
                    "i_batch": n, "j": n, "block_levels": "2^-3..2^-12",
