@@ -15,7 +15,9 @@ extern "C" {
 
 /* Periodic unit box: ghosts within rcut of a face are appended; i-groups are
  * octree nodes of <= ncrit particles; j-list = particles within rcut of the
- * group's bounding box. */
+ * group's bounding box, in the order of a uniform grid of cells of side
+ * >= rcut over [-rcut, 1 + rcut) (cells by x, y, z index; a cell's particles
+ * by index). */
 void *pgt_treepm_build(int n, const double *pos, double rcut, int ncrit);
 /* Open domain Barnes-Hut lists with the opening test of source.cpp:581-707. */
 void *pgt_bh_build(int n, const double *pos, const double *mass, double theta, int ncrit,
