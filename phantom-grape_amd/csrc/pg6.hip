@@ -341,10 +341,10 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
   g_timer.after(st);
   if (p.nsplit > 1) {
     // one thread per i walks the splits; with many splits a wavefront per i
-    // (up to 512 i) or four threads per i (more i, coalesced loads): measured
-    // per g6calc call at N = 16384, the four-thread sum is 3-5 % faster from
-    // ni = 1024 on
-    if (p.nsplit > 16 && ni <= 512)
+    // (up to 1024 i: four-thread sums would leave most CUs idle) or four
+    // threads per i (more i, coalesced loads; 2-5 % faster calls at 2048 and
+    // 4096 i against N = 16384)
+    if (p.nsplit > 16 && ni <= 1024)
       hipLaunchKernelGGL((pg6::pg6_reduce_w<NN>), dim3(ceil_div(ni, 4)), dim3(256), 0, st, ni,
                          p.nsplit, part, jp, ipos, eps2, out);
     else if (p.nsplit > 8)
