@@ -1241,12 +1241,16 @@ PGAMD_API int pg5t_pm_force(int n, const double* pos, const double* mass, int me
     hipLaunchKernelGGL(pm::pm_keys, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, pos, mesh, kin,
                        vin);
     PGAMD_HIP_CHECK(hipGetLastError());
+    // the keys interleave three coordinates of ceil(log2 mesh) bits each
+    int b = 1;
+    while ((1 << b) < mesh) ++b;
+    const int end_bit = 3 * b;
     size_t tmp = 0;
     PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kin + n, vin, vin + n,
-                                                       n, 0, 30, st));
+                                                       n, 0, end_bit, st));
     c.sort_tmp.reserve(tmp);
     PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.ptr, tmp, kin, kin + n, vin,
-                                                       vin + n, n, 0, 30, st));
+                                                       vin + n, n, 0, end_bit, st));
     order = vin + n;
     hipLaunchKernelGGL(pm::pm_assign_sorted, dim3(ceil_div(n, pm::kAssignRun)),
                        dim3(pm::kAssignThreads), 0, st, n, order, pos, mass, mesh, rho);
