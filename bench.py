@@ -91,9 +91,6 @@ def main():
     ap.add_argument("--launch", default="",
                     help="fixed PG6 launch 'variant,bi,blocks_per_cu' (no autotuning), e.g. 1,256,16")
     ap.add_argument("--no-sweep", action="store_true", help="skip the i-batch size sweep")
-    ap.add_argument("--isa-rates", action="store_true",
-                    help="only measure the issue rate of v_fma_f32 and v_rsq_f32 on this device "
-                         "(pg6_dev_isa_rates), print one JSON line and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -119,11 +116,6 @@ def main():
 
     import ics
     import pgamd
-
-    if args.isa_rates:
-        r = pgamd.G6().isa_rates()
-        print(json.dumps({"isa_rates_ns_per_wave_instruction_per_simd": r}))
-        return
 
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}")

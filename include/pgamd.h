@@ -161,12 +161,6 @@ int pg6_dev_plan(int ni, int nj, int *out5);
  * number of buckets tuned, or -1.  No counterpart in the reference. */
 int pg6_dev_autotune(int nj, double eps2, int reps, int *out);
 int pg6_dev_autotune_reset(void);
-/* Issue rates of the pair kernels' building blocks on this device: ns_out[3]
- * = nanoseconds per wavefront instruction per SIMD for v_fma_f32,
- * v_rsq_f32 + v_add_f32 (per pair) and v_rsq_f32, eight wavefronts per
- * SIMD running eight independent chains for `iters` rounds.  Synchronous.
- * No counterpart in the reference: a measurement aid for the roofline. */
-int pg6_dev_isa_rates(int iters, double *ns_out);
 /* Block-time-step Hermite integration on the GPU: the loop of the reference's
  * sample integrator PG6/banana (banana01.c:57-166, hermite.c:39-208,
  * timestep.c:24-92) with its state in HBM.  It owns the GRAPE-6 board (the

@@ -13,8 +13,14 @@
 extern "C" {
 #endif
 
-/* Periodic unit box: ghosts within rcut of a face are appended; i-groups are
- * octree nodes of <= ncrit particles; j-list = particles within rcut of the
+/* Every particle belongs to exactly one i-group.  An i-group is an octree
+ * node of <= ncrit particles whose parent holds more, or a node the 21-level
+ * Morton key cannot split (more than ncrit particles inside one cell of side
+ * L / 2^21, e.g. coincident particles), which is one group of all of them.
+ * In a Barnes-Hut walk such a node, when opened, lists its particles as a
+ * leaf does, so every mass is reachable.
+ * Periodic unit box: ghosts within rcut of a face are appended; i-groups as
+ * above; j-list = particles within rcut of the
  * group's bounding box, in the order of a uniform grid of cells of side
  * >= rcut over [-rcut, 1 + rcut) (cells by x, y, z index; a cell's particles
  * by index). */

@@ -54,6 +54,7 @@ struct JMem {
   DevBuf d, de;
   int hcap = 0;   // records held by h
   int hmax = 0;   // one past the highest address ever written
+  int dvalid = 0; // the device copy holds [0, dvalid) (and any range uploaded since)
   int lo = 0, hi = 0;  // dirty range [lo, hi)
   void write_range(int a0, int a1) {
     hmax = std::max(hmax, a1);
@@ -124,32 +125,50 @@ Board* board(int devid, const char* fn) {
   return &b;
 }
 
+// Grows the host mirror to hold `need` records.  Records never written read
+// as zero (x = y = z = m = 0: no force), as in the reference's zero-initialised
+// static j-memory (PG5/newton/libpg5/phantom_g5.c:25-28).
 void jmem_reserve_host(JMem& m, int need, size_t rec_bytes, bool eps) {
   if (need <= m.hcap) return;
   int cap = m.hcap > 0 ? m.hcap : 4096;
   while (cap < need) cap *= 2;
   m.h.reserve(rec_bytes * (size_t)cap, true);
-  if (eps) m.he.reserve(sizeof(float) * (size_t)cap, true);
+  std::memset(static_cast<char*>(m.h.ptr) + rec_bytes * (size_t)m.hcap, 0,
+              rec_bytes * (size_t)(cap - m.hcap));
+  if (eps) {
+    m.he.reserve(sizeof(float) * (size_t)cap, true);
+    std::memset(m.he.as<float>() + m.hcap, 0, sizeof(float) * (size_t)(cap - m.hcap));
+  }
   m.hcap = cap;
 }
 
-// Upload the dirty range of a j-memory before a kernel reads [0, nread).
+// Uploads, before a kernel reads [0, nread), the dirty range of a j-memory and
+// every address below nread whose device copy was never written (a g5_set_n
+// beyond the highest address set reads zero records, not stale memory).
 void jmem_sync(JMem& m, size_t rec_bytes, bool eps, int nread, UploadBatch& ub,
                hipStream_t st) {
-  const size_t need = rec_bytes * (size_t)std::max(nread, m.hcap);
+  jmem_reserve_host(m, nread, rec_bytes, eps);
+  const size_t need = rec_bytes * (size_t)m.hcap;
   if (need > m.d.bytes) {
     // A grown device buffer starts empty: upload everything written so far.
     m.d.reserve(need);
-    if (eps) m.de.reserve(sizeof(float) * (size_t)std::max(nread, m.hcap));
+    if (eps) m.de.reserve(sizeof(float) * (size_t)m.hcap);
     m.lo = 0;
     m.hi = m.hmax;
+    m.dvalid = 0;
   }
-  if (m.lo < m.hi) {
-    const size_t off = rec_bytes * (size_t)m.lo;
-    ub.add(static_cast<char*>(m.d.ptr) + off, m.h, off, rec_bytes * (size_t)(m.hi - m.lo), st);
+  int lo = m.lo, hi = m.hi;
+  if (nread > m.dvalid) {
+    lo = lo < hi ? std::min(lo, m.dvalid) : m.dvalid;
+    hi = std::max(hi, nread);
+  }
+  if (lo < hi) {
+    const size_t off = rec_bytes * (size_t)lo;
+    ub.add(static_cast<char*>(m.d.ptr) + off, m.h, off, rec_bytes * (size_t)(hi - lo), st);
     if (eps)
-      ub.add(m.de.as<float>() + m.lo, m.he, sizeof(float) * (size_t)m.lo,
-             sizeof(float) * (size_t)(m.hi - m.lo), st);
+      ub.add(m.de.as<float>() + lo, m.he, sizeof(float) * (size_t)lo,
+             sizeof(float) * (size_t)(hi - lo), st);
+    if (lo <= m.dvalid) m.dvalid = std::max(m.dvalid, hi);
     m.lo = m.hi = 0;
   }
 }

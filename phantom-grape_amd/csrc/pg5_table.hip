@@ -65,6 +65,7 @@ struct Board {
   HostBuf hj;
   int hcap = 0;
   int hmax = 0;  // one past the highest address ever written
+  int dvalid = 0;  // the device copy holds [0, dvalid) (and any range uploaded since)
   int lo = 0, hi = 0;
   DevBuf dj;
   int n = 0;
@@ -276,6 +277,15 @@ inline float4 scale_i(const double* x, double xs) {
                      (float)((double)(float)x[2] * xs), 0.f);
 }
 
+void jmem_reserve(Board& b, int need) {
+  if (need <= b.hcap) return;
+  int cap = b.hcap > 0 ? b.hcap : 4096;
+  while (cap < need) cap *= 2;
+  b.hj.reserve(sizeof(float4) * (size_t)cap, true);
+  std::memset(b.hj.as<float4>() + b.hcap, 0, sizeof(float4) * (size_t)(cap - b.hcap));
+  b.hcap = cap;
+}
+
 void board_calc(Board& b, const double (*x)[3], const float4* pre, int ni, double (*a)[3],
                 double* p) {
   if (ni <= 0) return;
@@ -296,16 +306,26 @@ void board_calc(Board& b, const double (*x)[3], const float4* pre, int ni, doubl
   else
     for (int i = 0; i < ni; ++i) hin[i] = scale_i(x[i], g_tab.xscale);
   upload(b.d_in.ptr, b.h_in, 0, in_bytes, b.stream);
-  const size_t need = sizeof(float4) * (size_t)std::max(nj, b.hcap);
+  // Addresses below nj never written read as zero records (no force), as in
+  // the reference's zero-initialised static j-memory.
+  jmem_reserve(b, nj);
+  const size_t need = sizeof(float4) * (size_t)b.hcap;
   if (need > b.dj.bytes) {
     // A grown device buffer starts empty: upload everything written so far.
     b.dj.reserve(need);
     b.lo = 0;
     b.hi = b.hmax;
+    b.dvalid = 0;
   }
-  if (b.lo < b.hi) {
-    upload(b.dj.as<float4>() + b.lo, b.hj, sizeof(float4) * (size_t)b.lo,
-           sizeof(float4) * (size_t)(b.hi - b.lo), b.stream);
+  int lo = b.lo, hi = b.hi;
+  if (nj > b.dvalid) {
+    lo = lo < hi ? std::min(lo, b.dvalid) : b.dvalid;
+    hi = std::max(hi, nj);
+  }
+  if (lo < hi) {
+    upload(b.dj.as<float4>() + lo, b.hj, sizeof(float4) * (size_t)lo,
+           sizeof(float4) * (size_t)(hi - lo), b.stream);
+    if (lo <= b.dvalid) b.dvalid = std::max(b.dvalid, hi);
     b.lo = b.hi = 0;
   }
   // The table kernels write the results straight into pinned host memory
@@ -324,14 +344,6 @@ void board_calc(Board& b, const double (*x)[3], const float4* pre, int ni, doubl
                                    b.stream));
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   std::memcpy(a, b.h_out.ptr, out_bytes);
-}
-
-void jmem_reserve(Board& b, int need) {
-  if (need <= b.hcap) return;
-  int cap = b.hcap > 0 ? b.hcap : 4096;
-  while (cap < need) cap *= 2;
-  b.hj.reserve(sizeof(float4) * (size_t)cap, true);
-  b.hcap = cap;
 }
 
 void mark(Board& b, int a0, int a1) {

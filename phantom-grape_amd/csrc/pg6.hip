@@ -18,7 +18,6 @@
 #include "host_upload.h"
 #include "pg6_nblist.h"
 #include "cluster_batch.h"
-#include "isa_probe.h"
 #include "hermite_kernels.h"
 
 #include <algorithm>
@@ -89,6 +88,11 @@ struct Board {
   // only once it is done
   hipEvent_t pend_ev = nullptr;
   bool pend_busy = false;
+  // Cross-stream order of the board's work (board_use_stream): the stream of
+  // the last call that touched the j-memory or the partial-sum workspace, and
+  // an event that marks the end of that call's work.
+  hipStream_t last_st = nullptr;
+  hipEvent_t order_ev = nullptr;
   double ti = 0.0;
   int predict_flag = 0;
   int nj_pred = -1;  // j count of the last prediction
@@ -138,8 +142,26 @@ struct KernelTimer {
 };
 KernelTimer g_timer;
 
+// Every call that reads or writes the j-memory or the partial-sum workspace
+// runs on the device after the previous such call, whichever streams the two
+// were queued on: board_use_stream(st) before queueing work on st makes st
+// wait for the last call's work if that went to another stream, and
+// board_done_on(st) after it marks its end.  Work on the board's own stream
+// is marked lazily, when another stream comes next.
+void board_use_stream(Board& b, hipStream_t st) {
+  if (st == b.last_st) return;
+  if (b.last_st == b.stream) PGAMD_HIP_CHECK(hipEventRecord(b.order_ev, b.stream));
+  PGAMD_HIP_CHECK(hipStreamWaitEvent(st, b.order_ev, 0));
+  b.last_st = st;
+}
+
+void board_done_on(Board& b, hipStream_t st) {
+  if (st != b.stream) PGAMD_HIP_CHECK(hipEventRecord(b.order_ev, st));
+}
+
 void board_alloc_j(Board& b, int cap) {
   // Grow j-state and prediction arrays, preserving the j-state contents.
+  if (b.cap > 0) board_use_stream(b, b.stream);
   DevBuf njs, njp;
   const size_t jsb = (size_t)cap * (14 * sizeof(double) + sizeof(int));
   const size_t jpb = (size_t)cap * (3 * sizeof(double) + 4 * sizeof(float4) + sizeof(int)) +
@@ -219,6 +241,8 @@ void board_open(Board& b) {
     if (v > 0) b.npipes = v;
   }
   PGAMD_HIP_CHECK(hipEventCreateWithFlags(&b.pend_ev, hipEventDisableTiming));
+  PGAMD_HIP_CHECK(hipEventCreateWithFlags(&b.order_ev, hipEventDisableTiming));
+  b.last_st = b.stream;
   b.open = true;
   board_alloc_j(b, kInitialJCap);
   b.pend.reserve(sizeof(JRecord) * 1024);
@@ -230,8 +254,10 @@ void board_open(Board& b) {
 
 void board_close(Board& b) {
   if (!b.open) return;
+  board_use_stream(b, b.stream);
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   PGAMD_HIP_CHECK(hipEventDestroy(b.pend_ev));
+  PGAMD_HIP_CHECK(hipEventDestroy(b.order_ev));
   b.js_buf.release();
   b.jp_buf.release();
   b.d_in.release();
@@ -279,6 +305,7 @@ void board_queue_j(Board& b, int padr, int pidx, double t, double m, const doubl
 
 void board_flush_j(Board& b) {
   if (b.npend == 0) return;
+  board_use_stream(b, b.stream);
   // the scatter kernel reads the pinned queue through its device mapping
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
                      b.npend, mapped<JRecord>(b.pend), b.js);
@@ -294,9 +321,11 @@ void board_flush_j(Board& b) {
 
 void board_predict(Board& b, int nj, hipStream_t st) {
   if (nj <= 0) return;
+  board_use_stream(b, st);
   hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
                      b.js, b.jp);
   PGAMD_HIP_CHECK(hipGetLastError());
+  board_done_on(b, st);
   b.nj_pred = nj;
 }
 
@@ -363,6 +392,7 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
 void board_force(Board& b, int ni, int nj, const double* ipos, const double* ivel,
                  const int* iid, double eps2, const Outputs& out, hipStream_t st) {
   if (ni <= 0) return;
+  board_use_stream(b, st);
   const LaunchPlan p = plan_launch(b, ni, nj > 0 ? nj : 1);
   Partials part{};
   if (p.nsplit > 1) {
@@ -395,6 +425,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
     default: PG6_LAUNCH(64); break;
   }
 #undef PG6_LAUNCH
+  board_done_on(b, st);
 }
 
 // Host-API lasthalf: flags select nearest neighbour, rnnb and neighbour lists.
@@ -785,6 +816,7 @@ PGAMD_API int pg6_dev_force(int nj, int ni, const double* pos, const double* vel
 PGAMD_API int pg6_dev_sync(void) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!board_check_open("pg6_dev_sync")) return -1;
+  board_use_stream(g_board, g_board.stream);
   PGAMD_HIP_CHECK(hipStreamSynchronize(g_board.stream));
   return 0;
 }
@@ -838,6 +870,7 @@ PGAMD_API int pg6_dev_read_predicted(int nj, double* pos, float* vel, float* mas
   if (nj == 0) return 0;
   std::vector<double> x((size_t)nj), y((size_t)nj), z((size_t)nj);
   std::vector<float4> vm((size_t)nj);
+  board_use_stream(b, b.stream);
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   PGAMD_HIP_CHECK(hipMemcpy(x.data(), b.jp.x, sizeof(double) * nj, hipMemcpyDeviceToHost));
   PGAMD_HIP_CHECK(hipMemcpy(y.data(), b.jp.y, sizeof(double) * nj, hipMemcpyDeviceToHost));
@@ -1179,47 +1212,6 @@ PGAMD_API int pg6_dev_autotune_reset(void) {
   return 0;
 }
 
-// Issue rates of the pair kernels' building blocks on this device
-// (isa_probe.h): ns_out[k] = nanoseconds per wavefront instruction per SIMD
-// for k = 0: v_fma_f32, 1: v_rsq_f32 + v_add_f32 (per pair of
-// instructions), 2: v_rsq_f32; eight wavefronts per SIMD, `iters` rounds of
-// eight independent chains.  Synchronous.  A measurement aid for the force
-// kernels' roofline; no counterpart in the reference.
-PGAMD_API int pg6_dev_isa_rates(int iters, double* ns_out) {
-  std::lock_guard<std::mutex> lk(g_mutex);
-  if (iters <= 0 || !ns_out) {
-    api_error(3, "pg6_dev_isa_rates: iters=%d out=%p", iters, (void*)ns_out);
-    return -1;
-  }
-  ensure_device();
-  const int simds = 4 * device_cu_count();
-  const int blocks = 2 * simds;  // 256 threads = 4 wavefronts: 8 per SIMD
-  DevBuf out;
-  out.reserve(sizeof(float) * 256 * (size_t)blocks);
-  hipStream_t st = nullptr;
-  PGAMD_HIP_CHECK(hipStreamCreate(&st));
-  hipEvent_t e0, e1;
-  PGAMD_HIP_CHECK(hipEventCreate(&e0));
-  PGAMD_HIP_CHECK(hipEventCreate(&e1));
-  void (*k[3])(int, float, float*) = {pgprobe::chains<0>, pgprobe::chains<1>, pgprobe::chains<2>};
-  const double wave_ops = (double)blocks * 4 / simds * iters * pgprobe::kChains;
-  for (int m = 0; m < 3; ++m) {
-    hipLaunchKernelGGL(k[m], dim3(blocks), dim3(256), 0, st, iters, 1.f, out.as<float>());
-    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k[m], dim3(blocks), dim3(256), 0, st, iters, 1.f, out.as<float>());
-    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
-    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    PGAMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    ns_out[m] = 1e6 * ms / wave_ops;
-  }
-  PGAMD_HIP_CHECK(hipEventDestroy(e0));
-  PGAMD_HIP_CHECK(hipEventDestroy(e1));
-  PGAMD_HIP_CHECK(hipStreamDestroy(st));
-  out.release();
-  return 0;
-}
-
 // ======================================================================
 // Block-time-step Hermite integration on the GPU (the reference's sample
 // integrator PG6/banana, banana01.c:57-166, on the device).  It owns the
@@ -1272,6 +1264,7 @@ void hermite_force(Board& b, Hermite& h, int ni) {
 // Queues one block step (three kernels, no host synchronisation).
 void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
   const int n = h.n;
+  board_use_stream(b, st);
   hipLaunchKernelGGL(pg6h::pg6h_select, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
                      h.ctl, b.js, b.jp, h.act, h.ipos, h.ivel, h.iid);
   hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(64 * pg6h::kForceWaves), 0, st,
@@ -1292,6 +1285,7 @@ void hermite_enqueue_batch(Board& b, Hermite& h, hipStream_t st) {
     PGAMD_HIP_CHECK(hipGraphExecDestroy(h.graph));
     h.graph = nullptr;
   }
+  board_use_stream(b, st);  // before the capture: inside it the stream is already current
   if (!h.graph) {
     hipGraph_t g = nullptr;
     PGAMD_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));

@@ -13,12 +13,19 @@ ErrorState& error_state() {
   return st;
 }
 
+// Every read or write of the shared error state holds this lock: the MC
+// entry points are called concurrently from one thread per devid.
+static std::mutex g_err_mutex;
+
 bool api_error(int code, const char* fmt, ...) {
   ErrorState& st = error_state();
+  char msg[sizeof(st.msg)];
   va_list ap;
   va_start(ap, fmt);
-  std::vsnprintf(st.msg, sizeof(st.msg), fmt, ap);
+  std::vsnprintf(msg, sizeof(msg), fmt, ap);
   va_end(ap);
+  std::lock_guard<std::mutex> lk(g_err_mutex);
+  std::memcpy(st.msg, msg, sizeof(msg));
   st.code = code;
   if (st.abort_on_error) {
     std::fprintf(stderr, "pgamd: %s\n", st.msg);
@@ -57,10 +64,12 @@ int device_cu_count() {
 }  // namespace pgamd
 
 PGAMD_API void pgamd_set_abort_on_error(int flag) {
+  std::lock_guard<std::mutex> lk(pgamd::g_err_mutex);
   pgamd::error_state().abort_on_error = flag;
 }
 
 PGAMD_API int pgamd_last_error(char* buf, int buflen) {
+  std::lock_guard<std::mutex> lk(pgamd::g_err_mutex);
   pgamd::ErrorState& st = pgamd::error_state();
   if (buf && buflen > 0) {
     std::strncpy(buf, st.msg, (size_t)buflen - 1);

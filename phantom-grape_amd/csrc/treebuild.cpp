@@ -60,6 +60,12 @@ struct Node {
   int cell = -1;     // index in the cell array (internal nodes)
 };
 
+inline bool is_leaf(const Node& nd) {
+  for (int o = 0; o < 8; ++o)
+    if (nd.child[o] >= 0) return false;
+  return true;
+}
+
 struct Octree {
   std::vector<int> perm;  // Morton order -> input index
   std::vector<uint64_t> key;
@@ -196,13 +202,15 @@ void build_octree(int n, const double* x, const double* m, const double lo[3], d
     q[5] = -(q[0] + q[3]);
     std::memcpy(nd.q, q, sizeof(q));
   }
-  // i-groups: nodes with <= ncrit particles whose parent has more
+  // i-groups: nodes with <= ncrit particles whose parent has more, and nodes
+  // the 21-level key cannot split (more than ncrit particles inside one cell
+  // of side L / 2^21), which form one group of all their particles
   std::vector<int> st{0};
   while (!st.empty()) {
     const int k = st.back();
     st.pop_back();
     const Node& nd = t.nodes[k];
-    if (nd.hi - nd.lo <= ncrit) {
+    if (nd.hi - nd.lo <= ncrit || is_leaf(nd)) {
       t.groups.push_back(k);
       continue;
     }
@@ -392,7 +400,7 @@ PGT_API void* pgt_bh_build(int n, const double* x, const double* m, double theta
     while (!st.empty()) {
       const Node& nd = t.nodes[st.back()];
       st.pop_back();
-      if (nd.hi - nd.lo <= nleaf) {
+      if (nd.hi - nd.lo <= nleaf || is_leaf(nd)) {  // a node the key cannot split lists its particles
         for (int p = nd.lo; p < nd.hi; ++p) pl[g].push_back(p);
         continue;
       }

@@ -434,13 +434,16 @@ __global__ void k_moments(int a, int b, Nodes nd, const double* __restrict__ pos
   for (int t = 0; t < 6; ++t) nd.q[6 * k + t] = q[t];
 }
 
-// i-groups: nodes of <= ncrit particles whose parent holds more.
+// i-groups: nodes of <= ncrit particles whose parent holds more, and nodes
+// the key cannot split (level kLevels: more than ncrit particles inside one
+// cell of side L / 2^21), which form one group of all their particles.
 __global__ void k_group_flag(int nn, Nodes nd, int ncrit, int* __restrict__ flag) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nn) return;
   const int cnt = nd.hi[k] - nd.lo[k];
   const int par = nd.par[k];
-  flag[k] = cnt <= ncrit && (par < 0 || nd.hi[par] - nd.lo[par] > ncrit);
+  flag[k] = (cnt <= ncrit || nd.lev[k] >= kLevels) &&
+            (par < 0 || nd.hi[par] - nd.lo[par] > ncrit);
 }
 
 __global__ void k_compact(int n, const int* __restrict__ flag, const int* __restrict__ off,
@@ -792,8 +795,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_bh_walk(
     nc = 1;
   } else {
     if (lane == 0)
-      stk[0] = wn[0].hi - wn[0].lo <= nleaf ? make_int3(wn[0].lo, wn[0].hi, -1)
-                                            : make_int3(wn[0].first, wn[0].mask, 0);
+      stk[0] = wn[0].hi - wn[0].lo <= nleaf || wn[0].mask == 0
+                   ? make_int3(wn[0].lo, wn[0].hi, -1)
+                   : make_int3(wn[0].first, wn[0].mask, 0);
     int sp = 1;
     while (sp > 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -853,8 +857,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_bh_walk(
       const u64 mpush = ms & (0xffULL << (8 * (ncommit - 1)));  // the last committed entry
       sp -= ncommit;
       if (kind == 3 && commit)
-        stk[sp + __popcll(mpush & above)] =
-            hi - lo <= nleaf ? make_int3(lo, hi, -1) : make_int3(first, mask, ent.z + 1);
+        stk[sp + __popcll(mpush & above)] = hi - lo <= nleaf || mask == 0
+                                                ? make_int3(lo, hi, -1)
+                                                : make_int3(first, mask, ent.z + 1);
       np += __popcll(mp) + leaftot;
       nc += __popcll(mc);
       sp += __popcll(mpush);

@@ -241,7 +241,6 @@ class G6:
         _sig(L, "pg6_dev_set_kernel", C.c_int, C.c_int)
         _sig(L, "pg6_dev_autotune", C.c_int, C.c_int, C.c_double, C.c_int, _ip)
         _sig(L, "pg6_dev_autotune_reset", C.c_int)
-        _sig(L, "pg6_dev_isa_rates", C.c_int, C.c_int, _dp)
         _sig(L, "pg6_hermite_init", C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
              C.c_double, C.c_double)
         _sig(L, "pg6_hermite_step", C.c_int, _dp, _ip)
@@ -366,13 +365,6 @@ class G6:
         ek, ep = np.zeros(1), np.zeros(1)
         assert self.lib.pg6_hermite_energy(dptr(ek), dptr(ep)) == 0
         return float(ek[0]), float(ep[0])
-
-    def isa_rates(self, iters=65536):
-        """pg6_dev_isa_rates: ns per wavefront instruction per SIMD."""
-        o = np.zeros(3)
-        assert self.lib.pg6_dev_isa_rates(int(iters), dptr(o)) == 0
-        return {"v_fma_f32": float(o[0]), "v_rsq_f32+v_add_f32": float(o[1]),
-                "v_rsq_f32": float(o[2])}
 
     def autotune(self, nj, eps2, reps=3):
         """Runs pg6_dev_autotune; returns {ni: (variant, bi, bpc, us)}."""

@@ -195,3 +195,21 @@ def test_quadrupole_near_cells(g5):
     assert np.percentile(rel_err(a, oa), 99) < 1e-5
     # phi_p and phi_q cancel partly for near cells: FP32 sums, 5e-5 relative
     np.testing.assert_allclose(-p, op, rtol=5e-5)
+
+
+def test_n_beyond_the_addresses_written(g5):
+    """g5_set_n larger than the highest address set reads zero records, as the
+    reference's zero-initialised j-memory does (PG5/newton/libpg5/
+    phantom_g5.c:25-28): the force equals the one over the addresses set."""
+    m, x, _ = ics.plummer(1000, 25)
+    g5.set_eps_to_all(0.01)
+    dev = 30  # a board no other test has written
+    g5.set_xmj(0, x, m, devid=dev)
+    g5.set_n(1000, devid=dev)
+    a0, p0 = g5.calculate_force_on_x(x[:64], devid=dev)
+    for n in (3000, 9000):  # within the first device buffer, then past it
+        g5.set_n(n, devid=dev)
+        a, p = g5.calculate_force_on_x(x[:64], devid=dev)
+        assert np.all(np.isfinite(a)) and np.all(np.isfinite(p))
+        assert rel_err(a, a0).max() < 1e-6
+        assert rel_err(p, p0).max() < 1e-6

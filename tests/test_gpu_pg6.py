@@ -265,10 +265,44 @@ def test_empty_and_error_paths(g6):
         g6.lib.pgamd_set_abort_on_error(1)
 
 
-def test_isa_rate_probe(g6):
-    """The issue-rate probe runs and orders its operations as the hardware
-    does: a transcendental costs more than an FMA, rsq + add more than rsq."""
-    r = g6.isa_rates(4096)
-    assert all(v > 0 for v in r.values())
-    assert r["v_rsq_f32"] > r["v_fma_f32"]
-    assert r["v_rsq_f32+v_add_f32"] > r["v_rsq_f32"]
+def test_device_calls_on_another_stream_are_ordered(g6):
+    """pg6_dev_set_j queues its scatter on the library's stream; predict and
+    force then go to a torch stream with no synchronisation in between, and a
+    second j-set follows at once.  Each force must see its own j-set: the
+    board orders its work across streams (scatter before predict, predict and
+    force before the next scatter)."""
+    import ctypes as C
+    import torch
+
+    L = g6.lib
+    n, eps, ti = 4096, 1.0 / 256, 1.0 / 512
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    cases, outs = [], []
+    for seed in (31, 32):
+        m, x, v, a2, j6, tj, idx = plummer_state(n, seed, eps)
+        xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+        isel = np.arange(0, n, 7)
+        d = {k: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+             for k, a in (("x", xo[isel]), ("v", vo[isel]), ("id", idx[isel]))}
+        o = {"acc": torch.zeros((len(isel), 3), dtype=torch.float64, device="cuda"),
+             "jrk": torch.zeros((len(isel), 3), dtype=torch.float64, device="cuda"),
+             "pot": torch.zeros(len(isel), dtype=torch.float64, device="cuda")}
+        cases.append((m, x, v, a2, j6, tj, idx, xo, vo, isel))
+        outs.append((d, o))
+    torch.cuda.synchronize()
+    for (m, x, v, a2, j6, tj, idx, xo, vo, isel), (d, o) in zip(cases, outs):
+        assert g6.set_j_bulk(x, v, a2, j6, m, tj, idx) == 0
+        assert L.pg6_dev_predict(n, ti, sp) == 0
+        assert L.pg6_dev_force(n, len(isel), d["x"].data_ptr(), d["v"].data_ptr(),
+                               d["id"].data_ptr(), eps * eps, o["acc"].data_ptr(),
+                               o["jrk"].data_ptr(), o["pot"].data_ptr(), None, None, sp) == 0
+    s.synchronize()
+    for (m, x, v, a2, j6, tj, idx, xo, vo, isel), (d, o) in zip(cases, outs):
+        ref = O.pg6_force(xo[isel], vo[isel], idx[isel], eps * eps, xo,
+                          vo.astype(np.float32).astype(np.float64), m, idx)
+        assert np.percentile(rel_err(o["acc"].cpu().numpy(), ref["acc"]), 99) < 1e-6
+        assert np.percentile(rel_err(o["jrk"].cpu().numpy(), ref["jrk"]), 99) < 1e-5
+        assert rel_err(o["pot"].cpu().numpy(), ref["pot"]).max() < 1e-6
+    # the library's own synchronisation waits for the torch stream's work too
+    assert L.pg6_dev_sync() == 0

@@ -131,3 +131,20 @@ def test_set_xj_and_set_mj(tl):
     tl.g5_calculate_force_on_x(pgamd.dptr(xi), pgamd.dptr(ai), pgamd.dptr(p), 50)
     scale = np.linalg.norm(a0, axis=1).max()
     assert np.percentile(rel_err(ai, 2 * a0, floor=1e-3 * scale), 99) < 1e-4
+
+
+def test_n_beyond_the_addresses_written(tl):
+    """g5_set_n past the addresses set adds zero records (no force)."""
+    import pgamd
+    tl.pg5_gen_s2_force_table(PP, PM)
+    m, x = ics.clumpy_box(1000, 33)
+    a0 = force(tl, x[:50], x, m)
+    scale = np.linalg.norm(a0, axis=1).max()
+    xi = pgamd.f64(x[:50])
+    for n in (3000, 20000):
+        tl.g5_set_n(n)
+        a = np.zeros((50, 3))
+        p = np.zeros(50)
+        tl.g5_calculate_force_on_x(pgamd.dptr(xi), pgamd.dptr(a), pgamd.dptr(p), 50)
+        assert np.all(np.isfinite(a))
+        assert rel_err(a, a0, floor=1e-3 * scale).max() < 1e-6

@@ -58,7 +58,7 @@ def test_bh_lists_match_host_builder(n, ncrit, nleaf):
 
 def test_coincident_particles_behave_as_host_builder():
     """More than ncrit particles at one point cannot be split by the 21-level
-    key: both builders leave that node out of every group, identically."""
+    key: both builders make that node one group of all its particles."""
     import pgamd
     rng = np.random.default_rng(5)
     x = np.vstack([rng.uniform(0.1, 0.9, (2000, 3)), np.full((300, 3), 0.5)])
@@ -68,15 +68,15 @@ def test_coincident_particles_behave_as_host_builder():
     d = L.host()
     L.free()
     same_lists(h, d, cells=False)
-    assert d["icnt"].sum() == 2000
+    assert d["icnt"].sum() == 2300 and d["icnt"].max() == 300
 
 
 @pytest.mark.parametrize("case", ["coincident", "n2", "n9"])
 def test_bh_edge_cases_behave_as_host_builder(case):
     """Barnes-Hut walk on a node that the key cannot split (more than nleaf
-    coincident particles: a walk that opens it finds no children and lists
-    nothing, in both builders) and on tiny systems (the root a leaf, or one
-    level of children)."""
+    coincident particles: a walk that opens it lists its particles as a leaf,
+    in both builders) and on tiny systems (the root a leaf, or one level of
+    children)."""
     import pgamd
     rng = np.random.default_rng(6)
     if case == "coincident":

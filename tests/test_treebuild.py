@@ -61,3 +61,28 @@ def test_bh_cells_hold_the_moments_of_their_particles():
     np.testing.assert_allclose(root[:3], com, rtol=1e-11, atol=1e-13)
     np.testing.assert_allclose(root[4:9], q, rtol=1e-9, atol=1e-12 * np.abs(q).max())
     np.testing.assert_allclose(root[9], -(q[0] + q[3]), rtol=1e-9, atol=1e-12 * np.abs(q).max())
+
+
+def test_unsplittable_nodes_keep_every_particle():
+    """More than ncrit (TreePM) or nleaf (Barnes-Hut) coincident particles
+    cannot be split by the 21-level key.  They form one i-group, and a
+    Barnes-Hut walk that opens their node lists them as a leaf: every
+    particle has a group and every group reaches the whole mass exactly once
+    (listed particles + accepted cells)."""
+    rng = np.random.default_rng(5)
+    x = np.vstack([rng.uniform(0.1, 0.9, (2000, 3)), np.full((300, 3), 0.5)])
+    L = pgamd.treepm_lists(x, 0.05, 128)
+    np.testing.assert_array_equal(np.sort(L["ilist"]), np.arange(len(x)))
+    assert L["icnt"].max() == 300  # the coincident particles: one group
+    x = np.vstack([rng.uniform(-1, 1, (2000, 3)), np.full((40, 3), 0.25)])
+    m = rng.uniform(0.5, 1.5, len(x)) / len(x)
+    L = pgamd.bh_lists(x, m, 0.5, 64, 4)
+    n = len(x)
+    np.testing.assert_array_equal(np.sort(L["ilist"]), np.arange(n))
+    assert L["icnt"].max() <= 64  # 40 coincident < ncrit: a group holds them with others
+    mp = m[L["src"]]  # masses in the builder's particle order
+    for g in range(len(L["ioff"])):
+        pj = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        cj = L["clist"][L["coff"][g]:L["coff"][g] + L["ccnt"][g]]
+        assert len(np.unique(pj)) == len(pj)
+        np.testing.assert_allclose(mp[pj].sum() + L["cells"][cj, 3].sum(), m.sum(), rtol=1e-12)
