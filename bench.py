@@ -45,15 +45,33 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_model():
+def cpu_info():
+    """Host CPU as the CPU baselines see it: model name (lscpu, else
+    /proc/cpuinfo), the host's physical cores (sockets x cores per socket),
+    and the logical CPUs this process may run on (its affinity mask, the
+    box's share of the host)."""
+    info = {"model": platform.processor() or "unknown", "host_physical_cores": None,
+            "process_cpus": len(os.sched_getaffinity(0))}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+        f = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip()
+             for l in out.splitlines() if ":" in l}
+        if f.get("Model name"):
+            info["model"] = f["Model name"]
+        if f.get("Socket(s)", "").isdigit() and f.get("Core(s) per socket", "").isdigit():
+            info["host_physical_cores"] = int(f["Socket(s)"]) * int(f["Core(s) per socket"])
     except OSError:
         pass
-    return platform.processor()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                if len(name) > len(info["model"]):
+                    info["model"] = name
+                break
+    except OSError:
+        pass
+    return info
 
 
 def block_steps(acc, jrk, eta=0.02):
@@ -66,15 +84,24 @@ def block_steps(acc, jrk, eta=0.02):
     return 2.0 ** (-lev)
 
 
-def read_traffic():
-    """HBM bytes per pg6_force launch from the committed rocprofv3 PMC summary."""
+def read_traffic(plan):
+    """HBM bytes per pg6_force launch from the committed rocprofv3 PMC summary,
+    if it was measured on the launch shape this run timed (kernel variant, i
+    per block, j-range count); else None.  Returns (bytes, note)."""
     p = os.path.join(ROOT, "profiles", "pg6_force_n65536_traffic.txt")
     if not os.path.exists(p):
-        return None
+        return None, "no traffic profile"
+    val, shape = None, None
     for line in open(p):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+            val = float(line.split(":", 1)[1])
+        elif line.startswith("launch_shape:"):
+            shape = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+    here = {"variant": str(plan.get("variant")), "bi": str(plan.get("bi")),
+            "nsplit": str(plan.get("nsplit"))}
+    if val is None or shape != here:
+        return None, f"profiled launch {shape} differs from the timed one {here}"
+    return val, f"rocprofv3 FETCH_SIZE + WRITE_SIZE of the timed launch shape {here}"
 
 
 def main():
@@ -280,7 +307,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "flops_per_interaction": FLOPS_PER_INTERACTION,
                      "interactions_per_launch": inter, "kernel_ms": round(k_ms, 5),
-                     "traffic": read_traffic()},
+                     "traffic": read_traffic(plan)[0], "traffic_note": read_traffic(plan)[1]},
         "sweep": sweep,
     }
 
@@ -328,7 +355,7 @@ def cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk, gpu_pot,
     if not O.ref_available("pg6"):
         out["cpu_baseline"] = None
         return out
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = _threads()
     R = O.RefPG6()
     R.set_j(x, v, acc2, jrk6, m, tj, idx)
     times = []
@@ -347,7 +374,7 @@ def cpu_leg(g6, x, v, xi, vi, m, idx, acc2, jrk6, tj, gpu_acc, gpu_jrk, gpu_pot,
         "sample": f"the whole step, {n} i x {n} j: g6_set_ti + g6calc_firsthalf + "
                   f"g6calc_lasthalf2p in chunks of 48 i (PG6/libpg6 AVX2+OpenMP), median of 5 "
                   f"after 1 warm-up, {tmed:.2f} s per step",
-        "cpu": cpu_model(), "gpu_over_cpu": round(gpu_value / cpu_value, 1),
+        "cpu": cpu_info(), "gpu_over_cpu": round(gpu_value / cpu_value, 1),
         "reference_accuracy": {"acc": stats(r["acc"][sel], o["acc"]),
                                "jerk": stats(r["jrk"][sel], o["jrk"])},
     }
@@ -377,7 +404,9 @@ def _time(fn, stream, reps):
 
 
 def _threads():
-    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+    """OpenMP threads of the CPU baselines: OMP_NUM_THREADS (the box sets it to
+    its CPU share), else the CPUs this process may use."""
+    return int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
 
 
 def _oracle():
@@ -429,15 +458,17 @@ def bench_c1(args, stream, cpu):
     if cpu and O.ref_available("pg5_omp"):
         R = O.RefPG5(omp=True)
         ts = []
-        for _ in range(4):
+        for _ in range(6):
             t0 = time.perf_counter()
-            ra, _ = R.force(x[sel], x, m, eps)
+            ra, _ = R.force(x, x, m, eps)
             ts.append(time.perf_counter() - t0)
         t = float(np.median(ts[1:]))
-        out["cpu_baseline"] = {"ginteractions_s": round(len(sel) * n / t / 1e9, 3),
-                               "kind": "reference", "sample": f"{len(sel)} i x {n} j",
-                               "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
-                               "accuracy_vs_fp64": _errs(ra, oa)}
+        out["cpu_baseline"] = {"ginteractions_s": round(n * n / t / 1e9, 3),
+                               "kind": "reference",
+                               "sample": f"the whole N x N = {n} x {n} g5_calculate_force_on_x "
+                                         "(OpenMP build), median of 5 after 1 warm-up",
+                               "seconds": round(t, 4), "threads": _threads(),
+                               "accuracy_vs_fp64": _errs(ra[sel], oa)}
     if cpu:
         out["dropin_direct"] = _direct_dropin()
     return out
@@ -472,37 +503,71 @@ def bench_c3(args, stream, cpu):
     L = DL.host()
     assert all(np.array_equal(L[k], Lh[k]) for k in ("pos", "src", "ioff", "jlist", "order"))
     T = pgamd.G5Table()
-    T.lib.pg5_set_table_frc_bit(6)  # 1024 entries
     T.lib.pg5_gen_s2_force_table(pp, pm)
-    _, xs = T.table()
     t0 = time.time()
-    loss = T.mlp_train(steps=6000, batch=4096, lr=3e-3, seed=7)
+    loss = T.mlp_train(iters=200, starts=4, seed=7)
     ttrain = time.time() - t0
     P = L["pos"]
     mm = m[L["src"]]
     d_pos = torch.empty((len(P), 4), dtype=torch.float32, device="cuda")
-    DL.pack(d_m.data_ptr(), xs, d_pos.data_ptr(), None, sp)
-    torch.cuda.synchronize()
     d = {k: DL.ptr(k) for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
-    acc = {k: torch.zeros((len(P), 3), dtype=torch.float64, device="cuda") for k in (0, 1)}
+    d_pos64 = DL.ptr("pos")  # the list builder's FP64 positions (ghosts included)
+    acc = torch.zeros((len(P), 3), dtype=torch.float64, device="cuda")
     ng = len(L["ioff"])
     res = {"workload": "treepm_clumpy_box_n2^20_mesh128", "groups": ng,
            "interactions": L["work_pp"], "list_build_host_s": round(tb, 3),
            "list_build_gpu_s": round(float(np.median(tgpu[1:])), 4),
-           "mlp_train_s": round(ttrain, 2), "mlp_train_loss": loss}
-    sp = stream.cuda_stream
-    for mode, name in ((0, "table1024"), (1, "mlp")):
+           "mlp_train": {"s": round(ttrain, 2), "loss": loss,
+                         "method": "Levenberg-Marquardt, FP64 GPU kernels, 4 starts x 200"}}
+    O = _oracle()
+    sel = np.sort(np.random.default_rng(4).choice(len(x), 512, replace=False))
+    f = O.table_force_f64(P[sel], P, mm, pp, pm)
+    tab5, _, xs5 = O.table_build(pp, pm, 4, 5)
+    # the reference's own output on these particles: its 512-entry table on
+    # FP32 positions (the oracle's bit-exact emulation of gravity_kernel_avx2.c)
+    fref = O.table_force_emul(P[sel], P, mm, tab5, xs5, 4, 5)
+    res["reference_table_vs_closed_form_s2"] = _errs(fref, f)
+    # (name, cut-off mode, table frc_bit, table order, FP64 group origins)
+    runs = [("table1024", 0, 6, 1, False), ("table1024_origin", 0, 6, 1, True),
+            ("table1024_quadratic_origin", 0, 6, 2, True),
+            ("table8192_origin", 0, 9, 1, True), ("mlp_origin", 1, 6, 1, True),
+            ("mlp", 1, 6, 1, False)]
+    for name, mode, frc, order, p64 in runs:
+        T.lib.pg5_set_table_order(1)
+        T.lib.pg5_set_table_frc_bit(frc)
+        T.lib.pg5_set_table_order(order)
+        _, xs = T.table()
+        DL.pack(d_m.data_ptr(), xs, d_pos.data_ptr(), None, sp)
+        torch.cuda.synchronize()
         args_ = (0, ng, d["ioff"], d["icnt"], d["ilist"], d["joff"], d["jcnt"], d["jlist"],
-                 d["order"], d_pos.data_ptr(), acc[mode].data_ptr(), mode)
+                 d["order"], d_pos.data_ptr(), d_pos64 if p64 else None, acc.data_ptr(), mode)
         best = np.zeros(1, np.int32)
         T.lib.pg5t_batch_autotune(*args_, 2, pgamd.iptr(best), None)
         h = T.lib.pg5t_batch_graph(*args_)
         ms = _time(lambda: T.lib.pg5t_batch_graph_launch(h, sp), stream, 5)
         T.lib.pg5t_batch_graph_destroy(h)
+        a = acc.cpu().numpy()[sel]
         res[name] = {"ms": round(ms, 3), "ginteractions_s": round(L["work_pp"] / ms / 1e6, 1),
                      "pct_fp32_peak": round(100 * 38 * L["work_pp"] / (ms * 1e-3) / 1e12
                                             / FP32_PEAK_TFLOPS, 2),
-                     "blocks_per_cu": int(best[0])}
+                     "blocks_per_cu": int(best[0]), "table": f"frc_bit {frc}, order {order}",
+                     "positions": "FP64 group origins" if p64 else "FP32 scaled",
+                     "accuracy_vs_closed_form_s2": _errs(a, f),
+                     "accuracy_vs_reference_table": _errs(a, fref)}
+        log(f"c3 {name}: {ms:.3f} ms, p99 {res[name]['accuracy_vs_closed_form_s2']['p99']}")
+    T.lib.pg5_set_table_order(1)
+    T.lib.pg5_set_table_frc_bit(6)
+    _, xs = T.table()
+    # the learned path against the table at equal accuracy: the cheapest table
+    # configuration whose p99 is at most the network's
+    net = res["mlp_origin"]
+    eq = [k for k in ("table1024_quadratic_origin", "table8192_origin", "table1024_origin")
+          if res[k]["accuracy_vs_closed_form_s2"]["p99"] <= net["accuracy_vs_closed_form_s2"]["p99"]]
+    best_tab = min(eq, key=lambda k: res[k]["ms"]) if eq else None
+    res["learned_vs_table_at_equal_error"] = {
+        "table": best_tab, "mlp_p99": net["accuracy_vs_closed_form_s2"]["p99"],
+        "table_p99": res[best_tab]["accuracy_vs_closed_form_s2"]["p99"] if best_tab else None,
+        "mlp_over_table_speed": round(res[best_tab]["ms"] / net["ms"], 4) if best_tab else None}
     res["mlp_over_table_speed"] = round(res["table1024"]["ms"] / res["mlp"]["ms"], 3)
     # long-range side on the 128^3 mesh, and the whole TreePM force step
     acc_pm = torch.zeros((len(x), 3), dtype=torch.float64, device="cuda")
@@ -514,12 +579,6 @@ def bench_c3(args, stream, cpu):
                              "long_range_pm": round(pm_ms, 3),
                              "total": round(1e3 * res["list_build_gpu_s"]
                                             + res["table1024"]["ms"] + pm_ms, 2)}
-    O = _oracle()
-    sel = np.sort(np.random.default_rng(4).choice(len(x), 512, replace=False))
-    f = O.table_force_f64(P[sel], P, mm, pp, pm)
-    for mode, name in ((0, "table1024"), (1, "mlp")):
-        a = acc[mode].cpu().numpy()[sel]
-        res[name]["accuracy_vs_closed_form_s2"] = _errs(a, f)
     T.lib.pg5_set_table_frc_bit(5)
     DL.free()
     if cpu and O.ref_available("pg5table"):
@@ -664,12 +723,11 @@ def bench_c5(args, stream, cpu):
                                "jerk": _errs(np.concatenate(ej), np.concatenate(oj)),
                                "sample": "24 whole clusters"}
     if cpu and O.ref_available("pg6"):
-        sel = np.sort(np.random.default_rng(7).choice(len(sizes), 512, replace=False))
+        sel = np.arange(len(sizes))
         t, *_ = O.ref_clusters(sel, coff, X, V, Mm, (4.0 / sizes) ** 2)
-        w = float((sizes[sel] * (sizes[sel] - 1)).sum())
         res["cpu_baseline"] = {
-            "ginteractions_s": round(w / t / 1e9, 3), "kind": "reference",
-            "sample": "512 random clusters, one PG6 session each (set_j_particle per member, "
+            "ginteractions_s": round(inter / t / 1e9, 3), "kind": "reference",
+            "sample": "all 4096 clusters, one PG6 session each (set_j_particle per member, "
                       "firsthalf + lasthalf2 in chunks of 48)",
             "threads": _threads(), "seconds": round(t, 3)}
     return res

@@ -36,6 +36,14 @@ int pg5_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
 int pg5_dev_quad(int devid, int ni, const void *xi, int nc, const void *cells,
                  double *acc, double *pot, void *stream);
 void *pg5_dev_stream(int devid);
+/* rsqrt-bias correction of the g5_* outputs, the reference's rule
+ * (PG5/newton/libpg5/rsqrt.c:42-81 measures the bias b of the rsqrt the
+ * kernel uses; phantom_g5.c:45-60,244-269 scales acc by float(1 - 3b) and pot
+ * by float(1 - b), the cell calls included): mode 0 (default) none, 1 with b
+ * measured by that rule for the device's v_rsq_f32 (once per process). */
+int pg5_set_rsqrt_bias_mode(int mode);
+/* b of the device's rsqrt by that rule (measured on the first call). */
+double pg5_rsqrt_bias(void);
 /* bi in {0 = automatic, 64, 128, 256} i-particles per block; blocks_per_cu and
  * min_jper (> 0) set the j-range split. */
 int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
@@ -72,6 +80,12 @@ int pg5_bh_autotune(int devid, int ngroups, const int *ioff, const int *icnt, co
 /* Table resolution 2^frc_bit entries per octave of s (reference FRC_BIT = 5,
  * compile-time, PG5/table/libpg5/pg5_table.h:7-15); rebuilds the table. */
 int pg5_set_table_frc_bit(int frc_bit);
+/* Interpolation order of the table in the batched calls (pg5t_batch_*):
+ * 1 = the reference's value + slope per float-bit tick (default), 2 = a
+ * quadratic through each interval's ends and midpoint from the shape in FP64
+ * (16 bytes per entry; frc_bit <= 8).  The drop-in calls keep the
+ * reference's table. */
+int pg5_set_table_order(int order);
 /* Copies the current table (2*ntab floats) if out != NULL; returns ntab. */
 int pg5_get_table(float *out, double *xscale, float *r2scale);
 /* Work of g5_calculate_force_on_xMC (PG5/table/libpg5/phantom_g5.c:160) on
@@ -85,33 +99,51 @@ int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 void *pg5t_dev_stream(int devid);
 /* Learned cut-off: a 1-16-16-1 tanh FP32 network, trained on the GPU on
  * samples of the current cut-off shape, replacing the table lookup of
- * gravity_kernel_avx2.c:74-100 (layout and target in mlp_kernels.h).
+ * gravity_kernel_avx2.c:74-100.  For the S2 shape of
+ * pg5_gen_s2_force_table it learns H(u) = R(u, 1) of pg5_table.c:8-22 on
+ * u in [0, 1] (form 1: rc = eps_PM, r0 = eps_PP); for other shapes
+ * Y(v) = f(r) (r^3 + r0^3), v = sqrt(r / rc) (form 0).  Parameters p321 in
+ * the tanh form, FP64 (layout in mlp_kernels.h).
  * mode 0 = table (default), 1 = network, for the drop-in calls. */
 int pg5_set_cutoff_mode(int mode);
 int pg5_get_cutoff_mode(void);
-int pg5_mlp_set_params(const float *p321, double rc, double r0);
-int pg5_mlp_get_params(float *p321, double *rc, double *r0);
-int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, double *loss);
-int pg5_mlp_grad(int n, const float *v, const float *t, const float *wt, const float *p321,
-                 float *grad321, double *loss);
+int pg5_mlp_set_params(const double *p321, int form, double rc, double r0);
+/* Returns 1 and copies the loaded network (each output nullable), 0 if none. */
+int pg5_mlp_get_params(double *p321, int *form, double *rc, double *r0);
+/* Levenberg-Marquardt on the GPU: `starts` seeded starts of `iters` trial
+ * steps each (FP64 Jacobians and normal equations from the training kernels,
+ * the damped 321 x 321 solve on the host); keeps the best.  *loss = weighted
+ * mean squared error of the kept network. */
+int pg5_mlp_train(int iters, int starts, unsigned seed, double *loss);
+/* Normal equations of the weighted fit at p321 to n host samples, from the
+ * FP64 training kernels: jtj[321*321] (nullable), jtr[321], *rtr, with
+ * residuals sqrt(wt) (y(v) - t).  Synchronous; for checking the kernels. */
+int pg5_mlp_normal_eq(int n, const double *v, const double *t, const double *wt,
+                      const double *p321, double *jtj, double *jtr, double *rtr);
+/* Learned f(r) at n physical radii (device arrays), as the force kernels decode it. */
 int pg5_mlp_eval_dev(int n, const float *r, float *f);
 /* Batched TreePM short-range force of every group in one persistent launch
  * (the per-group g5_set_xmjMC + g5_calculate_force_on_xMC loop of a TreePM
- * caller).  Device arrays: pos[np] float4 {x*xs,y*xs,z*xs,m}; per group g
- * ilist[ioff[g]..+icnt[g]), jlist[joff[g]..+jcnt[g]); order nullable;
- * acc[i][3] FP64 written for the listed i; mode 0 table, 1 network. */
+ * caller).  Device arrays: pos[np] float4 {x*xs,y*xs,z*xs,m}; pos64[np][3]
+ * (nullable) the unscaled FP64 positions, from which separations are formed
+ * relative to each group's first i-particle (the reference's ingest
+ * conversions applied to x - o, so close pairs keep their digits); per
+ * group g ilist[ioff[g]..+icnt[g]), jlist[joff[g]..+jcnt[g]); order
+ * nullable; acc[i][3] FP64 written for the listed i; mode 0 table,
+ * 1 network. */
 int pg5t_batch_force(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
                      const int *joff, const int *jcnt, const int *jlist, const int *order,
-                     const void *pos, double *acc, int mode, void *stream);
+                     const void *pos, const double *pos64, double *acc, int mode, void *stream);
 void *pg5t_batch_graph(int devid, int ngroups, const int *ioff, const int *icnt,
                        const int *ilist, const int *joff, const int *jcnt, const int *jlist,
-                       const int *order, const void *pos, double *acc, int mode);
+                       const int *order, const void *pos, const double *pos64, double *acc,
+                       int mode);
 int pg5t_batch_graph_launch(void *graph, void *stream);
 int pg5t_batch_graph_destroy(void *graph);
 int pg5t_batch_autotune(int devid, int ngroups, const int *ioff, const int *icnt,
                         const int *ilist, const int *joff, const int *jcnt, const int *jlist,
-                        const int *order, const void *pos, double *acc, int mode, int reps,
-                        int *best, float *ms);
+                        const int *order, const void *pos, const double *pos64, double *acc,
+                        int mode, int reps, int *best, float *ms);
 int pg5t_batch_set_blocks_per_cu(int mode, int bpc);
 /* Long-range (particle-mesh) side of the TreePM force, which the reference
  * does not ship (PG5/table/libpg5/pppm.h:4-6 names its mesh): the periodic
@@ -130,7 +162,11 @@ int pg5t_pm_force(int n, const double *pos, const double *mass, int mesh, double
 int pg6_dev_set_j(int nj, const double *pos, const double *vel, const double *acc2,
                   const double *jrk6, const double *mass, const double *tj, const int *idx);
 /* Work of g6_set_ti + g6calc_firsthalf (phantom_g6.c:138,153): predict
- * j-particles 0..nj-1 to ti. */
+ * j-particles 0..nj-1 to ti.
+ * Ordering: every call that reads or writes the board's j-memory or its
+ * partial sums (pg6_dev_set_j, _predict, _force, the g6calc_* calls) runs on
+ * the device after the previous such call, whichever streams they were
+ * queued on (the library makes the later stream wait for the earlier one). */
 int pg6_dev_predict(int nj, double ti, void *stream);
 /* Work of g6calc_lasthalf2p (phantom_g6.c:270) on device arrays: pos, vel
  * [ni][3] FP64, id[ni]; acc, jrk [ni][3], pot[ni] FP64; nnb/rnnb may be NULL

@@ -6,8 +6,9 @@ Network: 1-16-16-1, tanh, input x = 2v - 1, parameters p[321] laid out as
 w1[16], b1[16], W2[16][16] (row = second-layer unit), b2[16], w3[16], b3.
 Loss: sum_s wt_s (y_s - t_s)^2 / n.
 
-Pinned by central finite differences of the loss (tests/test_oracle.py) and
-by the closed-form cut-off shape it is trained to reproduce.
+Pinned by central finite differences of the loss and of the outputs (the
+Jacobian), tests/test_oracle.py, and by the closed-form cut-off shape it is
+trained to reproduce.
 """
 from __future__ import annotations
 
@@ -54,6 +55,43 @@ def loss_and_grad(p, v, t, wt):
     g[16:32] = da1.sum(0)
     g[0:16] = da1.T @ x
     return loss, g
+
+
+def jacobian(p, v):
+    """Outputs y[n] and Jacobian J[n][321] = dy/dp (FP64, backpropagation)."""
+    w1, b1, W2, b2, w3, b3 = unpack(p)
+    v = np.asarray(v, np.float64)
+    n = len(v)
+    x = 2.0 * v - 1.0
+    h1 = np.tanh(np.outer(x, w1) + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    y = h2 @ w3 + b3
+    J = np.zeros((n, NP))
+    J[:, 320] = 1.0
+    J[:, 304:320] = h2
+    da2 = w3[None, :] * (1.0 - h2 * h2)
+    J[:, 288:304] = da2
+    J[:, 32:288] = (da2[:, :, None] * h1[:, None, :]).reshape(n, 256)
+    da1 = (da2 @ W2) * (1.0 - h1 * h1)
+    J[:, 16:32] = da1
+    J[:, 0:16] = da1 * x[:, None]
+    return y, J
+
+
+def normal_eq(p, v, t, wt):
+    """J^T J, J^T r, r^T r of the weighted residuals r = sqrt(wt) (y - t),
+    J = dr/dp (what the GPU fit kernels compute)."""
+    y, J = jacobian(p, v)
+    sw = np.sqrt(np.asarray(wt, np.float64))
+    Jw = J * sw[:, None]
+    r = sw * (y - np.asarray(t, np.float64))
+    return Jw.T @ Jw, Jw.T @ r, float(r @ r)
+
+
+def s2_unit(u, reff):
+    """Target of the S2 form: H(u) = R(u, 1), the S2 shape at softening 1
+    (reff = the oracle's restatement of pg5_table.c:8-22)."""
+    return np.array([reff(1.0, float(x)) for x in np.atleast_1d(u)])
 
 
 def target(f, r, rc, r0):

@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
              _dp, _ip, _dp, _dp, _dp, _ip, _dp)
         _sig(L, "or_pg6_nblist", None, C.c_int, _dp, _ip, _dp, C.c_double, C.c_int, _dp, _ip,
              C.c_int, _ip, _ip)
+        _sig(L, "or_rsqrt_bias_of", C.c_double, _fp, _fp)
+        _sig(L, "or_rsqrtss_table", None, _fp, _fp)
         _sig(L, "or_s2_reff", C.c_double, C.c_double, C.c_double)
         _sig(L, "or_s2_force", C.c_double, C.c_double, C.c_double, C.c_double)
         _sig(L, "or_table_build", None, C.c_double, C.c_double, C.c_int, C.c_int, _fp, _fp, _dp)
@@ -128,6 +130,33 @@ def pg6_nblist(xi, idi, h2, eps2, xj, idj, maxlen=1024):
     lib().or_pg6_nblist(ni, _P(xi), _P(idi, _ip), _P(h2), float(eps2), len(idj), _P(xj),
                         _P(idj, _ip), maxlen, _P(nbl, _ip), _P(nblen, _ip))
     return [nbl[i, : min(nblen[i], maxlen)].copy() for i in range(ni)], nblen
+
+
+RSQRT_N = 1 << 23
+
+
+def rsqrt_args():
+    """The arguments of rsqrt.c:42-81: floats with bits (127 << 23) + i and
+    (128 << 23) + i, i < 2^23 (x0 in [1, 2), x1 = 2 x0 in [2, 4))."""
+    i = np.arange(RSQRT_N, dtype=np.int32)
+    return ((127 << 23) + i).view(np.float32), ((128 << 23) + i).view(np.float32)
+
+
+def rsqrt_bias_of(y0, y1):
+    """rsqrt_bias() of PG5/newton/libpg5/rsqrt.c:42-81 for the rsqrt values
+    y0, y1 at rsqrt_args() (or_rsqrt_bias_of)."""
+    y0 = np.ascontiguousarray(y0, np.float32)
+    y1 = np.ascontiguousarray(y1, np.float32)
+    assert len(y0) == len(y1) == RSQRT_N
+    return lib().or_rsqrt_bias_of(_P(y0, _fp), _P(y1, _fp))
+
+
+def rsqrtss_table():
+    """The reference's rsqrt() (SSE rsqrtss) at rsqrt_args()."""
+    y0 = np.zeros(RSQRT_N, np.float32)
+    y1 = np.zeros(RSQRT_N, np.float32)
+    lib().or_rsqrtss_table(_P(y0, _fp), _P(y1, _fp))
+    return y0, y1
 
 
 def s2_reff(a, r):

@@ -8,9 +8,50 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include <xmmintrin.h>
 #include "hermite_ref.h"
 
 /* ---------------------------------------------------------------- GRAPE-5 */
+/* PG5/newton/libpg5/rsqrt.c:42-81, the loop in its order, on given values. */
+double or_rsqrt_bias_of(const float *y0v, const float *y1v) {
+  union {
+    int i;
+    float f;
+  } b0, b1;
+  double sw = 0.0, se = 0.0;
+  for (int i = 0; i < (1 << 23); i++) {
+    b0.i = (127 << 23) + i;
+    b1.i = (128 << 23) + i;
+    const double x0 = (double)b0.f, x1 = (double)b1.f;
+    const double y0 = (double)y0v[i], y1 = (double)y1v[i];
+    const double e0 = y0 * y0 * x0 - 1., e1 = y1 * y1 * x1 - 1.;
+    const double w = y0 * y0;
+    sw += w;
+    se += w * (e0 + e1);
+  }
+  sw *= 2.0;
+  return 0.5 * se / sw;
+}
+
+/* The reference's rsqrt() (rsqrt.c:12-22): the SSE rsqrtss estimate. */
+void or_rsqrtss_table(float *y0, float *y1) {
+  union {
+    int i;
+    float f;
+  } b;
+  for (int i = 0; i < (1 << 23); i++) {
+    float r;
+    b.i = (127 << 23) + i;
+    __m128 v = _mm_rsqrt_ss(_mm_load_ss(&b.f));
+    _mm_store_ss(&r, v);
+    y0[i] = r;
+    b.i = (128 << 23) + i;
+    v = _mm_rsqrt_ss(_mm_load_ss(&b.f));
+    _mm_store_ss(&r, v);
+    y1[i] = r;
+  }
+}
+
 /* PG5/newton/libpg5/gravity_avx2.c:95-154: dx = xj - xi, r2 = |dx|^2 + eps2,
  * phi accumulates m/r (output sign made positive by phantom_g5.c:52,258),
  * a accumulates m dx / r^3.  Symmetric softening adds eps_j^2

@@ -63,6 +63,15 @@ void or_pg6_nblist(int ni, const double *xi, const int *idi, const double *h2,
 
 /* S2 (Hockney-Eastwood) force shape, r^-3 units (PG5/table/libpg5/
  * pg5_table.c:8-29). */
+/* rsqrt bias of PG5/newton/libpg5/rsqrt.c:42-81 for a given reciprocal
+ * square root: y0[i], y1[i] = rsqrt of the floats with bits (127 << 23) + i
+ * and (128 << 23) + i, i < 2^23 (x in [1, 2) and [2, 4)).  b = sum w (e0 + e1)
+ * / (2 * 2 sum w) with e = y^2 x - 1, w = y0^2.  Pinned: with the SSE
+ * rsqrtss values (or_rsqrtss_table, the reference's rsqrt(), rsqrt.c:12-22)
+ * it equals the compiled reference's rsqrt_bias() bit for bit; with the
+ * correctly rounded FP32 1/sqrt(x) it is ~0. */
+double or_rsqrt_bias_of(const float *y0, const float *y1);
+void or_rsqrtss_table(float *y0, float *y1);
 double or_s2_reff(double sft, double r);
 double or_s2_force(double eps_pp, double eps_pm, double r);
 

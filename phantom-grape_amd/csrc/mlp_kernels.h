@@ -2,21 +2,30 @@
 // table lookup of the tabulated-cutoff force (PG5/table/libpg5/
 // gravity_kernel_avx2.c:74-100).  No counterpart in the reference.
 //
-// What is learned.  With r_c the cut-off radius and r_0 a core length (the PP
-// softening for the S2 shape), the network maps v = sqrt(r / r_c) in [0, 1]
-// to Y(v) = f(r) (r^3 + r_0^3), where f is the tabulated shape in r^-3 units.
-// Y is bounded and smooth on [0, 1] (it tends to f(0) r_0^3 at r = 0 and to
-// r^3 f(r) <= 1 outside the core), so an absolute error of the network is a
-// relative error of the pair force wherever Y is O(1).  The kernel recovers
-// f = Y / (r^3 + r_0^3) and uses f = 0 for r >= r_c, as the table does.
+// What is learned depends on the shape the table holds.
+//  * S2 shape (pg5_gen_s2_force_table; f = R(r, eps_PP) - R(r, eps_PM) with
+//    R the force of two S2 spheres, PG5/table/libpg5/pg5_table.c:8-34).  With
+//    H(u) = a^3 R(u a, a), the universal S2 shape on u in [0, 1] (H(0) = 12.8,
+//    H(1) = 1, R = 1/r^3 beyond u = 1), the network learns H on u in [0, 1]:
+//      f(r) = [r >= eps_PP] / r^3 + [r < eps_PP] H(r/eps_PP) / eps_PP^3
+//             - H(r/eps_PM) / eps_PM^3                     for r < eps_PM,
+//    and f = 0 beyond eps_PM.  H is bounded and smooth, so the network's
+//    absolute error is at most the same relative error of a pair's Newtonian
+//    force.  Pairs inside eps_PP (rare) take a second evaluation.
+//  * Any other shape f (pg5_gen_force_table): with r_0 = r_c / 30 the network
+//    maps v = sqrt(r / r_c) to Y(v) = f(r) (r^3 + r_0^3), bounded on [0, 1],
+//    and f = Y / (r^3 + r_0^3) for r < r_c.
+// Both are decoded by Cut below:  f = cn [not core] / r^3 + y c1 / (c3 r^3 + c2)
+// (+ y_core cpp in the S2 core).
 //
 // Parameters (321 floats): w1[16] @0, b1[16] @16, W2[16][16] @32 (row k =
 // second-layer unit, column j = first-layer unit), b2[16] @288, w3[16] @304,
 // b3 @320.  Input scaling x = 2 v - 1.
 //
-// Training kernels: mlp_sample_grad writes each sample's weighted-MSE
-// gradient (transposed, [param][sample]); mlp_reduce_adam sums them in a
-// fixed order (deterministic) and applies one Adam step.
+// Training: Levenberg-Marquardt in FP64.  mlp_fwd_jac64 writes every
+// sample's residual and Jacobian row (FP64 forward and backward pass);
+// mlp_jtj64 forms J^T J, J^T r and r^T r in a fixed order (deterministic); the
+// host solves the damped 321 x 321 system.
 //
 // Inference form.  Since tanh(z) = 2 s(c z) - 1 with s(u) = 1 / (1 + 2^-u)
 // and c = 2 log2(e), the same network is evaluated with the cheaper
@@ -49,30 +58,6 @@ __device__ inline cfloat* as_const(const float* p) { return (cfloat*)p; }
 __device__ inline cfloat* fresh(cfloat* p) {
   asm volatile("" : "+s"(p));
   return p;
-}
-
-// tanh(x) = 1 - 2 / (1 + 2^(2 x log2 e)) with the hardware exp2 and
-// reciprocal (each about 1 ulp); saturates cleanly to +-1.
-__device__ inline float tanh_fast(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
-}
-
-// Forward pass.  `w` is uniform across the wavefront (scalar loads).
-__device__ inline float eval(cfloat* w, float v) {
-  const float x = fmaf(2.f, v, -1.f);
-  float h1[kH];
-#pragma unroll
-  for (int j = 0; j < kH; ++j) h1[j] = tanh_fast(fmaf(w[kW1 + j], x, w[kB1 + j]));
-  float y = w[kB3];
-#pragma unroll
-  for (int k = 0; k < kH; ++k) {
-    float a = w[kB2 + k];
-#pragma unroll
-    for (int j = 0; j < kH; ++j) a = fmaf(w[kW2 + kH * k + j], h1[j], a);
-    y = fmaf(w[kW3 + k], tanh_fast(a), y);
-  }
-  return y;
 }
 
 // Logistic activation of the inference form: 1 / (1 + 2^-u).
@@ -110,110 +95,148 @@ __device__ inline void eval_k(cfloat* w, const float (&v)[K], float (&y)[K]) {
   }
 }
 
-// Per-sample gradient of wt * (y - t)^2 (divided by n), transposed output
-// g[p * n + s]; loss[s] = wt * (y - t)^2 / n.
-__global__ void mlp_sample_grad(int n, const float* __restrict__ v, const float* __restrict__ t,
-                                const float* __restrict__ wt, const float* __restrict__ wg,
-                                float* __restrict__ g, float* __restrict__ loss) {
+// ---------------------------------------------------------------- FP64 fit
+// The network is fitted by Levenberg-Marquardt on the weighted least-squares
+// loss sum_s wt_s (y(v_s) - t_s)^2.  Each iteration needs the residuals and
+// the Jacobian of every sample, and their products J^T J, J^T r, r^T r; all in
+// FP64 on the GPU (the 321 x 321 damped solve runs on the host).
+
+constexpr int kExt = kParams + 1;  // Jacobian columns + the residual column
+constexpr int kTile = 16;          // J^T J output tile
+constexpr int kTiles = (kExt + kTile - 1) / kTile;  // 21
+constexpr int kChunk = 64;         // samples staged per step of mlp_jtj64
+
+// Per-sample forward pass and Jacobian row, FP64, transposed output:
+// jac[p * n + s] = sqrt(wt_s) dy_s/dp for p < kParams and
+// jac[kParams * n + s] = sqrt(wt_s) (y_s - t_s) (the residual).  p64: the
+// tanh-form parameters in FP64 (uniform loads).
+__global__ void mlp_fwd_jac64(int n, const double* __restrict__ v, const double* __restrict__ t,
+                              const double* __restrict__ sw, const double* __restrict__ p64,
+                              double* __restrict__ jac) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  cfloat* w = fresh(as_const(wg));
-  const float x = fmaf(2.f, v[s], -1.f);
-  float h1[kH], h2[kH];
+  const double x = 2.0 * v[s] - 1.0;
+  double h1[kH], h2[kH];
 #pragma unroll
-  for (int j = 0; j < kH; ++j) h1[j] = tanh_fast(fmaf(w[kW1 + j], x, w[kB1 + j]));
-  float y = w[kB3];
+  for (int j = 0; j < kH; ++j) h1[j] = tanh(fma(p64[kW1 + j], x, p64[kB1 + j]));
+  double y = p64[kB3];
 #pragma unroll
   for (int k = 0; k < kH; ++k) {
-    float a = w[kB2 + k];
+    double a = p64[kB2 + k];
 #pragma unroll
-    for (int j = 0; j < kH; ++j) a = fmaf(w[kW2 + kH * k + j], h1[j], a);
-    h2[k] = tanh_fast(a);
-    y = fmaf(w[kW3 + k], h2[k], y);
+    for (int j = 0; j < kH; ++j) a = fma(p64[kW2 + kH * k + j], h1[j], a);
+    h2[k] = tanh(a);
+    y = fma(p64[kW3 + k], h2[k], y);
   }
-  w = fresh(w);  // reload for the backward pass
-  const float inv_n = 1.f / (float)n;
-  const float r = y - t[s];
-  loss[s] = wt[s] * r * r * inv_n;
-  const float dy = 2.f * wt[s] * r * inv_n;
+  const double w = sw[s];
   const size_t N = (size_t)n;
-  g[(size_t)kB3 * N + s] = dy;
-  float dh1[kH];
+  jac[(size_t)kParams * N + s] = w * (y - t[s]);
+  jac[(size_t)kB3 * N + s] = w;
+  double dh1[kH];
 #pragma unroll
-  for (int j = 0; j < kH; ++j) dh1[j] = 0.f;
+  for (int j = 0; j < kH; ++j) dh1[j] = 0.0;
 #pragma unroll
   for (int k = 0; k < kH; ++k) {
-    g[(size_t)(kW3 + k) * N + s] = dy * h2[k];
-    const float da = dy * w[kW3 + k] * (1.f - h2[k] * h2[k]);
-    g[(size_t)(kB2 + k) * N + s] = da;
+    jac[(size_t)(kW3 + k) * N + s] = w * h2[k];
+    const double da = w * p64[kW3 + k] * (1.0 - h2[k] * h2[k]);
+    jac[(size_t)(kB2 + k) * N + s] = da;
 #pragma unroll
     for (int j = 0; j < kH; ++j) {
-      g[(size_t)(kW2 + kH * k + j) * N + s] = da * h1[j];
-      dh1[j] = fmaf(da, w[kW2 + kH * k + j], dh1[j]);
+      jac[(size_t)(kW2 + kH * k + j) * N + s] = da * h1[j];
+      dh1[j] = fma(da, p64[kW2 + kH * k + j], dh1[j]);
     }
   }
 #pragma unroll
   for (int j = 0; j < kH; ++j) {
-    const float da = dh1[j] * (1.f - h1[j] * h1[j]);
-    g[(size_t)(kB1 + j) * N + s] = da;
-    g[(size_t)(kW1 + j) * N + s] = da * x;
+    const double da = dh1[j] * (1.0 - h1[j] * h1[j]);
+    jac[(size_t)(kB1 + j) * N + s] = da;
+    jac[(size_t)(kW1 + j) * N + s] = da * x;
   }
 }
 
-// One block per parameter (blockIdx.x < kParams) sums its n per-sample
-// gradients in a fixed tree order; block kParams sums the loss.  With
-// adam != 0, thread 0 then applies an Adam step to w[p] (m, u: moments,
-// step = 1-based Adam step).  Sums are written to gsum[p] (gsum[kParams] =
-// loss).
-__global__ __launch_bounds__(256) void mlp_reduce_adam(int n, const float* __restrict__ g,
-                                                       const float* __restrict__ loss,
-                                                       float* __restrict__ w, float* m, float* u,
-                                                       float* gsum, int adam, int step, float lr,
-                                                       float b1, float b2, float eps) {
-  __shared__ double part[256];
-  const int p = blockIdx.x;
-  const float* src = p < kParams ? g + (size_t)p * n : loss;
+// M = E^T E for the n x kExt matrix E = [J | r] held transposed in jac
+// (jac[c * n + s]): one 16 x 16 output tile per block, blocks with tile row
+// <= tile column only (M is symmetric), samples summed in a fixed order (64
+// at a time through LDS, each thread's partial sum in sample order), so the
+// result is the same on every run.  out[a * kExt + b] for a <= b.
+// M holds J^T J (a, b < kParams), J^T r (b = kParams) and r^T r.
+__global__ __launch_bounds__(256) void mlp_jtj64(int n, const double* __restrict__ jac,
+                                                 double* __restrict__ out) {
+  const int ta = blockIdx.y, tb = blockIdx.x;
+  if (ta > tb) return;
+  __shared__ double sa[kTile][kChunk + 1];
+  __shared__ double sb[kTile][kChunk + 1];
+  const int tid = threadIdx.x;
+  const int ia = tid / kTile, ib = tid % kTile;
+  const int a0 = ta * kTile, b0 = tb * kTile;
   double acc = 0.0;
-  for (int s = threadIdx.x; s < n; s += 256) acc += (double)src[s];
-  part[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+  for (int s0 = 0; s0 < n; s0 += kChunk) {
+    const int m = min(kChunk, n - s0);
     __syncthreads();
+    for (int k = tid; k < kTile * kChunk; k += 256) {
+      const int r = k / kChunk, c = k % kChunk;
+      const int ca = a0 + r, cb = b0 + r;
+      sa[r][c] = (c < m && ca < kExt) ? jac[(size_t)ca * n + s0 + c] : 0.0;
+      sb[r][c] = (c < m && cb < kExt) ? jac[(size_t)cb * n + s0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int c = 0; c < kChunk; ++c) acc = fma(sa[ia][c], sb[ib][c], acc);
   }
-  if (threadIdx.x != 0) return;
-  const float gs = (float)part[0];
-  gsum[p] = gs;
-  if (!adam || p >= kParams) return;
-  const float mp = b1 * m[p] + (1.f - b1) * gs;
-  const float up = b2 * u[p] + (1.f - b2) * gs * gs;
-  m[p] = mp;
-  u[p] = up;
-  const float mh = mp / (1.f - powf(b1, (float)step));
-  const float uh = up / (1.f - powf(b2, (float)step));
-  w[p] -= lr * mh / (sqrtf(uh) + eps);
+  const int a = a0 + ia, b = b0 + ib;
+  if (a < kExt && b < kExt && a <= b) out[(size_t)a * kExt + b] = acc;
 }
 
-// Evaluates the learned f(r) at n physical radii (for tests and error
-// measurement) with the inference-form parameters `wi`, as the force kernel
-// does: f = Y(sqrt(r/rc)) / (r^3 + r0^3), 0 for r >= rc.
-__global__ void mlp_eval_f(int n, const float* __restrict__ r, const float* __restrict__ wi,
-                           float rc, float r03, float* __restrict__ out) {
+// Decoding of the network output into the pair shape f (physical r^-3
+// units) at the scaled radius r = |dx_s| (see the top of this file).
+struct Cut {
+  const float* w;  // inference-form parameters (device)
+  float inv_rc;    // 1 / r_c (scaled units)
+  float rc2;       // r_c^2
+  int sqrt_in;     // network input: 1: sqrt(r / r_c), 0: r / r_c
+  float cn, c1, c2, c3;
+  float inv_pp, pp2, cpp;  // S2 core: r^2 < pp2 adds y(r / eps_PP) cpp (pp2 = 0: none)
+};
+
+__device__ inline float cut_input(const Cut& c, float r) {
+  const float u = fminf(r * c.inv_rc, 1.f);
+  return c.sqrt_in ? u * __builtin_amdgcn_rsqf(u) : u;
+}
+
+// f from the main evaluation y at radius r (rinv = 1/r, r2 = r^2).
+__device__ inline float cut_main(const Cut& c, float y, float r, float r2, float rinv) {
+  const float rinv3 = rinv * rinv * rinv;
+  const float newton = r2 < c.pp2 ? 0.f : c.cn * rinv3;
+  return fmaf(y * c.c1, __builtin_amdgcn_rcpf(fmaf(c.c3, r2 * r, c.c2)), newton);
+}
+
+// Learned f at n physical radii (for tests and error measurement), decoded as
+// the force kernels do: 0 for r >= r_c and for r = 0.
+__global__ void mlp_eval_f(int n, const float* __restrict__ r, Cut c, float xs,
+                           float* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const float rr = r[k];
-  float v[1] = {sqrtf(fminf(rr / rc, 1.f))};
+  const float rr = r[k] * xs;
+  const float r2 = rr * rr;
+  float v[1] = {cut_input(c, rr)};
   float y[1];
-  eval_k<1>(fresh(as_const(wi)), v, y);
-  out[k] = rr < rc ? y[0] / (rr * rr * rr + r03) : 0.f;
+  eval_k<1>(fresh(as_const(c.w)), v, y);
+  float f = cut_main(c, y[0], rr, r2, 1.f / rr);
+  if (r2 < c.pp2) {
+    float vc[1] = {rr * c.inv_pp};
+    eval_k<1>(fresh(as_const(c.w)), vc, y);
+    f = fmaf(y[0], c.cpp, f);
+  }
+  out[k] = (r2 > 0.f && r2 < c.rc2) ? f : 0.f;
 }
 
-// Host: tanh-form parameters -> inference form (see above), in FP64.
-inline void to_inference_form(const float* p, float* q) {
+// Host: tanh-form parameters (FP64) -> inference form (see above), rounded
+// once to FP32.
+inline void to_inference_form(const double* p, float* q) {
   const double c = 2.0 / 0.69314718055994530942;  // 2 log2(e)
   for (int j = 0; j < kH; ++j) {
     q[kW1 + j] = (float)(2.0 * c * p[kW1 + j]);
-    q[kB1 + j] = (float)(c * ((double)p[kB1 + j] - p[kW1 + j]));
+    q[kB1 + j] = (float)(c * (p[kB1 + j] - p[kW1 + j]));
   }
   double sw3 = 0.0;
   for (int k = 0; k < kH; ++k) {
@@ -222,11 +245,11 @@ inline void to_inference_form(const float* p, float* q) {
       q[kW2 + kH * k + j] = (float)(2.0 * c * p[kW2 + kH * k + j]);
       sw += p[kW2 + kH * k + j];
     }
-    q[kB2 + k] = (float)(c * ((double)p[kB2 + k] - sw));
+    q[kB2 + k] = (float)(c * (p[kB2 + k] - sw));
     q[kW3 + k] = (float)(2.0 * p[kW3 + k]);
     sw3 += p[kW3 + k];
   }
-  q[kB3] = (float)((double)p[kB3] - sw3);
+  q[kB3] = (float)(p[kB3] - sw3);
 }
 
 }  // namespace mlp

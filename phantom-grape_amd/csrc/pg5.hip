@@ -18,9 +18,11 @@
 // g5_set_xmj takes eps_j^2 and r^2 = |dx|^2 + eps_i^2 + eps_j^2.
 //
 // rsqrt bias: the reference scales outputs by (1-3b) and -(1-b), where b is
-// the measured bias of the 12-bit rsqrtps (rsqrt.c:42-81).  The GPU uses the
-// ~1 ulp v_rsq_f32, whose bias is negligible, so b = 0 and the output is
-// acc = sum m dx / r^3, pot = + sum m / r including the self pair.
+// the measured bias of its 12-bit rsqrtps (rsqrt.c:42-81).  The GPU uses the
+// ~1 ulp v_rsq_f32, whose bias is negligible, so by default b = 0 and the
+// output is acc = sum m dx / r^3, pot = + sum m / r including the self pair.
+// pg5_set_rsqrt_bias_mode(1) applies the reference's rule with b measured the
+// same way for v_rsq_f32 (pg5::rsqrt_bias_sums), once per process.
 #include "pgamd_common.h"
 #include "pgamd_api.h"
 #include "pg5_kernels.h"
@@ -88,6 +90,12 @@ Board g_boards[kMaxDev];
 std::mutex g_init_mutex;
 double g_eps = 0.0;
 int g_npipes = 0;
+// rsqrt bias correction (phantom_g5.c:45-60,244-269): mode 0 = none, 1 = the
+// reference's output scaling with the bias b of the device's rsqrt
+int g_bias_mode = 0;
+bool g_bias_ok = false;
+double g_bias = 0.0;
+std::mutex g_bias_mutex;
 
 // launch tuning (autotuner overrides)
 int g_force_bi = 0;
@@ -197,18 +205,23 @@ Plan plan(int ni, int nj, bool cells = false) {
   return p;
 }
 
-Partials5 partials(DevBuf& buf, int nsplit, int ni) {
+// Partial sums of nsplit j-ranges for ni i-particles in d (4 arrays of
+// nsplit * ni doubles), or none when nsplit == 1.
+Partials5 partials_at(double* d, int nsplit, int ni) {
   Partials5 part{};
   if (nsplit > 1) {
     const size_t n = (size_t)nsplit * ni;
-    buf.reserve(n * 4 * sizeof(double));
-    double* d = buf.as<double>();
     part.ax = d;
     part.ay = d + n;
     part.az = d + 2 * n;
     part.pot = d + 3 * n;
   }
   return part;
+}
+
+Partials5 partials(DevBuf& buf, int nsplit, int ni) {
+  if (nsplit > 1) buf.reserve((size_t)nsplit * ni * 4 * sizeof(double));
+  return partials_at(buf.as<double>(), nsplit, ni);
 }
 
 template <int BI>
@@ -233,9 +246,15 @@ void launch_cells_t(const Plan& p, int ni, const float4* xi, int nc, const Cell*
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
+// Launches the kernels of one force evaluation with plan p and partial sums
+// in part (asynchronous on st).
+void launch_force(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
+                  const float* ej2, const Cell* cells, const Partials5& part,
+                  const Outputs5& out, hipStream_t st);
+
 // Force of nj particles (cells == nullptr) or nc cells on ni i-particles; all
-// device pointers; asynchronous on st.  Partial-sum scratch lives in `scratch`.
-void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* xj,
+// device pointers; asynchronous on st.  Partial sums live in `work`.
+void run_force(DevBuf& work, int ni, const float4* xi, int nj, const float4* xj,
                const float* ej2, const Cell* cells, const Outputs5& out, hipStream_t st) {
   if (ni <= 0) return;
   if (nj <= 0) {
@@ -244,7 +263,12 @@ void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* 
     return;
   }
   const Plan p = plan(ni, nj, cells != nullptr);
-  const Partials5 part = partials(scratch, p.nsplit, ni);
+  launch_force(p, ni, xi, nj, xj, ej2, cells, partials(work, p.nsplit, ni), out, st);
+}
+
+void launch_force(const Plan& p, int ni, const float4* xi, int nj, const float4* xj,
+                  const float* ej2, const Cell* cells, const Partials5& part,
+                  const Outputs5& out, hipStream_t st) {
   if (cells) {
     switch (p.bi) {
       case 256: launch_cells_t<256>(p, ni, xi, nj, cells, part, out, st); break;
@@ -263,6 +287,34 @@ void run_force(DevBuf& scratch, int ni, const float4* xi, int nj, const float4* 
                        p.nsplit, part, out);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
+}
+
+// The bias b of the device's rsqrt by the rule of rsqrt.c:42-81 (measured on
+// the first call, then kept): the block sums of pg5::rsqrt_bias_sums added in
+// block order, b = (sum se) / (2 * 2 sum sw).
+double rsqrt_bias(hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_bias_mutex);
+  if (!g_bias_ok) {
+    DevBuf d;
+    d.reserve(sizeof(double) * 2 * pg5::kBiasBlocks);
+    hipLaunchKernelGGL(pg5::rsqrt_bias_sums, dim3(pg5::kBiasBlocks), dim3(256), 0, st,
+                       d.as<double>());
+    PGAMD_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(2 * pg5::kBiasBlocks);
+    PGAMD_HIP_CHECK(hipMemcpyAsync(h.data(), d.ptr, sizeof(double) * h.size(),
+                                   hipMemcpyDeviceToHost, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    d.release();
+    double sw = 0.0, se = 0.0;
+    for (int k = 0; k < pg5::kBiasBlocks; ++k) {
+      sw += h[2 * k];
+      se += h[2 * k + 1];
+    }
+    sw *= 2.0;
+    g_bias = 0.5 * se / sw;
+    g_bias_ok = true;
+  }
+  return g_bias;
 }
 
 // Host-side force evaluation for a board.  i-particles: x[ni][3] and either a
@@ -319,6 +371,17 @@ void board_calc(Board& b, const double (*x)[3], const double* eps2, const float4
   const double psign = cellmem < 0 ? 1.0 : -1.0;
   std::memcpy(a, ha, sizeof(double) * 3 * (size_t)ni);
   for (int i = 0; i < ni; ++i) p[i] = psign * hp[i];
+  if (g_bias_mode == 1) {
+    // the reference's factors, FP32 as there (phantom_g5.c:49-50,250-257)
+    const double bias = rsqrt_bias(b.stream);
+    const double ca = (double)(float)(1.0 - 3.0 * bias), cp = (double)(float)(1.0 - bias);
+    for (int i = 0; i < ni; ++i) {
+      a[i][0] *= ca;
+      a[i][1] *= ca;
+      a[i][2] *= ca;
+      p[i] *= cp;
+    }
+  }
 }
 
 void set_xmj_impl(int devid, int adr, int nj, const double (*xj)[3], const double* mj,
@@ -665,6 +728,26 @@ PGAMD_API int pg5_dev_quad(int devid, int ni, const void* xi, int nc, const void
   run_force(b->d_part, ni, static_cast<const float4*>(xi), nc, nullptr, nullptr,
             static_cast<const Cell*>(cells), o, stream ? (hipStream_t)stream : b->stream);
   return 0;
+}
+
+// rsqrt bias correction of the g5 outputs (the reference's A = acc (1 - 3b),
+// pot (1 - b), PG5/newton/libpg5/phantom_g5.c:45-60,244-269, also for the
+// cell calls, quad_g5c.c:285-295): mode 0 (default) none, 1 with b measured
+// for the device's rsqrt by the rule of rsqrt.c:42-81.
+PGAMD_API int pg5_set_rsqrt_bias_mode(int mode) {
+  if (mode != 0 && mode != 1) {
+    api_error(3, "pg5_set_rsqrt_bias_mode: mode %d not in {0, 1}", mode);
+    return -1;
+  }
+  g_bias_mode = mode;
+  return 0;
+}
+
+// b of the device's rsqrt (measured on the first call), whatever the mode.
+PGAMD_API double pg5_rsqrt_bias(void) {
+  Board* b = board(0, "pg5_rsqrt_bias");
+  if (!b) return 0.0;
+  return rsqrt_bias(b->stream);
 }
 
 PGAMD_API void* pg5_dev_stream(int devid) {

@@ -261,4 +261,44 @@ __global__ __launch_bounds__(256) void pg5_reduce(int ni, int nsplit, Partials5 
   out.pot[i] = t[3];
 }
 
+// Bias of the reciprocal square root the force kernels use (v_rsq_f32),
+// measured as PG5/newton/libpg5/rsqrt.c:42-81 measures the reference's
+// rsqrtss: over x0 = the 2^23 floats in [1, 2) (bits (127 << 23) + i) and
+// x1 = 2 x0, e = y^2 x - 1, w = y0^2; each block sums sw = sum w and
+// se = sum w (e0 + e1) over its i in a fixed order (FP64, contraction off),
+// part[2 b] = sw, part[2 b + 1] = se.  Grid: kBiasBlocks blocks of 256.
+constexpr int kBiasBlocks = 2048;
+__global__ __launch_bounds__(256) void rsqrt_bias_sums(double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double s_w[256], s_e[256];
+  constexpr int per = (1 << 23) / (kBiasBlocks * 256);
+  double sw = 0.0, se = 0.0;
+  for (int k = 0; k < per; ++k) {
+    const int i = (blockIdx.x * per + k) * 256 + threadIdx.x;
+    const float x0f = __int_as_float((127 << 23) + i);
+    const float x1f = __int_as_float((128 << 23) + i);
+    const double x0 = (double)x0f, x1 = (double)x1f;
+    const double y0 = (double)__builtin_amdgcn_rsqf(x0f);
+    const double y1 = (double)__builtin_amdgcn_rsqf(x1f);
+    const double e0 = y0 * y0 * x0 - 1.0, e1 = y1 * y1 * x1 - 1.0;
+    const double w = y0 * y0;
+    sw += w;
+    se += w * (e0 + e1);
+  }
+  s_w[threadIdx.x] = sw;
+  s_e[threadIdx.x] = se;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_w[threadIdx.x] += s_w[threadIdx.x + h];
+      s_e[threadIdx.x] += s_e[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s_w[0];
+    part[2 * blockIdx.x + 1] = s_e[0];
+  }
+}
+
 }  // namespace pg5

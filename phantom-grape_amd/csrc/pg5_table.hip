@@ -51,6 +51,12 @@ struct Table {
   std::vector<float> host;  // 2*ntab
   DevBuf dev;
   bool dirty = true;
+  // Quadratic form of the same table for the batched calls (order 2): per
+  // interval {c0, c1, c2, 0} with f = c0 + d (c1 + d c2), d = s - s_k.
+  int order = 1;
+  std::vector<float> quad;  // 4*ntab
+  DevBuf dev_q;
+  bool dirty_q = true;
   double xscale = 0.0;
   float r2scale = 0.f;
   double (*func)(double) = nullptr;  // last shape given to pg5_gen_force_table
@@ -75,10 +81,13 @@ struct Board {
   DevBuf d_in, d_out, d_part, d_counter;
 };
 
-// Learned cut-off (mlp_kernels.h): parameters, the radii it was trained for.
+// Learned cut-off (mlp_kernels.h): tanh-form parameters in FP64, the form of
+// the target and the radii it was trained for.  form 1 (S2 shape): rc =
+// eps_PM, r0 = eps_PP; form 0 (other shapes): rc = r_c, r0 = core length.
 struct Mlp {
-  std::vector<float> params;  // mlp::kParams
+  std::vector<double> params;  // mlp::kParams
   bool valid = false;
+  int form = 1;
   double rc = 0.0, r0 = 0.0;
   DevBuf dev;
   bool dirty = true;
@@ -158,6 +167,27 @@ void build_table(double (*func)(double), double rcut, int frc_bit) {
     t.host[2 * i + 1] = (t.host[2 * (i + 1)] - t.host[2 * i]) / (x1 - x0);
   }
   t.host[2 * (t.ntab - 1) + 1] = 0.0f;
+  // Quadratic through each interval's ends and midpoint, from the shape in
+  // FP64 at the same s ticks (the last entry is constant, as its slope is 0).
+  t.quad.assign(4 * (size_t)t.ntab, 0.f);
+  auto fs = [&](double s) { return func(std::sqrt(std::max(0.0, (s - 2.0) / (double)r2scale))); };
+  m.f = 2.0f;
+  for (int i = 0; i < t.ntab; i++) {
+    const double s0 = m.f;
+    m.u += tick;
+    const double s1 = m.f;
+    const double f0 = fs(s0);
+    if (i == t.ntab - 1) {
+      t.quad[4 * (size_t)i] = (float)f0;
+      break;
+    }
+    const double h = s1 - s0, fm = fs(s0 + 0.5 * h), f1 = fs(s1);
+    const double c2 = 2.0 * (f1 - 2.0 * fm + f0) / (h * h);
+    t.quad[4 * (size_t)i] = (float)f0;
+    t.quad[4 * (size_t)i + 1] = (float)((f1 - f0) / h - c2 * h);
+    t.quad[4 * (size_t)i + 2] = (float)c2;
+  }
+  t.dirty_q = true;
   t.func = func;
   t.rcut = rcut;
   t.dirty = true;
@@ -172,6 +202,18 @@ void ensure_table() {
 }
 
 // Upload the table if it changed (callers hold g_mutex).
+const float4* table_quad_dev(hipStream_t st) {
+  Table& t = g_tab;
+  if (t.dirty_q) {
+    t.dev_q.reserve(sizeof(float) * t.quad.size());
+    PGAMD_HIP_CHECK(hipMemcpyAsync(t.dev_q.ptr, t.quad.data(), sizeof(float) * t.quad.size(),
+                                   hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipStreamSynchronize(st));
+    t.dirty_q = false;
+  }
+  return t.dev_q.as<float4>();
+}
+
 const float2* table_dev(hipStream_t st) {
   Table& t = g_tab;
   if (t.dirty) {
@@ -372,6 +414,8 @@ bool check_range(const char* fn, int adr, int nj) {
 const float* mlp_dev(hipStream_t st) {
   Mlp& m = g_mlp;
   if (m.dirty) {
+    // the network is read while a force kernel runs on any stream
+    PGAMD_HIP_CHECK(hipDeviceSynchronize());
     float q[mlp::kParams];
     mlp::to_inference_form(m.params.data(), q);
     m.dev.reserve(sizeof(float) * mlp::kParams);
@@ -383,40 +427,66 @@ const float* mlp_dev(hipStream_t st) {
   return m.dev.as<float>();
 }
 
-// Core length of the learned target Y = f (r^3 + r0^3): the PP softening for
-// the S2 shape, r_c/30 for a user shape.
-double mlp_r0() { return g_tab.is_s2 ? g_tab.sft_pp : g_tab.rcut / 30.0; }
-
-pgb::MlpCut mlp_cut(hipStream_t st) {
-  pgb::MlpCut mc;
-  const double xs = g_tab.xscale;
-  mc.w = mlp_dev(st);
-  mc.inv_rc_s = (float)(1.0 / (g_mlp.rc * xs));
-  mc.rc2_s = (float)((g_mlp.rc * xs) * (g_mlp.rc * xs));
-  mc.r03_s = (float)std::pow(g_mlp.r0 * xs, 3);
-  mc.xs3 = (float)(xs * xs * xs);
-  return mc;
+// Decoding constants of the network for the current scale (mlp_kernels.h):
+// S2 form f = [not core] 1/r^3 - H(r/rc)/rc^3 (+ H(r/r0)/r0^3 in the core),
+// other shapes f = Y(sqrt(r/rc)) / (r^3 + r0^3); scaled units, physical f.
+mlp::Cut mlp_cut(hipStream_t st) {
+  mlp::Cut c;
+  const double xs = g_tab.xscale, xs3 = xs * xs * xs;
+  const double rc = g_mlp.rc * xs, r0 = g_mlp.r0 * xs;
+  c.w = mlp_dev(st);
+  c.inv_rc = (float)(1.0 / rc);
+  c.rc2 = (float)(rc * rc);
+  if (g_mlp.form == 1) {
+    c.sqrt_in = 0;
+    c.cn = (float)xs3;
+    c.c1 = (float)(-xs3 / (rc * rc * rc));
+    c.c2 = 1.f;
+    c.c3 = 0.f;
+    c.inv_pp = (float)(1.0 / r0);
+    c.pp2 = (float)(r0 * r0);
+    c.cpp = (float)(xs3 / (r0 * r0 * r0));
+  } else {
+    c.sqrt_in = 1;
+    c.cn = 0.f;
+    c.c1 = (float)xs3;
+    c.c2 = (float)(r0 * r0 * r0);
+    c.c3 = 1.f;
+    c.inv_pp = 0.f;
+    c.pp2 = 0.f;
+    c.cpp = 0.f;
+  }
+  return c;
 }
 
 // Launches the persistent batch kernel (counter reset + kernel) on st.
+// pos64 (nullable): FP64 positions for group-origin separations.
 void launch_batch(int mode, int ngroups, const int* order, const int* ioff, const int* icnt,
                   const int* ilist, const int* joff, const int* jcnt, const int* jlist,
-                  const float4* pos, double* acc, int* counter, int bpc, hipStream_t st) {
+                  const float4* pos, const double* pos64, double* acc, int* counter, int bpc,
+                  hipStream_t st) {
   hipLaunchKernelGGL(pgb::zero_counter, dim3(1), dim3(64), 0, st, counter);
   PGAMD_HIP_CHECK(hipGetLastError());
   if (ngroups <= 0) return;
   const int blocks = std::max(1, std::min(ngroups, device_cu_count() * bpc));
   const double ascale = (double)(float)(1.0 / g_tab.xscale);
-  pgb::TableCut tc{table_dev(st), g_tab.ntab, g_tab.frc_bit};
+  const pgb::Origin og{pos64, g_tab.xscale, (float)g_tab.xscale};
+  pgb::TableCut tc{table_dev(st), table_quad_dev(st), g_tab.ntab, g_tab.frc_bit};
   if (mode == 1) {
-    pgb::MlpCut mc = mlp_cut(st);
+    const mlp::Cut mc = mlp_cut(st);
     hipLaunchKernelGGL(pgb::treepm_batch_mlp, dim3(blocks), dim3(pgb::kThreads), 0, st, ngroups,
-                       order, ioff, icnt, ilist, joff, jcnt, jlist, pos, mc, ascale, acc, counter);
+                       order, ioff, icnt, ilist, joff, jcnt, jlist, pos, og, mc, ascale, acc,
+                       counter);
+  } else if (g_tab.order == 2) {
+    const size_t lds = sizeof(float4) * (size_t)g_tab.ntab;
+    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock, true>), dim3(blocks),
+                       dim3(pgb::kTableBlock), lds, st, ngroups, order, ioff, icnt, ilist, joff,
+                       jcnt, jlist, pos, og, tc, ascale, acc, counter);
   } else {
     const size_t lds = sizeof(float2) * (size_t)g_tab.ntab;
-    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock>), dim3(blocks),
+    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock, false>), dim3(blocks),
                        dim3(pgb::kTableBlock), lds, st, ngroups, order, ioff, icnt, ilist, joff,
-                       jcnt, jlist, pos, tc, ascale, acc, counter);
+                       jcnt, jlist, pos, og, tc, ascale, acc, counter);
   }
   PGAMD_HIP_CHECK(hipGetLastError());
 }
@@ -457,7 +527,7 @@ void run_mlp_dropin(DevBuf& scratch, int ni, const float4* xi, int nj, const flo
     h[(size_t)ni + nj + 3 * ng + g] = nj;
   }
   PGAMD_HIP_CHECK(hipMemcpyAsync(ilist, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
-  launch_batch(1, ng, nullptr, ioff, icnt, ilist, joff, jcnt, jlist, pos, accw, counter,
+  launch_batch(1, ng, nullptr, ioff, icnt, ilist, joff, jcnt, jlist, pos, nullptr, accw, counter,
                default_bpc(1), st);
   PGAMD_HIP_CHECK(hipMemcpyAsync(acc, accw + 3 * (size_t)nj, sizeof(double) * 3 * (size_t)ni,
                                  hipMemcpyDeviceToDevice, st));
@@ -496,9 +566,33 @@ PGAMD_API int pg5_set_table_frc_bit(int frc_bit) {
     return -1;
   }
   std::lock_guard<std::mutex> lk(g_mutex);
+  if (g_tab.order == 2 && frc_bit > 8) {
+    api_error(3, "pg5_set_table_frc_bit: %d with the quadratic table (at most 8)", frc_bit);
+    return -1;
+  }
   double (*f)(double) = g_tab.func ? g_tab.func : s2_force;
   const double rc = g_tab.func ? g_tab.rcut : g_tab.sft_pm;
   build_table(f, rc, frc_bit);
+  return 0;
+}
+
+// Interpolation order of the table in the batched calls (pg5t_batch_*):
+// 1 = the reference's value + slope per float-bit tick (default), 2 = a
+// quadratic through each interval's ends and midpoint, evaluated from the
+// shape in FP64 (16 bytes per entry in LDS: frc_bit <= 8).  The drop-in calls
+// keep the reference's table.  No counterpart in the reference.
+PGAMD_API int pg5_set_table_order(int order) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (order != 1 && order != 2) {
+    api_error(3, "pg5_set_table_order: %d not in {1, 2}", order);
+    return -1;
+  }
+  if (order == 2 && g_tab.frc_bit > 8) {
+    api_error(3, "pg5_set_table_order: the quadratic table needs frc_bit <= 8 (now %d)",
+              g_tab.frc_bit);
+    return -1;
+  }
+  g_tab.order = order;
   return 0;
 }
 
@@ -727,15 +821,18 @@ PGAMD_API int pg5_set_cutoff_mode(int mode) {
 
 PGAMD_API int pg5_get_cutoff_mode(void) { return g_cut_mode; }
 
-// Loads a network: p[321] (layout in mlp_kernels.h), trained for cut-off
-// radius rc and core length r0 (physical units).
-PGAMD_API int pg5_mlp_set_params(const float* p, double rc, double r0) {
+// Loads a network: tanh-form parameters p[321] in FP64 (layout in
+// mlp_kernels.h) of the given form (1: S2 shape H(u) with rc = eps_PM and
+// r0 = eps_PP; 0: Y(v) = f (r^3 + r0^3), v = sqrt(r / rc)); physical units.
+PGAMD_API int pg5_mlp_set_params(const double* p, int form, double rc, double r0) {
   std::lock_guard<std::mutex> lk(g_mutex);
-  if (!(rc > 0.0) || !(r0 >= 0.0)) {
-    api_error(3, "pg5_mlp_set_params: rc=%g r0=%g", rc, r0);
+  if (!p || (form != 0 && form != 1) || !(rc > 0.0) || !(r0 >= 0.0) ||
+      (form == 1 && !(r0 > 0.0 && r0 < rc))) {
+    api_error(3, "pg5_mlp_set_params: p=%p form=%d rc=%g r0=%g", (const void*)p, form, rc, r0);
     return -1;
   }
   g_mlp.params.assign(p, p + mlp::kParams);
+  g_mlp.form = form;
   g_mlp.rc = rc;
   g_mlp.r0 = r0;
   g_mlp.valid = true;
@@ -743,260 +840,269 @@ PGAMD_API int pg5_mlp_set_params(const float* p, double rc, double r0) {
   return 0;
 }
 
-// Copies the current network (p may be null); returns 1 if one is loaded.
-PGAMD_API int pg5_mlp_get_params(float* p, double* rc, double* r0) {
+// Copies the current network (each output may be null); returns 1 if one is
+// loaded, 0 if not.
+PGAMD_API int pg5_mlp_get_params(double* p, int* form, double* rc, double* r0) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!g_mlp.valid) return 0;
-  if (p) std::memcpy(p, g_mlp.params.data(), sizeof(float) * mlp::kParams);
+  if (p) std::memcpy(p, g_mlp.params.data(), sizeof(double) * mlp::kParams);
+  if (form) *form = g_mlp.form;
   if (rc) *rc = g_mlp.rc;
   if (r0) *r0 = g_mlp.r0;
   return 1;
 }
 
 namespace {
-struct TrainBufs {
-  DevBuf v, t, wt, w, m, u, g, loss, gsum;
+
+// Training set and work arrays of the FP64 fit (device).
+struct Fit {
+  int n = 0;
+  DevBuf v, t, sw, p, jac, m;
+  void alloc(int ns) {
+    n = ns;
+    v.reserve(sizeof(double) * ns);
+    t.reserve(sizeof(double) * ns);
+    sw.reserve(sizeof(double) * ns);
+    p.reserve(sizeof(double) * mlp::kParams);
+    jac.reserve(sizeof(double) * (size_t)mlp::kExt * ns);
+    m.reserve(sizeof(double) * mlp::kExt * mlp::kExt);
+  }
+  void upload(const double* hv, const double* ht, const double* hsw, hipStream_t st) {
+    PGAMD_HIP_CHECK(hipMemcpyAsync(v.ptr, hv, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(t.ptr, ht, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    PGAMD_HIP_CHECK(hipMemcpyAsync(sw.ptr, hsw, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  }
+  void release() {
+    for (DevBuf* d : {&v, &t, &sw, &p, &jac, &m}) d->release();
+  }
 };
 
-// One gradient evaluation (+ optional Adam step) on n samples at device
-// offset `off` of the training set; returns the summed loss through gsum.
-void mlp_step(TrainBufs& tb, int n, int off, int adam, int step, float lr, hipStream_t st) {
-  hipLaunchKernelGGL(mlp::mlp_sample_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, n,
-                     tb.v.as<float>() + off, tb.t.as<float>() + off, tb.wt.as<float>() + off,
-                     tb.w.as<float>(), tb.g.as<float>(), tb.loss.as<float>());
-  PGAMD_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(mlp::mlp_reduce_adam, dim3(mlp::kParams + 1), dim3(256), 0, st, n,
-                     tb.g.as<float>(), tb.loss.as<float>(), tb.w.as<float>(), tb.m.as<float>(),
-                     tb.u.as<float>(), tb.gsum.as<float>(), adam, step, lr, 0.9f, 0.999f, 1e-8f);
-  PGAMD_HIP_CHECK(hipGetLastError());
-}
-
-void tb_alloc(TrainBufs& tb, int nset, int batch) {
-  tb.v.reserve(sizeof(float) * nset);
-  tb.t.reserve(sizeof(float) * nset);
-  tb.wt.reserve(sizeof(float) * nset);
-  tb.w.reserve(sizeof(float) * mlp::kParams);
-  tb.m.reserve(sizeof(float) * mlp::kParams);
-  tb.u.reserve(sizeof(float) * mlp::kParams);
-  tb.g.reserve(sizeof(float) * (size_t)mlp::kParams * batch);
-  tb.loss.reserve(sizeof(float) * batch);
-  tb.gsum.reserve(sizeof(float) * (mlp::kParams + 1));
-}
-
-void tb_release(TrainBufs& tb) {
-  for (DevBuf* d : {&tb.v, &tb.t, &tb.wt, &tb.w, &tb.m, &tb.u, &tb.g, &tb.loss, &tb.gsum})
-    d->release();
-}
-
-// Full-batch loss and gradient at parameters x (host, FP64) over the nset
-// training samples on the device: the per-sample kernels in FP32, the sums in
-// FP64 in a fixed order.
-double full_loss_grad(TrainBufs& tb, int nset, const std::vector<double>& x,
-                      std::vector<double>& g, hipStream_t st) {
-  std::vector<float> xf(x.begin(), x.end());
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, xf.data(), sizeof(float) * mlp::kParams,
+// M = [J | r]^T [J | r] at parameters x (full symmetric, kExt x kExt, host):
+// J^T J, J^T r and r^T r of the weighted residuals sqrt(wt) (y - t).
+void fit_normal(Fit& f, const double* x, std::vector<double>& M, hipStream_t st) {
+  constexpr int E = mlp::kExt;
+  PGAMD_HIP_CHECK(hipMemcpyAsync(f.p.ptr, x, sizeof(double) * mlp::kParams,
                                  hipMemcpyHostToDevice, st));
-  mlp_step(tb, nset, 0, 0, 1, 0.f, st);
-  std::vector<float> gs(mlp::kParams + 1);
-  PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
+  hipLaunchKernelGGL(mlp::mlp_fwd_jac64, dim3(ceil_div(f.n, 64)), dim3(64), 0, st, f.n,
+                     f.v.as<double>(), f.t.as<double>(), f.sw.as<double>(), f.p.as<double>(),
+                     f.jac.as<double>());
+  PGAMD_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(mlp::mlp_jtj64, dim3(mlp::kTiles, mlp::kTiles), dim3(256), 0, st, f.n,
+                     f.jac.as<double>(), f.m.as<double>());
+  PGAMD_HIP_CHECK(hipGetLastError());
+  M.resize((size_t)E * E);
+  PGAMD_HIP_CHECK(hipMemcpyAsync(M.data(), f.m.ptr, sizeof(double) * E * E,
                                  hipMemcpyDeviceToHost, st));
   PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-  g.assign(gs.begin(), gs.begin() + mlp::kParams);
-  return gs[mlp::kParams];
+  for (int a = 0; a < E; ++a)
+    for (int b = 0; b < a; ++b) M[(size_t)a * E + b] = M[(size_t)b * E + a];
 }
 
-// Limited-memory BFGS (two-loop recursion, 20 pairs) with Armijo
-// backtracking, on full-batch GPU gradients.  Returns the final loss.
-double lbfgs(TrainBufs& tb, int nset, std::vector<double>& x, int iters, hipStream_t st) {
-  const int M = 20, P = mlp::kParams;
-  std::vector<std::vector<double>> S, Y;
-  std::vector<double> rho;
-  std::vector<double> g, gn, d(P), xn(P);
-  double f = full_loss_grad(tb, nset, x, g, st);
-  auto dot = [](const std::vector<double>& a, const std::vector<double>& b) {
-    double s = 0;
-    for (size_t k = 0; k < a.size(); ++k) s += a[k] * b[k];
-    return s;
-  };
-  for (int it = 0; it < iters; ++it) {
-    // d = -H g
-    std::vector<double> q = g, al(S.size());
-    for (int k = (int)S.size() - 1; k >= 0; --k) {
-      al[k] = rho[k] * dot(S[k], q);
-      for (int p = 0; p < P; ++p) q[p] -= al[k] * Y[k][p];
+// Solves A x = b in place (b <- x) by Cholesky; A (P x P, row-major) is
+// overwritten.  Returns false if A is not positive definite.
+bool chol_solve(std::vector<double>& A, std::vector<double>& b, int P) {
+  for (int j = 0; j < P; ++j) {
+    double d = A[(size_t)j * P + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * P + k] * A[(size_t)j * P + k];
+    if (!(d > 0.0)) return false;
+    const double l = std::sqrt(d);
+    A[(size_t)j * P + j] = l;
+    for (int i = j + 1; i < P; ++i) {
+      double s = A[(size_t)i * P + j];
+      for (int k = 0; k < j; ++k) s -= A[(size_t)i * P + k] * A[(size_t)j * P + k];
+      A[(size_t)i * P + j] = s / l;
     }
-    const double gamma = S.empty() ? 1e-3 / std::max(1e-30, std::sqrt(dot(g, g)))
-                                   : dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
-    for (int p = 0; p < P; ++p) q[p] *= gamma;
-    for (size_t k = 0; k < S.size(); ++k) {
-      const double b = rho[k] * dot(Y[k], q);
-      for (int p = 0; p < P; ++p) q[p] += S[k][p] * (al[k] - b);
-    }
-    for (int p = 0; p < P; ++p) d[p] = -q[p];
-    double gd = dot(g, d);
-    if (!(gd < 0)) {  // not a descent direction: restart from steepest descent
-      S.clear();
-      Y.clear();
-      rho.clear();
-      for (int p = 0; p < P; ++p) d[p] = -g[p] * 1e-3 / std::max(1e-30, std::sqrt(dot(g, g)));
-      gd = dot(g, d);
-    }
-    double step = 1.0, fn = 0.0;
-    bool ok = false;
-    for (int ls = 0; ls < 30; ++ls) {
-      for (int p = 0; p < P; ++p) xn[p] = x[p] + step * d[p];
-      fn = full_loss_grad(tb, nset, xn, gn, st);
-      if (std::isfinite(fn) && fn <= f + 1e-4 * step * gd) {
-        ok = true;
-        break;
-      }
-      step *= 0.5;
-    }
-    if (!ok) break;
-    std::vector<double> s(P), y(P);
-    for (int p = 0; p < P; ++p) {
-      s[p] = xn[p] - x[p];
-      y[p] = gn[p] - g[p];
-    }
-    const double sy = dot(s, y);
-    if (sy > 1e-20) {
-      if ((int)S.size() == M) {
-        S.erase(S.begin());
-        Y.erase(Y.begin());
-        rho.erase(rho.begin());
-      }
-      S.push_back(s);
-      Y.push_back(y);
-      rho.push_back(1.0 / sy);
-    }
-    x = xn;
-    g = gn;
-    const double rel = (f - fn) / std::max(f, 1e-300);
-    f = fn;
-    if (rel < 1e-12 && it > 50) break;
   }
-  return f;
+  for (int i = 0; i < P; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= A[(size_t)i * P + k] * b[k];
+    b[i] = s / A[(size_t)i * P + i];
+  }
+  for (int i = P - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < P; ++k) s -= A[(size_t)k * P + i] * b[k];
+    b[i] = s / A[(size_t)i * P + i];
+  }
+  return true;
 }
+
+// Levenberg-Marquardt (Marquardt's diagonal scaling) from x for at most
+// `iters` trial steps; returns r^T r at the final x.
+double lm_fit(Fit& f, std::vector<double>& x, int iters, hipStream_t st) {
+  constexpr int P = mlp::kParams, E = mlp::kExt;
+  std::vector<double> M, Mn, A((size_t)P * P), d(P), xn(P);
+  fit_normal(f, x.data(), M, st);
+  double F = M[(size_t)P * E + P];
+  double lam = 1e-3;
+  for (int it = 0; it < iters && lam < 1e12; ++it) {
+    for (int a = 0; a < P; ++a) {
+      for (int b = 0; b < P; ++b) A[(size_t)a * P + b] = M[(size_t)a * E + b];
+      A[(size_t)a * P + a] += lam * M[(size_t)a * E + a] + 1e-300;
+      d[a] = -M[(size_t)a * E + P];
+    }
+    if (!chol_solve(A, d, P)) {
+      lam *= 4.0;
+      continue;
+    }
+    for (int a = 0; a < P; ++a) xn[a] = x[a] + d[a];
+    fit_normal(f, xn.data(), Mn, st);
+    const double Fn = Mn[(size_t)P * E + P];
+    if (std::isfinite(Fn) && Fn < F) {
+      x.swap(xn);
+      M.swap(Mn);
+      F = Fn;
+      lam = std::max(lam / 3.0, 1e-15);
+    } else {
+      lam *= 4.0;
+    }
+  }
+  return F;
+}
+
+// Seeded initial parameters (tanh form): spread first-layer units, Glorot
+// second layer and output, output bias = mean target.
+void mlp_init(std::mt19937_64& rng, double tmean, std::vector<double>& p) {
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  p.assign(mlp::kParams, 0.0);
+  for (int j = 0; j < mlp::kH; ++j) {
+    p[mlp::kW1 + j] = 4.0 * U(rng);
+    p[mlp::kB1 + j] = 2.0 * U(rng);
+  }
+  const double lim = std::sqrt(6.0 / (2 * mlp::kH));
+  for (int k = 0; k < mlp::kH * mlp::kH; ++k) p[mlp::kW2 + k] = lim * U(rng);
+  for (int k = 0; k < mlp::kH; ++k) p[mlp::kW3 + k] = lim * U(rng);
+  p[mlp::kB3] = tmean;
+}
+
+// The S2 shape in units of a^-3 at u = r / a (pg5_table.c:8-22 with a = 1).
+double s2_unit(double u) { return s2_reff(1.0, u); }
+
 }  // namespace
 
-// Summed gradient and loss of the weighted MSE sum_s wt_s (Y(v_s) - t_s)^2 / n
-// for host arrays v, t, wt (n samples) at parameters p[321], computed by the
-// training kernels on the GPU.  For checking them against a CPU reference.
-PGAMD_API int pg5_mlp_grad(int n, const float* v, const float* t, const float* wt,
-                           const float* p, float* grad, double* loss) {
-  Board* b = board(0, "pg5_mlp_grad");
+// Normal equations of the weighted least-squares fit of the network at
+// parameters p[321] (tanh form, FP64) to n host samples (v, t, wt), computed
+// by the FP64 training kernels: jtj[321*321] (nullable) = J^T J, jtr[321] =
+// J^T r and *rtr = r^T r, with r_s = sqrt(wt_s) (y(v_s) - t_s) and J = dr/dp.
+// The gradient of the loss sum_s wt_s (y_s - t_s)^2 / n is 2 jtr / n.
+// Synchronous.  For checking the kernels against a CPU restatement.
+PGAMD_API int pg5_mlp_normal_eq(int n, const double* v, const double* t, const double* wt,
+                                const double* p, double* jtj, double* jtr, double* rtr) {
+  Board* b = board(0, "pg5_mlp_normal_eq");
   if (!b) return -1;
-  if (n <= 0) {
-    api_error(3, "pg5_mlp_grad: n=%d", n);
+  if (n <= 0 || !v || !t || !wt || !p || !jtr || !rtr) {
+    api_error(3, "pg5_mlp_normal_eq: n=%d (need n > 0 and non-null arrays)", n);
     return -1;
   }
-  TrainBufs tb;
-  tb_alloc(tb, n, n);
-  hipStream_t st = b->stream;
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.v.ptr, v, sizeof(float) * n, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.t.ptr, t, sizeof(float) * n, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.wt.ptr, wt, sizeof(float) * n, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, p, sizeof(float) * mlp::kParams,
-                                 hipMemcpyHostToDevice, st));
-  mlp_step(tb, n, 0, 0, 1, 0.f, st);
-  std::vector<float> gs(mlp::kParams + 1);
-  PGAMD_HIP_CHECK(hipMemcpyAsync(gs.data(), tb.gsum.ptr, sizeof(float) * gs.size(),
-                                 hipMemcpyDeviceToHost, st));
-  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-  std::memcpy(grad, gs.data(), sizeof(float) * mlp::kParams);
-  if (loss) *loss = gs[mlp::kParams];
-  tb_release(tb);
+  constexpr int P = mlp::kParams, E = mlp::kExt;
+  std::vector<double> sw(n);
+  for (int s = 0; s < n; ++s) sw[s] = std::sqrt(wt[s]);
+  Fit f;
+  f.alloc(n);
+  f.upload(v, t, sw.data(), b->stream);
+  std::vector<double> M;
+  fit_normal(f, p, M, b->stream);
+  f.release();
+  for (int a = 0; a < P; ++a) {
+    jtr[a] = M[(size_t)a * E + P];
+    if (jtj)
+      for (int c = 0; c < P; ++c) jtj[(size_t)a * P + c] = M[(size_t)a * E + c];
+  }
+  *rtr = M[(size_t)P * E + P];
   return 0;
 }
 
 // Trains the network on the GPU for the current cut-off shape (the function
-// the table was generated from, pg5_gen_force_table / pg5_gen_s2_force_table):
-// 65536 seeded samples of r in [0, r_c) (half uniform in v = sqrt(r/r_c),
-// half uniform in volume), target Y = f(r) (r^3 + r0^3) evaluated in FP64,
-// weight 1 / (|Y| + 0.05)^2 (relative accuracy where Y is O(1)).  Optimiser:
-// min(steps, 1000) Adam steps on mini-batches of `batch` at rate lr, then up
-// to `steps` L-BFGS iterations on the full set (loss and gradient from the GPU
-// kernels, the two-loop recursion on the host).  Deterministic for a given
-// seed.  *loss_out = weighted MSE on the full set.
-PGAMD_API int pg5_mlp_train(int steps, int batch, double lr, unsigned seed, double* loss_out) {
+// the table was generated from).  S2 shape (pg5_gen_s2_force_table): the
+// network learns H(u) = a^3 R(u a, a) of pg5_table.c:8-22 on 8192 stratified
+// samples of u in [0, 1] plus both ends, residuals weighted by u^3 + 0.2.  Other shapes
+// (pg5_gen_force_table): Y(v) = f(r) (r^3 + r0^3), r0 = r_c / 30, on 8192
+// seeded samples (half uniform in v = sqrt(r / r_c), half uniform in volume),
+// weights 1 / (|Y| + 0.05)^2.  Targets are evaluated in FP64.  `starts`
+// seeded initialisations each run `iters` Levenberg-Marquardt trial steps
+// (FP64 residuals, Jacobians and normal equations from the GPU kernels, the
+// damped 321 x 321 solve on the host); the one with the smallest loss is
+// kept.  Deterministic for a given seed.  *loss_out = sum wt (y - t)^2 / n.
+PGAMD_API int pg5_mlp_train(int iters, int starts, unsigned seed, double* loss_out) {
   Board* b = board(0, "pg5_mlp_train");
   if (!b) return -1;
-  if (steps <= 0 || batch <= 0 || batch > (1 << 16) || !(lr > 0.0)) {
-    api_error(3, "pg5_mlp_train: steps=%d batch=%d lr=%g", steps, batch, lr);
+  if (iters <= 0 || starts <= 0 || starts > 64) {
+    api_error(3, "pg5_mlp_train: iters=%d starts=%d (need iters > 0, 0 < starts <= 64)", iters,
+              starts);
     return -1;
   }
   std::lock_guard<std::mutex> lk(g_mutex);
   ensure_table();
-  double (*func)(double) = g_tab.func ? g_tab.func : s2_force;
-  const double rc = g_tab.rcut;
-  const double r0 = mlp_r0();
-  const double r03 = r0 * r0 * r0;
-  const int nset = 1 << 16;
+  const bool s2 = g_tab.is_s2;
+  const int n = 8192 + 2;
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
-  std::vector<float> hv(nset), ht(nset), hw(nset);
-  for (int k = 0; k < nset; ++k) {
-    double r, v;
-    if (k & 1) {
-      r = rc * std::cbrt(U(rng));
-      v = std::sqrt(r / rc);
-    } else {
-      v = U(rng);
-      r = rc * v * v;
+  std::vector<double> hv(n), ht(n), hsw(n);
+  double rc, r0;
+  if (s2) {
+    rc = g_tab.sft_pm;
+    r0 = g_tab.sft_pp;
+    // A pair outside the core carries the network's error times u^3 relative
+    // to its Newtonian force; inside the core (rare pairs) times 1/H:
+    // weight the residuals by u^3 + 0.2.
+    for (int k = 0; k < n - 2; ++k) {
+      hv[k] = (k + U(rng)) / (n - 2);
+      ht[k] = s2_unit(hv[k]);
+      hsw[k] = hv[k] * hv[k] * hv[k] + 0.2;
     }
-    const double y = func(r) * (r * r * r + r03);
-    hv[k] = (float)v;
-    ht[k] = (float)y;
-    hw[k] = (float)(1.0 / ((std::fabs(y) + 0.05) * (std::fabs(y) + 0.05)));
+    for (int e = 0; e < 2; ++e) {
+      hv[n - 2 + e] = e;
+      ht[n - 2 + e] = s2_unit(e);
+      hsw[n - 2 + e] = e + 0.2;
+    }
+  } else {
+    double (*func)(double) = g_tab.func;
+    rc = g_tab.rcut;
+    r0 = g_tab.rcut / 30.0;
+    const double r03 = r0 * r0 * r0;
+    for (int k = 0; k < n; ++k) {
+      double r, v;
+      if (k & 1) {
+        r = rc * std::cbrt(U(rng));
+        v = std::sqrt(r / rc);
+      } else {
+        v = U(rng);
+        r = rc * v * v;
+      }
+      const double y = func(r) * (r * r * r + r03);
+      hv[k] = v;
+      ht[k] = y;
+      hsw[k] = 1.0 / (std::fabs(y) + 0.05);
+    }
   }
-  // initial parameters
-  std::vector<float> p(mlp::kParams, 0.f);
-  std::uniform_real_distribution<float> Uf(-1.f, 1.f);
-  for (int j = 0; j < mlp::kH; ++j) {
-    p[mlp::kW1 + j] = 4.f * Uf(rng);
-    p[mlp::kB1 + j] = 2.f * Uf(rng);
+  double tmean = 0.0;
+  for (double y : ht) tmean += y;
+  tmean /= n;
+  Fit f;
+  f.alloc(n);
+  f.upload(hv.data(), ht.data(), hsw.data(), b->stream);
+  std::vector<double> best, x;
+  double bestF = HUGE_VAL;
+  for (int s = 0; s < starts; ++s) {
+    mlp_init(rng, tmean, x);
+    const double F = lm_fit(f, x, iters, b->stream);
+    if (F < bestF) {
+      bestF = F;
+      best = x;
+    }
   }
-  const float lim = std::sqrt(6.f / (2 * mlp::kH));
-  for (int k = 0; k < mlp::kH * mlp::kH; ++k) p[mlp::kW2 + k] = lim * Uf(rng);
-  for (int k = 0; k < mlp::kH; ++k) p[mlp::kW3 + k] = lim * Uf(rng);
-  double mean = 0.0;
-  for (float y : ht) mean += y;
-  p[mlp::kB3] = (float)(mean / nset);
-
-  TrainBufs tb;
-  tb_alloc(tb, nset, nset);
-  hipStream_t st = b->stream;
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.v.ptr, hv.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.t.ptr, ht.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.wt.ptr, hw.data(), sizeof(float) * nset, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.w.ptr, p.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
-  std::vector<float> zero(mlp::kParams, 0.f);
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.m.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
-  PGAMD_HIP_CHECK(hipMemcpyAsync(tb.u.ptr, zero.data(), sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, st));
-  // Adam warm-up on mini-batches, then L-BFGS on the full set
-  const int warm = std::min(steps, 1000);
-  const int span = nset - batch + 1;
-  for (int step = 1; step <= warm; ++step) {
-    const int off = (int)(((long long)(step - 1) * batch) % span);
-    mlp_step(tb, batch, off, 1, step, (float)lr, st);
-  }
-  PGAMD_HIP_CHECK(hipMemcpyAsync(p.data(), tb.w.ptr, sizeof(float) * mlp::kParams,
-                                 hipMemcpyDeviceToHost, st));
-  PGAMD_HIP_CHECK(hipStreamSynchronize(st));
-  std::vector<double> x(p.begin(), p.end());
-  const double tot = lbfgs(tb, nset, x, steps, st);
-  for (int k = 0; k < mlp::kParams; ++k) p[k] = (float)x[k];
-  tb_release(tb);
-  g_mlp.params = p;
+  f.release();
+  g_mlp.params = best;
+  g_mlp.form = s2 ? 1 : 0;
   g_mlp.rc = rc;
   g_mlp.r0 = r0;
   g_mlp.valid = true;
   g_mlp.dirty = true;
-  if (loss_out) *loss_out = tot;
+  if (loss_out) *loss_out = bestF / n;
   return 0;
 }
 
-// Learned f(r) at n physical radii (device arrays), synchronously.
+// Learned f(r) at n physical radii (device arrays), decoded as the force
+// kernels do (0 at r = 0 and beyond the cut-off).  Synchronous.
 PGAMD_API int pg5_mlp_eval_dev(int n, const float* r, float* f) {
   Board* b = board(0, "pg5_mlp_eval_dev");
   if (!b) return -1;
@@ -1005,10 +1111,10 @@ PGAMD_API int pg5_mlp_eval_dev(int n, const float* r, float* f) {
     api_error(5, "pg5_mlp_eval_dev: no learned cut-off");
     return -1;
   }
-  const float* w = mlp_dev(b->stream);
+  const mlp::Cut c = mlp_cut(b->stream);
   if (n > 0) {
-    hipLaunchKernelGGL(mlp::mlp_eval_f, dim3(ceil_div(n, 256)), dim3(256), 0, b->stream, n, r, w,
-                       (float)g_mlp.rc, (float)(g_mlp.r0 * g_mlp.r0 * g_mlp.r0), f);
+    hipLaunchKernelGGL(mlp::mlp_eval_f, dim3(ceil_div(n, 256)), dim3(256), 0, b->stream, n, r,
+                       c, (float)g_tab.xscale, f);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
   PGAMD_HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -1042,15 +1148,17 @@ struct BatchGraph {
 }  // namespace
 
 // Short-range force of every group in one persistent launch (treepm_batch.h):
-// pos[np] float4 scaled {x*xs, y*xs, z*xs, m}; group g: i-particles
-// ilist[ioff[g] .. +icnt[g]), j-particles jlist[joff[g] .. +jcnt[g]); order
-// (nullable) = processing order of the groups; acc[np][3] FP64 written for
-// the listed i.  mode 0 = table, 1 = learned MLP.  All device pointers;
-// asynchronous on stream.  Calls on one devid share its work counter.
+// pos[np] float4 scaled {x*xs, y*xs, z*xs, m}; pos64[np][3] (nullable) the
+// same positions unscaled in FP64, for separations formed relative to each
+// group's origin; group g: i-particles ilist[ioff[g] .. +icnt[g]),
+// j-particles jlist[joff[g] .. +jcnt[g]); order (nullable) = processing
+// order of the groups; acc[np][3] FP64 written for the listed i.  mode 0 =
+// table, 1 = learned MLP.  All device pointers; asynchronous on stream.
+// Calls on one devid share its work counter.
 PGAMD_API int pg5t_batch_force(int devid, int ngroups, const int* ioff, const int* icnt,
                                const int* ilist, const int* joff, const int* jcnt,
-                               const int* jlist, const int* order, const void* pos, double* acc,
-                               int mode, void* stream) {
+                               const int* jlist, const int* order, const void* pos,
+                               const double* pos64, double* acc, int mode, void* stream) {
   Board* b = board(devid, "pg5t_batch_force");
   if (!b) return -1;
   std::lock_guard<std::mutex> lk(g_mutex);
@@ -1059,8 +1167,8 @@ PGAMD_API int pg5t_batch_force(int devid, int ngroups, const int* ioff, const in
   hipStream_t st = stream ? (hipStream_t)stream : b->stream;
   b->d_counter.reserve(256);
   launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
-               static_cast<const float4*>(pos), acc, b->d_counter.as<int>(), default_bpc(mode),
-               st);
+               static_cast<const float4*>(pos), pos64, acc, b->d_counter.as<int>(),
+               default_bpc(mode), st);
   return 0;
 }
 
@@ -1070,7 +1178,7 @@ PGAMD_API int pg5t_batch_force(int devid, int ngroups, const int* ioff, const in
 PGAMD_API void* pg5t_batch_graph(int devid, int ngroups, const int* ioff, const int* icnt,
                                  const int* ilist, const int* joff, const int* jcnt,
                                  const int* jlist, const int* order, const void* pos,
-                                 double* acc, int mode) {
+                                 const double* pos64, double* acc, int mode) {
   Board* b = board(devid, "pg5t_batch_graph");
   if (!b) return nullptr;
   std::lock_guard<std::mutex> lk(g_mutex);
@@ -1085,7 +1193,8 @@ PGAMD_API void* pg5t_batch_graph(int devid, int ngroups, const int* ioff, const 
   PGAMD_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
-               static_cast<const float4*>(pos), acc, bg->counter.as<int>(), default_bpc(mode), cs);
+               static_cast<const float4*>(pos), pos64, acc, bg->counter.as<int>(),
+               default_bpc(mode), cs);
   PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &bg->graph));
   PGAMD_HIP_CHECK(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
   PGAMD_HIP_CHECK(hipStreamDestroy(cs));
@@ -1121,7 +1230,8 @@ PGAMD_API int pg5t_batch_graph_destroy(void* h) {
 PGAMD_API int pg5t_batch_autotune(int devid, int ngroups, const int* ioff, const int* icnt,
                                   const int* ilist, const int* joff, const int* jcnt,
                                   const int* jlist, const int* order, const void* pos,
-                                  double* acc, int mode, int reps, int* best, float* ms) {
+                                  const double* pos64, double* acc, int mode, int reps,
+                                  int* best, float* ms) {
   Board* b = board(devid, "pg5t_batch_autotune");
   if (!b) return -1;
   std::lock_guard<std::mutex> lk(g_mutex);
@@ -1139,7 +1249,8 @@ PGAMD_API int pg5t_batch_autotune(int devid, int ngroups, const int* ioff, const
   for (int k = 0; k < 8; ++k) {
     auto go = [&] {
       launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
-                   static_cast<const float4*>(pos), acc, b->d_counter.as<int>(), cand[k], st);
+                   static_cast<const float4*>(pos), pos64, acc, b->d_counter.as<int>(), cand[k],
+                   st);
     };
     go();
     PGAMD_HIP_CHECK(hipEventRecord(e0, st));

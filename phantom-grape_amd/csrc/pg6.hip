@@ -20,6 +20,8 @@
 #include "cluster_batch.h"
 #include "hermite_kernels.h"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -43,7 +45,9 @@ struct LaunchPlan {
   int nsplit;   // j-ranges
   int jper;     // j-particles per range
   int variant;  // 0: j-tiles in LDS (pg6_force), 1: j broadcast in SGPRs (pg6_force_s),
-                // 2: as 1 with two-float positions and the self filter (pg6_force_h)
+                // 2: as 1 with two-float positions and the self filter (pg6_force_h),
+                // 3: one launch with the predictor and the reduction inside (pg6_force_f),
+                // 4: the sorted layout with chunk origins (pg6_force_o)
   int bpc;      // target blocks (of 256 threads) per CU
 };
 
@@ -53,6 +57,13 @@ struct LaunchChoice {
   int variant, bi, bpc;
 };
 constexpr int kBuckets = 11;
+constexpr int kFusedMaxNi = 1024;  // pg6_force_f (variant 3) up to this many i
+// pg6_force_o (variant 4) is chosen automatically only from this many j up:
+// its positions are FP32 offsets from the origin of their 64-particle chunk,
+// so its dx error grows with the chunk extent, which is large where the
+// j-set is sparse (p99 relative force error about 1e-6 on a 4096-particle
+// Plummer sphere, 1.5e-7 at 65536).
+constexpr int kSortedMinNj = 16384;
 
 inline int ni_bucket(int ni) {
   int k = 0;
@@ -65,7 +76,7 @@ inline int ni_bucket(int ni) {
 // kernel for large ones, and many j-ranges so that the last wave of blocks is
 // a small fraction of the launch.
 inline LaunchChoice default_choice(int ni) {
-  if (ni <= 128) return {0, 64, 32};
+  if (ni <= 256) return {3, 64, 4};
   if (ni <= 4096) return {0, 64, 8};
   if (ni <= 8192) return {2, 128, 16};
   if (ni <= 32768) return {2, 128, 32};
@@ -94,11 +105,28 @@ struct Board {
   hipStream_t last_st = nullptr;
   hipEvent_t order_ev = nullptr;
   double ti = 0.0;
+  // predict_flag: the predicted j-set is stale (g6_set_ti since the last
+  // prediction); the prediction runs with the first force that needs it (in
+  // the force kernel itself for pg6_force_f).  pred_armed: a firsthalf (or
+  // pg6_dev_predict) asked for it at pred_nj j-particles, so j-records flushed
+  // before that force must not enter it (the reference predicts in firsthalf,
+  // PG6/libpg6/phantom_g6.c:153-164): the flush predicts first.
   int predict_flag = 0;
+  bool pred_armed = false;
+  int pred_nj = 0;
   int nj_pred = -1;  // j count of the last prediction
   // i-set and outputs of the host API
   HostBuf h_in, h_out;
   DevBuf d_in, d_part;
+  DevBuf d_fcount;  // pg6_force_f's per-i-block completion counters (kept at zero)
+  // Sorted j layout of pg6_force_o (pg6::JSort): built from the predicted
+  // positions when missing or older than kSortAge predictions, its records
+  // rewritten by every prediction made while it is in use (srec_ok: they hold
+  // the current prediction).
+  DevBuf d_sort, d_sort_tmp;
+  pg6::JSort so{};
+  int sort_nj = 0, sort_age = 0;
+  bool srec_ok = false;
   // neighbour lists of the last lasthalf*n call
   DevBuf d_nb;
   std::vector<int> nbl, nblen;
@@ -262,6 +290,9 @@ void board_close(Board& b) {
   b.jp_buf.release();
   b.d_in.release();
   b.d_part.release();
+  b.d_fcount.release();
+  b.d_sort.release();
+  b.d_sort_tmp.release();
   b.d_nb.release();
   b.pend.release();
   b.h_in.release();
@@ -303,8 +334,11 @@ void board_queue_j(Board& b, int padr, int pidx, double t, double m, const doubl
   r.addr = padr;
 }
 
+void board_predict(Board& b, int nj, hipStream_t st, bool sorted = false);
+
 void board_flush_j(Board& b) {
   if (b.npend == 0) return;
+  if (b.predict_flag && b.pred_armed) board_predict(b, b.pred_nj, b.stream);
   board_use_stream(b, b.stream);
   // the scatter kernel reads the pinned queue through its device mapping
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
@@ -319,14 +353,76 @@ void board_flush_j(Board& b) {
   b.npend = 0;
 }
 
-void board_predict(Board& b, int nj, hipStream_t st) {
+void board_predict(Board& b, int nj, hipStream_t st, bool sorted) {
+  b.predict_flag = 0;
+  b.pred_armed = false;
   if (nj <= 0) return;
   board_use_stream(b, st);
+  // the sorted records are rewritten only while the layout fits this nj
+  const bool srt = sorted && b.sort_nj == nj;
+  pg6::JSort so = b.so;
+  if (!srt) so.rec = nullptr;
   hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
-                     b.js, b.jp);
+                     b.js, b.jp, so);
   PGAMD_HIP_CHECK(hipGetLastError());
   board_done_on(b, st);
   b.nj_pred = nj;
+  b.srec_ok = srt;
+  if (srt) ++b.sort_age;
+}
+
+constexpr int kSortAge = 64;  // predictions between rebuilds of the sorted layout
+
+// (Re)builds the sorted layout of the first nj predicted j-particles on st
+// (asynchronously): box of the positions, Morton keys, radix sort, then the
+// inverse map, chunk origins and records (pg6_sort_finish).
+void board_build_sort(Board& b, int nj, hipStream_t st) {
+  const int nc = ceil_div(nj, pg6::kChunk);
+  const int nbox = std::min(256, ceil_div(nj, 256));
+  // perm, inv (nj each), keys (2 nj u64), vals (2 nj), rec (2 nj float4), org (3 nc),
+  // flags, box partials (6 nbox)
+  const size_t bytes = sizeof(int) * 2 * (size_t)nj + sizeof(unsigned long long) * 2 * (size_t)nj +
+                       sizeof(int) * 2 * (size_t)nj + sizeof(float4) * 2 * (size_t)nj +
+                       sizeof(double) * (3 * (size_t)nc + 6 * (size_t)nbox + 4) + 8 * 256;
+  b.d_sort.reserve(bytes);
+  char* base = static_cast<char*>(b.d_sort.ptr);
+  auto take = [&](size_t n) {
+    char* p = base;
+    base += (n + 255) / 256 * 256;
+    return p;
+  };
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * (size_t)nj));
+  int* vals = reinterpret_cast<int*>(take(sizeof(int) * 2 * (size_t)nj));
+  b.so.perm = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nj));
+  b.so.inv = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nj));
+  b.so.rec = reinterpret_cast<float4*>(take(sizeof(float4) * 2 * (size_t)nj));
+  b.so.org = reinterpret_cast<double*>(take(sizeof(double) * 3 * (size_t)nc));
+  double* boxp = reinterpret_cast<double*>(take(sizeof(double) * 6 * (size_t)nbox));
+  double* box = reinterpret_cast<double*>(take(sizeof(double) * 4));
+  b.so.flags = reinterpret_cast<int*>(take(sizeof(int)));
+  hipLaunchKernelGGL(pg6::pg6_box_part, dim3(nbox), dim3(256), 0, st, nj, b.jp, boxp);
+  hipLaunchKernelGGL(pg6::pg6_box_final, dim3(1), dim3(256), 0, st, nbox, boxp, box);
+  hipLaunchKernelGGL(pg6::pg6_sort_keys, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.jp,
+                     box, keys, vals);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  size_t tmp = 0;
+  PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys + nj, vals,
+                                                     b.so.perm, nj, 0, 63, st));
+  b.d_sort_tmp.reserve(tmp);
+  PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.d_sort_tmp.ptr, tmp, keys, keys + nj,
+                                                     vals, b.so.perm, nj, 0, 63, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(b.so.flags, 0, sizeof(int), st));
+  hipLaunchKernelGGL(pg6::pg6_sort_finish, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.jp,
+                     b.so);
+  PGAMD_HIP_CHECK(hipGetLastError());
+  b.sort_nj = nj;
+  b.sort_age = 0;
+  b.srec_ok = true;
+}
+
+// The predicted j-set is needed for the first nj j-particles.
+inline bool board_needs_pred(const Board& b, int nj) {
+  return nj > 0 && (b.predict_flag || b.nj_pred < nj);
 }
 
 LaunchPlan plan_launch(const Board& b, int ni, int nj) {
@@ -336,9 +432,14 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   p.bi = b.force_bi > 0 ? b.force_bi : c.bi;
   p.variant = b.force_variant >= 0 ? b.force_variant : c.variant;
   p.bpc = b.force_bpc > 0 ? b.force_bpc : c.bpc;
+  if (p.variant == 3 && ni > kFusedMaxNi) p.variant = 2;
+  if (p.variant == 4 && b.force_variant != 4 && nj < kSortedMinNj) p.variant = 2;
+  if (p.variant == 3) p.bi = 64;
   const int nbi = ceil_div(ni, p.bi);
-  // the SGPR-broadcast kernel runs bi threads per block, the LDS one 256
-  const int target = device_cu_count() * p.bpc * (p.variant >= 1 ? 256 / p.bi : 1);
+  // the SGPR-broadcast kernels run bi threads per block, the LDS one 256, the
+  // one-launch kernel 512 (bpc counts quarters of a block per CU there)
+  const int target = p.variant == 3 ? std::max(1, device_cu_count() * p.bpc / 4)
+                                    : device_cu_count() * p.bpc * (p.variant >= 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
   const int maxs = std::max(1, nj / std::max(1, b.min_jper));
   ns = std::min(ns, maxs);
@@ -353,11 +454,14 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
 
 template <int BI, bool NN>
 void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const double* ivel,
-                    const int* iid, float eps2, int nj, const JPred& jp, const Partials& part,
-                    const Outputs& out, hipStream_t st) {
+                    const int* iid, float eps2, int nj, const JPred& jp, const pg6::JSort& so,
+                    const Partials& part, const Outputs& out, hipStream_t st) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
   g_timer.before(st);
-  if (p.variant == 2)  // plan_launch keeps jper a multiple of pg6::kChunk
+  if (p.variant == 4)  // ranges of sorted positions, multiples of pg6::kChunk
+    hipLaunchKernelGGL((pg6::pg6_force_o<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
+                       nj, p.jper, so, jp, part, out, p.nsplit);
+  else if (p.variant == 2)  // plan_launch keeps jper a multiple of pg6::kChunk
     hipLaunchKernelGGL((pg6::pg6_force_h<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
                        nj, p.jper, jp, part, out, p.nsplit);
   else if (p.variant == 1)
@@ -394,6 +498,21 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   if (ni <= 0) return;
   board_use_stream(b, st);
   const LaunchPlan p = plan_launch(b, ni, nj > 0 ? nj : 1);
+  const bool pred = board_needs_pred(b, nj);
+  if (p.variant == 4) {
+    // the sorted layout of the current prediction
+    const bool fresh = b.sort_nj == nj && b.sort_age < kSortAge;
+    if (pred || !b.srec_ok) {
+      if (fresh) {
+        board_predict(b, nj, st, true);
+      } else {
+        if (pred) board_predict(b, nj, st);
+        board_build_sort(b, nj, st);
+      }
+    }
+  } else if (pred && p.variant != 3) {
+    board_predict(b, nj, st);
+  }
   Partials part{};
   if (p.nsplit > 1) {
     const size_t n = (size_t)p.nsplit * ni;
@@ -412,12 +531,41 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   const bool nn = out.nnb != nullptr;
   const float e2 = (float)eps2;
   const int njr = nj > 0 ? nj : 0;
+  if (p.variant == 3) {
+    // one launch: prediction (if due) and j-range reduction inside
+    const int nib = ceil_div(ni, 64);
+    if (b.d_fcount.bytes < sizeof(int) * (size_t)nib) {
+      b.d_fcount.reserve(sizeof(int) * (size_t)std::max(nib, 1024));
+      PGAMD_HIP_CHECK(hipMemsetAsync(b.d_fcount.ptr, 0, b.d_fcount.bytes, st));
+    }
+    int* cnt = b.d_fcount.as<int>();
+    dim3 grid(nib, p.nsplit);
+    g_timer.before(st);
+#define PG6_FUSED(NN, PRED)                                                                  \
+    hipLaunchKernelGGL((pg6::pg6_force_f<NN, PRED>), grid, dim3(pg6::kFThreads), 0, st, ni,  \
+                       ipos, ivel, iid, e2, njr, p.jper, b.ti, b.js, b.jp, part, out,        \
+                       p.nsplit, cnt)
+    if (nn && pred) PG6_FUSED(true, true);
+    else if (nn) PG6_FUSED(true, false);
+    else if (pred) PG6_FUSED(false, true);
+    else PG6_FUSED(false, false);
+#undef PG6_FUSED
+    PGAMD_HIP_CHECK(hipGetLastError());
+    g_timer.after(st);
+    if (pred) {
+      b.nj_pred = nj;
+      b.predict_flag = 0;
+      b.pred_armed = false;
+    }
+    board_done_on(b, st);
+    return;
+  }
 #define PG6_LAUNCH(BI)                                                                     \
   do {                                                                                     \
     if (nn)                                                                                \
-      launch_force_t<BI, true>(p, ni, ipos, ivel, iid, e2, njr, b.jp, part, out, st);      \
+      launch_force_t<BI, true>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st); \
     else                                                                                   \
-      launch_force_t<BI, false>(p, ni, ipos, ivel, iid, e2, njr, b.jp, part, out, st);     \
+      launch_force_t<BI, false>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st);\
   } while (0)
   switch (p.bi) {
     case 256: PG6_LAUNCH(256); break;
@@ -445,12 +593,9 @@ int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double 
     return -1;
   }
   if (ni == 0) return 0;
-  // Pending j records or a missing prediction: do what firsthalf would.
+  // Pending j records (a firsthalf's prediction, if armed, runs first); a
+  // missing prediction runs with the force.
   board_flush_j(b);
-  if (b.predict_flag || b.nj_pred < nj) {
-    board_predict(b, nj, b.stream);
-    b.predict_flag = 0;
-  }
   // i-set: pos[3] double, vel[3] double, id int, h2 float
   const size_t in_bytes = (size_t)ni * (6 * sizeof(double) + sizeof(int) + sizeof(float));
   b.h_in.reserve(in_bytes);
@@ -597,9 +742,9 @@ PGAMD_API void g6calc_firsthalf(int gpid, int nj, int ni, int* pidx, double (*po
     return;
   }
   board_flush_j(b);
-  if (b.predict_flag) {
-    board_predict(b, nj, b.stream);
-    b.predict_flag = 0;
+  if (b.predict_flag) {  // predicted with the first force that needs it
+    b.pred_armed = true;
+    b.pred_nj = nj;
   }
   b.nblerror = 0;
 }
@@ -782,9 +927,14 @@ PGAMD_API int pg6_dev_predict(int nj, double ti, void* stream) {
     api_error(2, "pg6_dev_predict: nj=%d outside the j-memory (capacity %d)", nj, g_board.cap);
     return -1;
   }
+  // Lazy: the first force after this (or a flush of new j-records) predicts;
+  // pg6_force_f does it inside its own launch.  `stream` orders nothing here.
+  (void)stream;
+  board_flush_j(g_board);
   g_board.ti = ti;
-  g_board.predict_flag = 0;
-  board_predict(g_board, nj, stream ? (hipStream_t)stream : g_board.stream);
+  g_board.predict_flag = 1;
+  g_board.pred_armed = true;
+  g_board.pred_nj = nj;
   return 0;
 }
 
@@ -870,6 +1020,8 @@ PGAMD_API int pg6_dev_read_predicted(int nj, double* pos, float* vel, float* mas
   if (nj == 0) return 0;
   std::vector<double> x((size_t)nj), y((size_t)nj), z((size_t)nj);
   std::vector<float4> vm((size_t)nj);
+  board_flush_j(b);
+  if (board_needs_pred(b, nj)) board_predict(b, nj, b.stream);
   board_use_stream(b, b.stream);
   PGAMD_HIP_CHECK(hipStreamSynchronize(b.stream));
   PGAMD_HIP_CHECK(hipMemcpy(x.data(), b.jp.x, sizeof(double) * nj, hipMemcpyDeviceToHost));
@@ -1083,13 +1235,17 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
 // Force kernel choice: 0 = j-particles staged in LDS tiles (pg6_force),
 // 1 = j-particles broadcast through scalar registers (pg6_force_s), 2 = as 1
 // with two-float positions and the chunked self-pair filter (pg6_force_h),
-// -1 = the autotuned or default choice for each i-count.  0 and 1 compute
-// the same arithmetic; 2 forms dx in FP32 (within about one FP32 ulp of the
-// others).
+// 3 = prediction, force and j-range reduction in one launch (pg6_force_f, up
+// to kFusedMaxNi i; more fall back to 2), 4 = j-particles in Morton order as
+// FP32 offsets from their chunk's origin (pg6_force_o; its dx error scales
+// with the chunk extent, so the automatic choice takes it only from
+// kSortedMinNj j up; set here it runs at any nj), -1 = the autotuned or
+// default choice for each i-count.  0 and 1 compute the same arithmetic; 2
+// and 3 form dx in FP32 (within about one FP32 ulp of the others).
 PGAMD_API int pg6_dev_set_kernel(int variant) {
   std::lock_guard<std::mutex> lk(g_mutex);
-  if (variant < -1 || variant > 2) {
-    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1, 2}", variant);
+  if (variant < -1 || variant > 4) {
+    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1, 2, 3, 4}", variant);
     return -1;
   }
   g_board.force_variant = variant;
@@ -1126,11 +1282,12 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!board_check_open("pg6_dev_autotune")) return -1;
   Board& b = g_board;
-  if (nj < 64 || nj > b.cap || b.nj_pred < nj) {
-    api_error(3, "pg6_dev_autotune: nj=%d must be >= 64 and predicted (last prediction %d)",
-              nj, b.nj_pred);
+  if (nj < 64 || nj > b.cap) {
+    api_error(3, "pg6_dev_autotune: nj=%d must be in [64, %d]", nj, b.cap);
     return -1;
   }
+  board_flush_j(b);
+  if (board_needs_pred(b, nj)) board_predict(b, nj, b.stream);
   if (reps <= 0) reps = 3;
   const int nimax = std::min(nj, 65536);
   DevBuf buf;
@@ -1157,14 +1314,18 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   g_timer.on = false;
   const int save_bi = b.force_bi, save_bpc = b.force_bpc, save_var = b.force_variant;
   static const int kBpc[] = {4, 8, 16, 32, 64};
+  static const int kBpcF[] = {1, 2, 4, 8, 16};  // variant 3: 64 .. 1024 blocks
   int tuned = 0;
   for (int k = 0; k < kBuckets && (64 << k) <= nimax; ++k) {
     const int ni = 64 << k;
     float best = 1e30f;
     LaunchChoice bc = default_choice(ni);
-    for (int var = 0; var <= 2; ++var)
+    for (int var = 0; var <= 4; ++var)
       for (int bi : {64, 128, 256})
-        for (int bpc : kBpc) {
+        for (int bk = 0; bk < 5; ++bk) {
+          const int bpc = var == 3 ? kBpcF[bk] : kBpc[bk];
+          if (var == 3 && (bi != 64 || ni > kFusedMaxNi)) continue;
+          if (var == 4 && nj < kSortedMinNj) continue;
           b.force_variant = var;
           b.force_bi = bi;
           b.force_bpc = bpc;
@@ -1232,8 +1393,9 @@ struct Hermite {
   int* iid = nullptr;
   double* epart = nullptr;  // energy partial sums, 2 per block
   Partials part{};
-  int target = 0, min_jper = 16;  // force split (pg6h_plan)
-  int min_blocks = 0, force_blocks = 0, finish_blocks = 0;
+  int target = 0, min_jper = 16, max_ns = 32;  // force split (pg6h_plan)
+  int min_blocks = 0, force_blocks = 0;
+  int* tile_done = nullptr;  // per i-tile range counters of pg6h_force_finish (kept at 0)
   HostBuf host;  // pinned: a Ctl image, then t_end
   // a captured batch of kGraphSteps steps, and the board arrays it was captured with
   hipGraphExec_t graph = nullptr;
@@ -1261,17 +1423,16 @@ void hermite_force(Board& b, Hermite& h, int ni) {
   board_force(b, ni, h.n, h.ipos, h.ivel, h.iid, h.eps2, o, b.stream);
 }
 
-// Queues one block step (three kernels, no host synchronisation).
+// Queues one block step (two kernels, no host synchronisation).
 void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
   const int n = h.n;
   board_use_stream(b, st);
   hipLaunchKernelGGL(pg6h::pg6h_select, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
                      h.ctl, b.js, b.jp, h.act, h.ipos, h.ivel, h.iid);
-  hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(64 * pg6h::kForceWaves), 0, st,
-                     n, (float)h.eps2,
-                     h.P, h.ctl, h.target, h.min_jper, h.ipos, h.ivel, h.iid, b.jp, h.part);
-  hipLaunchKernelGGL(pg6h::pg6h_finish, dim3(h.finish_blocks), dim3(256), 0, st, n, h.P, h.ctl,
-                     h.target, h.min_jper, h.act, h.S, h.ipos, h.ivel, h.part, b.js);
+  hipLaunchKernelGGL(pg6h::pg6h_force_finish, dim3(h.force_blocks),
+                     dim3(64 * pg6h::kForceWaves), 0, st, n, (float)h.eps2, h.P, h.ctl,
+                     h.target, h.min_jper, h.max_ns, h.act, h.S, h.ipos, h.ivel, h.iid, b.jp,
+                     b.js, h.part, h.tile_done);
   PGAMD_HIP_CHECK(hipGetLastError());
   b.nj_pred = -1;  // the board's predictions now belong to the integrator
   b.predict_flag = 1;
@@ -1354,9 +1515,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.target = 16 * cus;  // wavefront items per force launch: 4 per SIMD
   h.min_blocks = std::min(ceil_div(n, 256), 4 * cus);
   h.force_blocks = 4 * cus * 4 / pg6h::kForceWaves;  // 4 wavefronts per SIMD
-  // blocks of 256 threads: a wavefront per active particle of a typical block
-  // step (a few hundred), and few blocks to count in Ctl::done
-  h.finish_blocks = std::max(1, std::min(ceil_div(n, 256), cus));
+  if (const char* e = std::getenv("PGAMD_HERMITE_MAX_NS")) h.max_ns = std::max(1, std::atoi(e));
   // largest force split: ns * ntiles <= target / kForceWaves + ntiles (pg6h_plan)
   const size_t nsplit_i = ((size_t)h.target / pg6h::kForceWaves + ceil_div(n, 64)) * 64;
   // state: x, v, a, j (3n each), pot, t, dt, m (n each)
@@ -1374,7 +1533,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   // nsplit_i), Ctl; act, iid (n each)
   const size_t dbytes =
       sizeof(double) * ((size_t)n * 13 + 2 * kHermBlocks + 7 * nsplit_i) + 256;
-  const size_t ibytes = sizeof(int) * ((size_t)n * 2);
+  const size_t ibytes = sizeof(int) * ((size_t)n * 2 + ceil_div(n, 64));
   h.work.reserve(dbytes + ibytes);
   double* w = h.work.as<double>();
   h.ipos = w;
@@ -1394,6 +1553,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   int* wi = reinterpret_cast<int*>(reinterpret_cast<char*>(h.ctl) + 256);
   h.act = wi;
   h.iid = wi + n;
+  h.tile_done = wi + 2 * (size_t)n;
   h.host.reserve(512);
   hipStream_t st = b.stream;
   // upload x, v, m; t = 0, a = j = 0
@@ -1403,6 +1563,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.a, 0, sizeof(double) * 6 * (size_t)n, st));
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.t, 0, sizeof(double) * (size_t)n, st));
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.dt, 0, sizeof(double) * (size_t)n, st));
+  PGAMD_HIP_CHECK(hipMemsetAsync(h.tile_done, 0, sizeof(int) * (size_t)ceil_div(n, 64), st));
   // j-memory = the particles at t = 0 (set_particle_on_grape, banana01.c:206-236):
   // the start-up kernel below writes it; the i-set is every particle as is.
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.ipos, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));

@@ -102,11 +102,12 @@ __device__ inline double pg6_pred_vel(double dt, double v, double a2, double j6)
   return v + (2.0 * a2) * dt + (3.0 * j6) * dt * dt;
 }
 
-// Predicts j-particle j (valid = j < nj) to time ti into every JPred array.
+// Predicts j-particle j (valid = j < nj) to time ti into every JPred array,
+// and returns its two-float record in rec[3] (left unset if !valid).
 // Called by whole wavefronts whose lane l holds j = 64 w + l, so that the
 // wavefront fills in the id range of j-chunk w.
 __device__ inline void pg6_predict_j(int j, bool valid, double ti, const JState& js,
-                                     const JPred& jp) {
+                                     const JPred& jp, float4* rec = nullptr) {
   int id = 0;
   if (valid) {
     const double dt = ti - js.t[j];
@@ -126,11 +127,19 @@ __device__ inline void pg6_predict_j(int j, bool valid, double ti, const JState&
     jp.id[j] = id;
     const float hx = (float)x, hy = (float)y, hz = (float)z;
     float4* r = jp.rec + 3 * (size_t)j;
-    r[0] = make_float4(hx, hy, hz, vm.w);
+    const float4 r0 = make_float4(hx, hy, hz, vm.w);
     // x - (double)hx is exact in FP64
-    r[1] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz),
-                       vm.x);
-    r[2] = make_float4(vm.y, vm.z, __int_as_float(id), vm.w);
+    const float4 r1 = make_float4((float)(x - (double)hx), (float)(y - (double)hy),
+                                  (float)(z - (double)hz), vm.x);
+    const float4 r2 = make_float4(vm.y, vm.z, __int_as_float(id), vm.w);
+    r[0] = r0;
+    r[1] = r1;
+    r[2] = r2;
+    if (rec) {
+      rec[0] = r0;
+      rec[1] = r1;
+      rec[2] = r2;
+    }
   }
   int lo = valid ? id : INT_MAX, hi = valid ? id : INT_MIN;
   for (int o = 32; o > 0; o >>= 1) {
@@ -141,10 +150,151 @@ __device__ inline void pg6_predict_j(int j, bool valid, double ti, const JState&
 }
 
 // Block = 256 threads, one j per thread.
-__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp) {
+// Sorted j layout for pg6_force_o: the predicted j-set in the order of a
+// Morton sort of their positions (taken when the layout was built), in
+// chunks of kChunk; chunk c carries an FP64 origin org[3c..3c+2] (the centre
+// of its members' box when built) and member k (sorted position) the record
+// rec[2k] = {x - o, y - o, z - o (FP32), m}, rec[2k+1] = {vx, vy, vz, id bits}.
+// perm[k] = j address of sorted position k, inv[j] = k; flags != 0 when some
+// id differs from its address (then pg6_force_o tests the self pair by id on
+// every chunk).
+struct JSort {
+  int* perm;
+  int* inv;
+  float4* rec;
+  double* org;
+  int* flags;
+};
+
+__device__ inline void pg6_sorted_rec(const JSort& so, int k, double x, double y, double z,
+                                      float4 vm, int id, const double* o) {
+  so.rec[2 * (size_t)k] = make_float4((float)(x - o[0]), (float)(y - o[1]), (float)(z - o[2]),
+                                      vm.w);
+  so.rec[2 * (size_t)k + 1] = make_float4(vm.x, vm.y, vm.z, __int_as_float(id));
+}
+
+// Block = 256 threads, one j per thread.  so.rec != null: the sorted records
+// of the predicted j are written too (through so.inv).
+__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp,
+                                                   JSort so) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   pg6_predict_j(j, j < nj, ti, js, jp);
+  if (so.rec && j < nj) {
+    const int k = so.inv[j];
+    pg6_sorted_rec(so, k, jp.x[j], jp.y[j], jp.z[j], jp.vm[j], jp.id[j],
+                   so.org + 3 * (size_t)(k / kChunk));
+  }
 }
+
+// ---------------------------------------------------------- sorted layout
+// Box of the predicted positions: per block min and max of x, y, z in
+// part[6 b .. 6 b + 5] (a host pass finishes it).
+__global__ __launch_bounds__(256) void pg6_box_part(int nj, JPred jp, double* __restrict__ part) {
+  __shared__ double s[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
+    const double p[3] = {jp.x[j], jp.y[j], jp.z[j]};
+    for (int d = 0; d < 3; ++d) {
+      v[d] = fmin(v[d], p[d]);
+      v[3 + d] = fmax(v[3 + d], p[d]);
+    }
+  }
+  for (int d = 0; d < 6; ++d) s[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int d = 0; d < 6; ++d)
+        s[d][threadIdx.x] = d < 3 ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + h])
+                                  : fmax(s[d][threadIdx.x], s[d][threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__device__ inline unsigned long long morton_spread(unsigned long long a) {
+  a &= 0x1fffffull;
+  a = (a | (a << 32)) & 0x1f00000000ffffull;
+  a = (a | (a << 16)) & 0x1f0000ff0000ffull;
+  a = (a | (a << 8)) & 0x100f00f00f00f00full;
+  a = (a | (a << 4)) & 0x10c30c30c30c30c3ull;
+  a = (a | (a << 2)) & 0x1249249249249249ull;
+  return a;
+}
+
+// The box from nbox block partials: box[0..2] = lower corner, box[3] =
+// 2^21 / (largest side), the Morton grid scale.  One block of 256 threads.
+__global__ __launch_bounds__(256) void pg6_box_final(int nbox, const double* __restrict__ part,
+                                                     double* __restrict__ box) {
+  __shared__ double s[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int k = threadIdx.x; k < nbox; k += 256)
+    for (int d = 0; d < 6; ++d)
+      v[d] = d < 3 ? fmin(v[d], part[6 * k + d]) : fmax(v[d], part[6 * k + d]);
+  for (int d = 0; d < 6; ++d) s[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int d = 0; d < 6; ++d)
+        s[d][threadIdx.x] = d < 3 ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + h])
+                                  : fmax(s[d][threadIdx.x], s[d][threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double L = fmax(fmax(s[3][0] - s[0][0], s[4][0] - s[1][0]), fmax(s[5][0] - s[2][0], 1e-300));
+    box[0] = s[0][0];
+    box[1] = s[1][0];
+    box[2] = s[2][0];
+    box[3] = 2097152.0 / (L * (1.0 + 1e-9));
+  }
+}
+
+// 63-bit Morton key of each predicted position in the box of pg6_box_final.
+__global__ void pg6_sort_keys(int nj, JPred jp, const double* __restrict__ box,
+                              unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const double scale = box[3];
+  const double p[3] = {jp.x[j] - box[0], jp.y[j] - box[1], jp.z[j] - box[2]};
+  unsigned long long k = 0;
+  for (int d = 0; d < 3; ++d) {
+    double u = p[d] * scale;
+    u = u < 0.0 ? 0.0 : (u > 2097151.0 ? 2097151.0 : u);
+    k |= morton_spread((unsigned long long)u) << (2 - d);
+  }
+  keys[j] = k;
+  vals[j] = j;
+}
+
+// After the sort (perm = addresses in key order): inverse map, chunk origins
+// (centre of each chunk's box, one wavefront per chunk), the id check, and
+// the sorted records of the current prediction.  Block = 256 threads.
+__global__ __launch_bounds__(256) void pg6_sort_finish(int nj, JPred jp, JSort so) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool v = k < nj;
+  const int j = v ? so.perm[k] : 0;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  double p[3] = {0.0, 0.0, 0.0};
+  if (v) {
+    so.inv[j] = k;
+    p[0] = jp.x[j];
+    p[1] = jp.y[j];
+    p[2] = jp.z[j];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = p[d];
+    if (jp.id[j] != j) atomicOr(so.flags, 1);
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+    }
+  // blocks of 256 threads hold four whole chunks of kChunk = 64; after the
+  // butterfly every lane has its chunk's box
+  const double o[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+  if ((k % kChunk) == 0 && v)
+    for (int d = 0; d < 3; ++d) so.org[3 * (size_t)(k / kChunk) + d] = o[d];
+  if (v) pg6_sorted_rec(so, k, p[0], p[1], p[2], jp.vm[j], jp.id[j], o);
+}
+
 
 // Nearest-neighbour key of one pair: the top 24 bits of 1/r (a larger key is
 // a nearer j) with the tile slot 255 - jj in the low byte, so that max()
@@ -606,6 +756,129 @@ __global__ __launch_bounds__(kThreads) void pg6_force_h(
   }
 }
 
+// Pair loop of pg6_force_o for the i-particle of this lane against the
+// sorted positions [k0, k1) of the sorted layout (JSort):
+//  * dx = (u_j - d_i) - l_i, two FP32 operations: u_j = x_j - o_c (FP32,
+//    stored), d_i = FP32(x_i - o_c) and l_i = FP32((x_i - o_c) - d_i) formed
+//    in FP64 once per chunk and lane.  The i offset is exact to FP32 order
+//    in l_i; the j offset's rounding is at most 2^-24 |x_j - o_c|, half an ulp
+//    of the chunk's size, so a close pair keeps its digits when the chunk is
+//    small (the Morton order keeps it so);
+//  * the self pair is tested by id only on chunks that hold this wavefront's
+//    i itself (self_k = the i's sorted position, from inv when ids are the
+//    addresses; -1 with flags set: every chunk is tested).
+// FP32 sums over tiles of kJTile positions, folded into FP64.  rmax/nn as
+// pg6_range_h, nn converted to the j address.  Every lane of the wavefront
+// calls it with the same k0, k1 (the records are read by scalar loads).
+template <bool NN>
+__device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi, float vxi,
+                                                 float vyi, float vzi, int idi, int self_k,
+                                                 bool test_all, float eps2, int k0, int k1,
+                                                 const JSort& so) {
+  cdouble* porg = (cdouble*)so.org;
+  RangeSums r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f, -1};
+  const unsigned kmask = 0xFFFFFF00u;
+  const int self_c = self_k >= 0 ? self_k / kChunk : -1;
+  for (int kt = k0; kt < k1; kt += kJTile) {
+    const int nt = min(kJTile, k1 - kt);
+    TileAcc t{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u};
+    for (int c = 0, ce; c < nt; c = ce) {
+      const int ch = (kt + c) / kChunk;
+      ce = min(nt, (ch + 1) * kChunk - kt);  // to the end of the chunk or the tile
+      const double ox = porg[3 * ch], oy = porg[3 * ch + 1], oz = porg[3 * ch + 2];
+      const double ex = xi - ox, ey = yi - oy, ez = zi - oz;
+      const float dxi = (float)ex, dyi = (float)ey, dzi = (float)ez;
+      const float lxi = (float)(ex - (double)dxi);
+      const float lyi = (float)(ey - (double)dyi);
+      const float lzi = (float)(ez - (double)dzi);
+      const bool self_here = test_all || __builtin_amdgcn_ballot_w64(self_c == ch) != 0ull;
+      cfloat_s* q = (cfloat_s*)(so.rec + 2 * (size_t)(kt + c));
+#define PG6_PAIR_O(SELF)                                                                   \
+  {                                                                                        \
+    const float dx = (q[0] - dxi) - lxi;                                                   \
+    const float dy = (q[1] - dyi) - lyi;                                                   \
+    const float dz = (q[2] - dzi) - lzi;                                                   \
+    pg6_pair_h<SELF, NN>(t, dx, dy, dz, q[4] - vxi, q[5] - vyi, q[6] - vzi, q[3], eps2,    \
+                         SELF && __float_as_int(q[7]) == idi, (unsigned)(255 - jj));       \
+  }
+      if (self_here) {
+#pragma unroll 4
+        for (int jj = c; jj < ce; ++jj, q += 8) PG6_PAIR_O(true)
+      } else {
+#pragma unroll 4
+        for (int jj = c; jj < ce; ++jj, q += 8) PG6_PAIR_O(false)
+      }
+#undef PG6_PAIR_O
+    }
+    if (NN) {
+      const float rq = __uint_as_float(t.key & kmask);
+      if (rq > r.rmax) {
+        r.rmax = rq;
+        r.nn = kt + 255 - (int)(t.key & 0xFFu);
+      }
+    }
+    r.ax += (double)t.ax;
+    r.ay += (double)t.ay;
+    r.az += (double)t.az;
+    r.pot += (double)t.pot;
+    r.jx += (double)t.jx;
+    r.jy += (double)t.jy;
+    r.jz += (double)t.jz;
+  }
+  if (NN && r.nn >= 0) r.nn = so.perm[r.nn];
+  return r;
+}
+
+// pg6_force_h on the sorted layout (pg6_range_o): one i-particle per lane,
+// blocks of BI threads, blockIdx.y = range of sorted positions.  Same outputs
+// and partials as pg6_force_h (nearest neighbours by j address).
+template <bool NN>
+__global__ __launch_bounds__(kThreads) void pg6_force_o(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JSort so, JPred jp, Partials part,
+    Outputs out, int nsplit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;  // padding lanes repeat i = ni-1
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const int idi = iid[ic];
+  const bool test_all = *(volatile const int*)so.flags != 0;
+  const int self_k = (!test_all && idi >= 0 && idi < nj) ? so.inv[idi] : -1;
+  const int j0 = blockIdx.y * jper;
+  const RangeSums r = pg6_range_o<NN>(xi, yi, zi, (float)ivel[3 * ic + 0],
+                                      (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], idi,
+                                      self_k, test_all, eps2, j0, min(nj, j0 + jper), so);
+  if (!ivalid) return;
+  if (nsplit == 1) {
+    out.acc[3 * i + 0] = r.ax;
+    out.acc[3 * i + 1] = r.ay;
+    out.acc[3 * i + 2] = r.az;
+    out.jrk[3 * i + 0] = r.jx;
+    out.jrk[3 * i + 1] = r.jy;
+    out.jrk[3 * i + 2] = r.jz;
+    out.pot[i] = r.pot;
+    if (NN) {
+      out.nnb[i] = r.nn >= 0 ? jp.id[r.nn] : -1;
+      out.rnnb[i] = r.nn >= 0 ? nn_distance(jp, r.nn, xi, yi, zi, eps2) : 0.f;
+    }
+  } else {
+    const size_t o = (size_t)blockIdx.y * ni + i;
+    part.ax[o] = r.ax;
+    part.ay[o] = r.ay;
+    part.az[o] = r.az;
+    part.pot[o] = r.pot;
+    part.jx[o] = r.jx;
+    part.jy[o] = r.jy;
+    part.jz[o] = r.jz;
+    if (NN) {
+      part.rmax[o] = r.rmax;
+      part.nn[o] = r.nn;
+    }
+  }
+}
+
 // Nearest-neighbour candidate order: larger (quantised) 1/r wins, equal
 // values go to the lower j address; (0, -1) is "none".  The order is total,
 // so any reduction tree gives the same winner.
@@ -781,6 +1054,220 @@ __global__ void pg6_reduce(int ni, int nsplit, Partials part, JPred jp,
     out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
                                         eps2)
                           : 0.f;
+  }
+}
+
+// ------------------------------------------------------------- small i-sets
+// One launch per call for small i-sets (the block-step sizes of a Hermite
+// code, PG6/banana/hermite.c:214-277): the predictor runs inside the force
+// kernel on the block's own j-range, and the partial sums of the j-ranges
+// are added in the same launch by the last block of each i-block to finish.
+//
+// Block = kFThreads = 8 wavefronts, 64 i-particles (one per lane), blockIdx.x
+// = i-block, blockIdx.y = j-range [y jper, min(nj, (y + 1) jper)).  The range
+// is staged in LDS tiles of kFTile two-float records (JPred::rec layout):
+// with PRED, predicted from the j-state to ti (every JPred array is written
+// too, so later calls at the same ti find the prediction); without, copied
+// from JPred::rec.  Wave w takes tile slots w, w + 8, ... (FP32 sums per tile
+// folded into FP64, as pg6_range_h), and the eight wave sums are added in
+// wave order through LDS.  With nsplit > 1 each block writes its partials
+// ([split][i], as pg6_force_h), releases them (device-scope fence) and counts
+// itself done on counter[blockIdx.x]; the last block of the i-block acquires
+// them and adds the nsplit partials of each i in a fixed order (eight threads
+// per i take splits q, q + 8, ..., then the eight sums in q order, and the
+// nearest neighbour by nn_better, a total order), writes the outputs and
+// resets the counter to zero for the next launch.  Deterministic; the self
+// pair is excluded by id on every pair.
+constexpr int kFWaves = 8;
+constexpr int kFThreads = 64 * kFWaves;
+constexpr int kFTile = 256;
+
+template <bool NN, bool PRED>
+__global__ __launch_bounds__(kFThreads) void pg6_force_f(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, double ti, JState js, JPred jp,
+    Partials part, Outputs out, int nsplit, int* __restrict__ counter) {
+  __shared__ float4 s_rec[kFTile][3];
+  __shared__ double s_v[7][kFWaves][64];
+  __shared__ float s_r[kFWaves][64];
+  __shared__ int s_n[kFWaves][64];
+  __shared__ int s_last;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;  // padding lanes repeat i = ni-1
+  const double xi = ipos[3 * ic + 0], yi = ipos[3 * ic + 1], zi = ipos[3 * ic + 2];
+  const float vxi = (float)ivel[3 * ic + 0], vyi = (float)ivel[3 * ic + 1],
+              vzi = (float)ivel[3 * ic + 2];
+  const int idi = iid[ic];
+  const float xih = (float)xi, yih = (float)yi, zih = (float)zi;
+  const float xil = (float)(xi - (double)xih);
+  const float yil = (float)(yi - (double)yih);
+  const float zil = (float)(zi - (double)zih);
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min(nj, j0 + jper);
+  RangeSums r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f, -1};
+  const unsigned kmask = 0xFFFFFF00u;
+  for (int jt = j0; jt < j1; jt += kFTile) {
+    const int nt = min(kFTile, j1 - jt);
+    __syncthreads();
+    if (threadIdx.x < kFTile) {  // waves 0..3: one j each, 64-aligned chunks
+      const int t = threadIdx.x;
+      const bool v = t < nt;
+      if (PRED) {
+        float4 q[3];
+        pg6_predict_j(jt + t, v, ti, js, jp, q);
+        if (v) {
+          s_rec[t][0] = q[0];
+          s_rec[t][1] = q[1];
+          s_rec[t][2] = q[2];
+        }
+      } else if (v) {
+        const float4* q = jp.rec + 3 * (size_t)(jt + t);
+        s_rec[t][0] = q[0];
+        s_rec[t][1] = q[1];
+        s_rec[t][2] = q[2];
+      }
+    }
+    __syncthreads();
+    TileAcc ta{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0u};
+#pragma unroll 2
+    for (int jj = w; jj < nt; jj += kFWaves) {
+      const float4 a = s_rec[jj][0], b = s_rec[jj][1], c = s_rec[jj][2];
+      const float dx = (a.x - xih) + (b.x - xil);
+      const float dy = (a.y - yih) + (b.y - yil);
+      const float dz = (a.z - zih) + (b.z - zil);
+      pg6_pair_h<true, NN>(ta, dx, dy, dz, b.w - vxi, c.x - vyi, c.y - vzi, a.w, eps2,
+                           __float_as_int(c.z) == idi, (unsigned)(255 - jj));
+    }
+    if (NN) {
+      const float rq = __uint_as_float(ta.key & kmask);
+      if (rq > r.rmax) {
+        r.rmax = rq;
+        r.nn = jt + 255 - (int)(ta.key & 0xFFu);
+      }
+    }
+    r.ax += (double)ta.ax;
+    r.ay += (double)ta.ay;
+    r.az += (double)ta.az;
+    r.pot += (double)ta.pot;
+    r.jx += (double)ta.jx;
+    r.jy += (double)ta.jy;
+    r.jz += (double)ta.jz;
+  }
+  // the eight wave sums of each i, in wave order
+  s_v[0][w][lane] = r.ax;
+  s_v[1][w][lane] = r.ay;
+  s_v[2][w][lane] = r.az;
+  s_v[3][w][lane] = r.pot;
+  s_v[4][w][lane] = r.jx;
+  s_v[5][w][lane] = r.jy;
+  s_v[6][w][lane] = r.jz;
+  s_r[w][lane] = r.rmax;
+  s_n[w][lane] = r.nn;
+  __syncthreads();
+  if (w == 0) {
+    double v[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = s_v[k][0][lane];
+    float rmax = s_r[0][lane];
+    int nn = s_n[0][lane];
+    for (int g = 1; g < kFWaves; ++g) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) v[k] += s_v[k][g][lane];
+      if (NN && nn_better(s_r[g][lane], s_n[g][lane], rmax, nn)) {
+        rmax = s_r[g][lane];
+        nn = s_n[g][lane];
+      }
+    }
+    if (ivalid) {
+      if (nsplit == 1) {
+        out.acc[3 * i + 0] = v[0];
+        out.acc[3 * i + 1] = v[1];
+        out.acc[3 * i + 2] = v[2];
+        out.jrk[3 * i + 0] = v[4];
+        out.jrk[3 * i + 1] = v[5];
+        out.jrk[3 * i + 2] = v[6];
+        out.pot[i] = v[3];
+        if (NN) {
+          out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+          out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+        }
+      } else {
+        const size_t o = (size_t)blockIdx.y * ni + i;
+        part.ax[o] = v[0];
+        part.ay[o] = v[1];
+        part.az[o] = v[2];
+        part.pot[o] = v[3];
+        part.jx[o] = v[4];
+        part.jy[o] = v[5];
+        part.jz[o] = v[6];
+        if (NN) {
+          part.rmax[o] = rmax;
+          part.nn[o] = nn;
+        }
+      }
+    }
+  }
+  if (nsplit == 1) return;
+  // Release this block's partials (and, with PRED, its predicted j), then
+  // count it done; the last block of the i-block acquires everything.
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&counter[blockIdx.x], 1) == nsplit - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int q = w;  // thread (q, lane): i = lane of this i-block, splits q, q + 8, ...
+  double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  float rmax = 0.f;
+  int nn = -1;
+  if (ivalid) {
+    for (int sp = q; sp < nsplit; sp += kFWaves) {
+      const size_t o = (size_t)sp * ni + i;
+      v[0] += part.ax[o];
+      v[1] += part.ay[o];
+      v[2] += part.az[o];
+      v[3] += part.pot[o];
+      v[4] += part.jx[o];
+      v[5] += part.jy[o];
+      v[6] += part.jz[o];
+      if (NN && nn_better(part.rmax[o], part.nn[o], rmax, nn)) {
+        rmax = part.rmax[o];
+        nn = part.nn[o];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 7; ++k) s_v[k][q][lane] = v[k];
+  s_r[q][lane] = rmax;
+  s_n[q][lane] = nn;
+  __syncthreads();
+  if (q == 0) {
+    for (int g = 1; g < kFWaves; ++g) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) v[k] += s_v[k][g][lane];
+      if (NN && nn_better(s_r[g][lane], s_n[g][lane], rmax, nn)) {
+        rmax = s_r[g][lane];
+        nn = s_n[g][lane];
+      }
+    }
+    if (ivalid) {
+      out.acc[3 * i + 0] = v[0];
+      out.acc[3 * i + 1] = v[1];
+      out.acc[3 * i + 2] = v[2];
+      out.jrk[3 * i + 0] = v[4];
+      out.jrk[3 * i + 1] = v[5];
+      out.jrk[3 * i + 2] = v[6];
+      out.pot[i] = v[3];
+      if (NN) {
+        out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+        out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+      }
+    }
+    if (lane == 0) counter[blockIdx.x] = 0;
   }
 }
 

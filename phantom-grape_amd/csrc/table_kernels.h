@@ -32,6 +32,17 @@ __device__ inline float table_f(const float2* __restrict__ tab, float s, float s
   return fmaf(frac, t.y, t.x);
 }
 
+// The same index with the quadratic entries {c0, c1, c2, 0}.
+__device__ inline float table_f_quad(const float4* __restrict__ tab, float s, float scut,
+                                     int shift, int base) {
+  const float sc = fminf(s, scut);
+  const unsigned bits = __float_as_uint(sc);
+  const unsigned hi = bits >> shift;
+  const float4 t = tab[(int)hi - base];
+  const float d = sc - __uint_as_float(hi << shift);
+  return fmaf(d, fmaf(d, t.z, t.y), t.x);
+}
+
 // Block: 256 threads, BI i-particles, G = 256/BI groups share each j-tile.
 template <int BI>
 __global__ __launch_bounds__(kThreads) void table_force(

@@ -5,7 +5,15 @@
 //
 // Inputs (device): pos[np] float4 {x*xs, y*xs, z*xs, m} (positions scaled as
 // in PG5/table/libpg5/phantom_g5.c:112-128; periodic images are ghost copies
-// appended by the caller), per group g an i-list ilist[ioff[g] .. +icnt[g])
+// appended by the caller); optionally pos64[np][3], the same positions in
+// FP64 (unscaled), from which each group forms its separations relative to a
+// group origin o (its first i-particle): a position enters the pair loop
+// through the reference's ingest conversions applied to x - o instead of x
+// (i: float(double(float(x - o)) * xs), PG5/table/libpg5/phantom_g5.c:102-104;
+// j: float(x - o) * float(xs) in FP32, :116-127), so separations keep their
+// digits where |x| >> |dx| (masses still come from pos[].w); per group g an
+// i-list
+// ilist[ioff[g] .. +icnt[g])
 // and a j-list jlist[joff[g] .. +jcnt[g]) (groups may share list ranges), an
 // optional processing order (groups by decreasing cost), and a work counter
 // that must be 0 at launch.
@@ -26,21 +34,39 @@ constexpr int kThreads = 256;    // treepm_batch_mlp block
 constexpr int kJTile = 256;
 constexpr int kTableBlock = 1024;  // treepm_batch_table block
 
-// Table evaluator: s = 2 + |dx_s|^2, f from the LDS table.
+// Table evaluator: s = 2 + |dx_s|^2, f from the LDS table (linear: the
+// reference's {value, slope}; quadratic: {c0, c1, c2, 0}).
 struct TableCut {
   const float2* gtab;
+  const float4* gquad;
   int ntab, frc_bit;
 };
 
-// MLP evaluator, scaled units: r_s = |dx_s|, v = sqrt(r_s / rc_s),
-// f = xs^3 Y(v) / (r_s^3 + r0_s^3) for 0 < r_s < rc_s.
-struct MlpCut {
-  const float* w;
-  float inv_rc_s;  // 1 / (r_c xs)
-  float rc2_s;     // (r_c xs)^2
-  float r03_s;     // (r_0 xs)^3
-  float xs3;       // xs^3
+// Group-origin positions (see above).  o: the group's origin in FP64.
+struct Origin {
+  const double* pos64;  // null: take the scaled float4 positions as they are
+  double xs;
+  float xsf;  // float(xs)
 };
+
+template <bool I>
+__device__ inline float4 load_pos(const float4* __restrict__ pos, const Origin& og, int k,
+                                  double ox, double oy, double oz) {
+  float4 q = pos[k];
+  if (og.pos64) {
+    const double* p = og.pos64 + 3 * (size_t)k;
+    if (I) {
+      q.x = (float)((double)(float)(p[0] - ox) * og.xs);
+      q.y = (float)((double)(float)(p[1] - oy) * og.xs);
+      q.z = (float)((double)(float)(p[2] - oz) * og.xs);
+    } else {
+      q.x = (float)(p[0] - ox) * og.xsf;
+      q.y = (float)(p[1] - oy) * og.xsf;
+      q.z = (float)(p[2] - oz) * og.xsf;
+    }
+  }
+  return q;
+}
 
 __device__ inline int take_item(int* counter, int* s_item) {
   __syncthreads();
@@ -56,8 +82,8 @@ __global__ __launch_bounds__(kThreads) void treepm_batch_mlp(
     int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
     const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ joff,
     const int* __restrict__ jcnt, const int* __restrict__ jlist,
-    const float4* __restrict__ pos, MlpCut mc, double ascale, double* __restrict__ acc,
-    int* counter) {
+    const float4* __restrict__ pos, Origin og, mlp::Cut mc, double ascale,
+    double* __restrict__ acc, int* counter) {
   __shared__ float4 s_j[kJTile];
   __shared__ double s_red[3][kThreads];
   __shared__ int s_item;
@@ -68,6 +94,13 @@ __global__ __launch_bounds__(kThreads) void treepm_batch_mlp(
     const int grp = order ? order[it] : it;
     const int i0 = ioff[grp], ni = icnt[grp];
     const int j0 = joff[grp], nj = jcnt[grp];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (og.pos64 && ni > 0) {
+      const double* o = og.pos64 + 3 * (size_t)ilist[i0];
+      ox = o[0];
+      oy = o[1];
+      oz = o[2];
+    }
     for (int ib = 0; ib < ni; ib += kThreads) {
       const int nib = min(kThreads, ni - ib);
       const int bi = nib <= 64 ? 64 : (nib <= 128 ? 128 : 256);
@@ -76,64 +109,72 @@ __global__ __launch_bounds__(kThreads) void treepm_batch_mlp(
       const int g = __builtin_amdgcn_readfirstlane(tid / bi);  // uniform per wave
       const bool iv = il < nib;
       const int ii = ilist[i0 + ib + (iv ? il : 0)];
-      const float4 ip = pos[ii];
+      const float4 ip = load_pos<true>(pos, og, ii, ox, oy, oz);
       double ax = 0.0, ay = 0.0, az = 0.0;
       for (int jt = 0; jt < nj; jt += kJTile) {
         const int nt = min(kJTile, nj - jt);
         __syncthreads();
-        if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
+        if (tid < nt) s_j[tid] = load_pos<false>(pos, og, jlist[j0 + jt + tid], ox, oy, oz);
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f;
         {
           // This wave's j are jj = g + G t.  Pass 1 marks (64 t at a time,
           // bit t of a wave-uniform mask) the j that some lane has inside
-          // the cut-off; pass 2 evaluates the network only for those, four
-          // j per parameter pass.  Pairs outside contribute exactly 0, as
-          // in the table path, so skipping them changes no sum.
+          // the cut-off, and those inside the S2 core; pass 2 evaluates the
+          // network only for the first, four j per parameter pass, pass 3
+          // the core term for the second.  Pairs outside contribute exactly
+          // 0, as in the table path, so skipping them changes no sum.
           const int nsub = nt > g ? (nt - g + G - 1) / G : 0;
           for (int c0 = 0; c0 < nsub; c0 += 64) {
             const int cn = min(64, nsub - c0);
-            unsigned long long mask = 0ull;
+            unsigned long long mask = 0ull, cmask = 0ull;
             for (int t = 0; t < cn; ++t) {
               const float4 q = s_j[g + G * (c0 + t)];
               const float dx = q.x - ip.x;
               const float dy = q.y - ip.y;
               const float dz = q.z - ip.z;
               const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-              const bool in = iv && r2 > 0.f && r2 < mc.rc2_s;
+              const bool in = iv && r2 > 0.f && r2 < mc.rc2;
               if (__builtin_amdgcn_ballot_w64(in)) mask |= 1ull << t;
+              if (__builtin_amdgcn_ballot_w64(in && r2 < mc.pp2)) cmask |= 1ull << t;
             }
-            while (mask) {
-              float v[4], fac[4], ddx[4], ddy[4], ddz[4];
-              bool inb[4];
+            for (int pass = 0; pass < 2; ++pass) {
+              unsigned long long mk = pass == 0 ? mask : cmask;
+              while (mk) {
+                float v[4], r2s[4], rs[4], rinvs[4], ddx[4], ddy[4], ddz[4];
+                bool inb[4];
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const bool ok = mask != 0ull;
-                const int t = ok ? __builtin_ctzll(mask) : 0;
-                if (ok) mask &= mask - 1ull;
-                const float4 q = s_j[g + G * (c0 + t)];
-                const float dx = q.x - ip.x;
-                const float dy = q.y - ip.y;
-                const float dz = q.z - ip.z;
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float rinv = __builtin_amdgcn_rsqf(r2);
-                const float r = r2 * rinv;
-                const float uu = fminf(r * mc.inv_rc_s, 1.f);
-                v[u] = uu * __builtin_amdgcn_rsqf(uu);
-                inb[u] = ok && r2 > 0.f && r2 < mc.rc2_s;
-                fac[u] = mc.xs3 * __builtin_amdgcn_rcpf(fmaf(r2, r, mc.r03_s));
-                ddx[u] = q.w * dx;
-                ddy[u] = q.w * dy;
-                ddz[u] = q.w * dz;
-              }
-              float y[4];
-              mlp::eval_k<4>(mlp::fresh(mlp::as_const(mc.w)), v, y);
+                for (int u = 0; u < 4; ++u) {
+                  const bool ok = mk != 0ull;
+                  const int t = ok ? __builtin_ctzll(mk) : 0;
+                  if (ok) mk &= mk - 1ull;
+                  const float4 q = s_j[g + G * (c0 + t)];
+                  const float dx = q.x - ip.x;
+                  const float dy = q.y - ip.y;
+                  const float dz = q.z - ip.z;
+                  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                  const float rinv = __builtin_amdgcn_rsqf(r2);
+                  const float r = r2 * rinv;
+                  v[u] = pass == 0 ? mlp::cut_input(mc, r) : r * mc.inv_pp;
+                  inb[u] = ok && r2 > 0.f && r2 < (pass == 0 ? mc.rc2 : mc.pp2);
+                  r2s[u] = r2;
+                  rs[u] = r;
+                  rinvs[u] = rinv;
+                  ddx[u] = q.w * dx;
+                  ddy[u] = q.w * dy;
+                  ddz[u] = q.w * dz;
+                }
+                float y[4];
+                mlp::eval_k<4>(mlp::fresh(mlp::as_const(mc.w)), v, y);
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const float f = inb[u] ? y[u] * fac[u] : 0.f;
-                fax = fmaf(f, ddx[u], fax);
-                fay = fmaf(f, ddy[u], fay);
-                faz = fmaf(f, ddz[u], faz);
+                for (int u = 0; u < 4; ++u) {
+                  const float fm = pass == 0 ? mlp::cut_main(mc, y[u], rs[u], r2s[u], rinvs[u])
+                                             : y[u] * mc.cpp;
+                  const float f = inb[u] ? fm : 0.f;
+                  fax = fmaf(f, ddx[u], fax);
+                  fay = fmaf(f, ddy[u], fay);
+                  faz = fmaf(f, ddz[u], faz);
+                }
               }
             }
           }
@@ -171,20 +212,27 @@ __global__ __launch_bounds__(kThreads) void treepm_batch_mlp(
 // order at the end.  Lanes in use: n p of BLOCK (at least half, usually far
 // more), where whole-wavefront i-blocks of 64, 128 or 256 lanes leave the
 // rest of a wavefront idle for every group that is not a multiple of 64.
-// Groups larger than BLOCK run in passes of BLOCK.
-template <int BLOCK>
+// Groups larger than BLOCK run in passes of BLOCK.  QUAD: the quadratic
+// table (one more FMA and a 16-byte LDS read per pair).
+template <int BLOCK, bool QUAD>
 __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
     int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
     const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ joff,
     const int* __restrict__ jcnt, const int* __restrict__ jlist,
-    const float4* __restrict__ pos, TableCut tc, double ascale, double* __restrict__ acc,
-    int* counter) {
-  extern __shared__ float2 s_tab[];
+    const float4* __restrict__ pos, Origin og, TableCut tc, double ascale,
+    double* __restrict__ acc, int* counter) {
+  extern __shared__ float4 s_tab4[];
+  float2* s_tab = reinterpret_cast<float2*>(s_tab4);
   __shared__ float4 s_j[BLOCK];
   __shared__ double s_red[3][BLOCK];
   __shared__ int s_item;
   const int tid = threadIdx.x;
-  for (int k = tid; k < tc.ntab; k += BLOCK) s_tab[k] = tc.gtab[k];
+  for (int k = tid; k < tc.ntab; k += BLOCK) {
+    if (QUAD)
+      s_tab4[k] = tc.gquad[k];
+    else
+      s_tab[k] = tc.gtab[k];
+  }
   const int shift = 23 - tc.frc_bit;
   const int base = 1 << (30 - shift);
   const float scut = __uint_as_float((1u << 30) | ((unsigned)(tc.ntab - 1) << shift));
@@ -194,6 +242,13 @@ __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
     const int grp = order ? order[it] : it;
     const int i0 = ioff[grp], ni = icnt[grp];
     const int j0 = joff[grp], nj = jcnt[grp];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (og.pos64 && ni > 0) {
+      const double* o = og.pos64 + 3 * (size_t)ilist[i0];
+      ox = o[0];
+      oy = o[1];
+      oz = o[2];
+    }
     for (int ib = 0; ib < ni; ib += BLOCK) {
       const int nib = min(BLOCK, ni - ib);
       const int p = BLOCK / nib;  // phases
@@ -201,12 +256,12 @@ __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
       const int phase = tid / nib;
       const bool live = phase < p;
       const int ii = ilist[i0 + ib + il];
-      const float4 ip = pos[ii];
+      const float4 ip = load_pos<true>(pos, og, ii, ox, oy, oz);
       double ax = 0.0, ay = 0.0, az = 0.0;
       for (int jt = 0; jt < nj; jt += BLOCK) {
         const int nt = min(BLOCK, nj - jt);
         __syncthreads();
-        if (tid < nt) s_j[tid] = pos[jlist[j0 + jt + tid]];
+        if (tid < nt) s_j[tid] = load_pos<false>(pos, og, jlist[j0 + jt + tid], ox, oy, oz);
         __syncthreads();
         float fax = 0.f, fay = 0.f, faz = 0.f;
         if (live) {
@@ -217,7 +272,8 @@ __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
             const float dy = q.y - ip.y;
             const float dz = q.z - ip.z;
             const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
-            const float f = pgt::table_f(s_tab, s, scut, shift, base);
+            const float f = QUAD ? pgt::table_f_quad(s_tab4, s, scut, shift, base)
+                                 : pgt::table_f(s_tab, s, scut, shift, base);
             fax = fmaf(f, q.w * dx, fax);
             fay = fmaf(f, q.w * dy, fay);
             faz = fmaf(f, q.w * dz, faz);

@@ -144,6 +144,8 @@ class G5:
              _vp)
         _sig(L, "pg5_dev_quad", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
         _sig(L, "pg5_dev_stream", _vp, C.c_int)
+        _sig(L, "pg5_set_rsqrt_bias_mode", C.c_int, C.c_int)
+        _sig(L, "pg5_rsqrt_bias", C.c_double)
         _sig(L, "pg5_dev_set_launch", C.c_int, C.c_int, C.c_int, C.c_int)
         _sig(L, "pg5_dev_set_kernel", C.c_int, C.c_int)
         bh = [C.c_int] * 1 + [_vp] * 10 + [_vp, _vp, C.c_double, _vp, _vp]
@@ -398,6 +400,7 @@ class G5Table:
         _sig(L, "g5_open", None)
         _sig(L, "pg5_gen_s2_force_table", None, C.c_double, C.c_double)
         _sig(L, "pg5_set_table_frc_bit", C.c_int, C.c_int)
+        _sig(L, "pg5_set_table_order", C.c_int, C.c_int)
         _sig(L, "pg5_get_table", C.c_int, _fp, _dp, _fp)
         _sig(L, "g5_set_xmj", None, C.c_int, C.c_int, _dp, _dp)
         _sig(L, "g5_set_xmjMC", None, C.c_int, C.c_int, C.c_int, _dp, _dp)
@@ -407,14 +410,14 @@ class G5Table:
         _sig(L, "g5_calculate_force_on_xMC", None, C.c_int, _dp, _dp, _dp, C.c_int)
         _sig(L, "pg5_set_cutoff_mode", C.c_int, C.c_int)
         _sig(L, "pg5_get_cutoff_mode", C.c_int)
-        _sig(L, "pg5_mlp_set_params", C.c_int, _fp, C.c_double, C.c_double)
-        _sig(L, "pg5_mlp_get_params", C.c_int, _fp, _dp, _dp)
-        _sig(L, "pg5_mlp_grad", C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp, _dp)
-        _sig(L, "pg5_mlp_train", C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint, _dp)
+        _sig(L, "pg5_mlp_set_params", C.c_int, _dp, C.c_int, C.c_double, C.c_double)
+        _sig(L, "pg5_mlp_get_params", C.c_int, _dp, _ip, _dp, _dp)
+        _sig(L, "pg5_mlp_normal_eq", C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp)
+        _sig(L, "pg5_mlp_train", C.c_int, C.c_int, C.c_int, C.c_uint, _dp)
         _sig(L, "pg5_mlp_eval_dev", C.c_int, C.c_int, _vp, _vp)
         _sig(L, "pg5t_dev_eval", C.c_int, C.c_int, _vp, _vp)
         _sig(L, "pg5t_dev_force", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp)
-        bt = [C.c_int] + [_vp] * 9 + [C.c_int]
+        bt = [C.c_int] + [_vp] * 10 + [C.c_int]  # ngroups, 7 lists, pos, pos64, acc, mode
         _sig(L, "pg5t_batch_force", C.c_int, C.c_int, *bt, _vp)
         _sig(L, "pg5t_batch_graph", _vp, C.c_int, *bt)
         _sig(L, "pg5t_batch_graph_launch", C.c_int, _vp, _vp)
@@ -443,25 +446,34 @@ class G5Table:
         return a
 
     def mlp_params(self):
-        p = np.zeros(321, np.float32)
+        """(p[321] tanh form FP64, form, rc, r0) of the loaded network, or None."""
+        p = np.zeros(321)
+        form = np.zeros(1, np.int32)
         rc = np.zeros(1)
         r0 = np.zeros(1)
-        ok = self.lib.pg5_mlp_get_params(fptr(p), dptr(rc), dptr(r0))
-        return (p, float(rc[0]), float(r0[0])) if ok else None
+        ok = self.lib.pg5_mlp_get_params(dptr(p), iptr(form), dptr(rc), dptr(r0))
+        return (p, int(form[0]), float(rc[0]), float(r0[0])) if ok else None
 
-    def mlp_train(self, steps=4000, batch=4096, lr=3e-3, seed=1):
+    def mlp_set_params(self, p, form, rc, r0):
+        assert self.lib.pg5_mlp_set_params(dptr(f64(p)), int(form), float(rc), float(r0)) == 0
+
+    def mlp_train(self, iters=200, starts=4, seed=1):
+        """pg5_mlp_train: Levenberg-Marquardt from `starts` seeded starts;
+        returns the final weighted MSE."""
         loss = np.zeros(1)
-        rc = self.lib.pg5_mlp_train(steps, batch, lr, seed, dptr(loss))
+        rc = self.lib.pg5_mlp_train(int(iters), int(starts), int(seed), dptr(loss))
         assert rc == 0
         return float(loss[0])
 
-    def mlp_grad(self, v, t, wt, p):
-        v, t, wt, p = (np.ascontiguousarray(a, np.float32) for a in (v, t, wt, p))
-        g = np.zeros(321, np.float32)
-        loss = np.zeros(1)
-        assert self.lib.pg5_mlp_grad(len(v), fptr(v), fptr(t), fptr(wt), fptr(p), fptr(g),
-                                     dptr(loss)) == 0
-        return g, float(loss[0])
+    def mlp_normal_eq(self, v, t, wt, p):
+        """(J^T J, J^T r, r^T r) of the weighted fit at p, from the GPU kernels."""
+        v, t, wt, p = (f64(a) for a in (v, t, wt, p))
+        jtj = np.zeros((321, 321))
+        jtr = np.zeros(321)
+        rtr = np.zeros(1)
+        assert self.lib.pg5_mlp_normal_eq(len(v), dptr(v), dptr(t), dptr(wt), dptr(p),
+                                          dptr(jtj), dptr(jtr), dptr(rtr)) == 0
+        return jtj, jtr, float(rtr[0])
 
 
 # ------------------------------------------------------- interaction lists

@@ -54,6 +54,17 @@ def test_pg6_step_n65536():
     assert np.percentile(rel_err(r["acc"][sel], o["acc"]), 99) < 1e-6
     assert np.percentile(rel_err(r["jrk"][sel], o["jrk"]), 99) < 1e-5
     assert rel_err(r["pot"][sel], o["pot"]).max() < 1e-6
+    # each large-i kernel at full size, the sorted chunk-origin one included
+    for k in (2, 4):
+        assert g6.lib.pg6_dev_set_kernel(k) == 0
+        rk = g6.calc(n, x, v, idx, eps * eps, variant="2p")
+        g6.lib.pg6_dev_set_kernel(-1)
+        e = rel_err(rk["acc"][sel], o["acc"])
+        print(f"kernel {k}: acc p99 {np.percentile(e, 99):.3e} max {e.max():.3e}")
+        assert np.percentile(e, 99) < 1e-6, (k, np.percentile(e, 99))
+        assert np.percentile(rel_err(rk["jrk"][sel], o["jrk"]), 99) < 1e-5
+        assert rel_err(rk["pot"][sel], o["pot"]).max() < 1e-6
+        assert momentum_ratio(m, rk["acc"]) < 1e-6
 
 
 def test_treepm_short_range_n2p20():
@@ -79,14 +90,27 @@ def test_treepm_short_range_n2p20():
         d = {k: DL.ptr(k) for k in ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")}
         acc = torch.zeros((len(P), 3), dtype=torch.float64, device="cuda")
         ng = len(L["ioff"])
-        args = (0, ng, d["ioff"], d["icnt"], d["ilist"], d["joff"], d["jcnt"], d["jlist"],
-                d["order"], d_pos.data_ptr(), acc.data_ptr(), 0, None)
+        lists = (d["ioff"], d["icnt"], d["ilist"], d["joff"], d["jcnt"], d["jlist"], d["order"])
+        args = (0, ng, *lists, d_pos.data_ptr(), None, acc.data_ptr(), 0, None)
         assert T.lib.pg5t_batch_force(*args) == 0
         torch.cuda.synchronize()
         a = acc.cpu().numpy()[: len(x)]
         assert T.lib.pg5t_batch_force(*args) == 0
         torch.cuda.synchronize()
         np.testing.assert_array_equal(acc.cpu().numpy()[: len(x)], a)
+        # FP64 positions relative to each group's origin (the list builder's
+        # positions in HBM), table and learned cut-off
+        acc64 = torch.zeros((len(P), 3), dtype=torch.float64, device="cuda")
+        assert T.lib.pg5t_batch_force(0, ng, *lists, d_pos.data_ptr(), DL.ptr("pos"),
+                                      acc64.data_ptr(), 0, None) == 0
+        torch.cuda.synchronize()
+        a64 = acc64.cpu().numpy()
+        tab, _ = T.table()
+        T.mlp_train(iters=150, starts=3, seed=7)
+        assert T.lib.pg5t_batch_force(0, ng, *lists, d_pos.data_ptr(), DL.ptr("pos"),
+                                      acc64.data_ptr(), 1, None) == 0
+        torch.cuda.synchronize()
+        a_net = acc64.cpu().numpy()
     finally:
         DL.free()
         T.lib.pg5_set_table_frc_bit(5)
@@ -98,6 +122,29 @@ def test_treepm_short_range_n2p20():
     f = O.table_force_f64(P[sel], P, mm, pp, pm)
     e = rel_err(a[sel], f, floor=1e-3 * np.linalg.norm(f, axis=1).max())
     assert np.median(e) < 3e-4 and np.percentile(e, 99) < 3e-3
+    # the group-origin path: the reference's pair terms (the oracle's
+    # bit-exact emulation of gravity_kernel_avx2.c:22-137 on x - o over each
+    # sampled particle's own list) up to summation order, and against the
+    # closed form the 1024-entry table's own interpolation error
+    pos_in = np.full(len(P), -1, np.int64)  # place of each particle in the i-lists
+    pos_in[L["ilist"]] = np.arange(len(L["ilist"]))
+    emul = np.zeros((len(sel), 3))
+    for q, i in enumerate(sel):
+        g = np.searchsorted(L["ioff"], pos_in[i], side="right") - 1
+        il = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        jl = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        o = P[il[0]]
+        emul[q] = O.table_force_emul(P[i:i + 1] - o, P[jl] - o, mm[jl], tab, xs, 4, 6)[0]
+    fl = 1e-3 * np.linalg.norm(f, axis=1).max()  # floor for near-zero short-range forces
+    e = rel_err(a64[sel], emul, floor=fl)
+    assert np.percentile(e, 99) < 2e-6 and e.max() < 1e-5, (np.percentile(e, 99), e.max())
+    e = rel_err(a64[sel], f, floor=fl)
+    assert np.percentile(e, 99) < 3e-3
+    # the learned cut-off at full size
+    e = rel_err(a_net[sel], f, floor=fl)
+    print(f"learned cut-off N=2^20: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}")
+    assert np.median(e) < 1e-5 and np.percentile(e, 99) < 1e-4, (np.median(e),
+                                                                 np.percentile(e, 99))
 
 
 def test_barnes_hut_n2p21():
@@ -128,11 +175,26 @@ def test_barnes_hut_n2p21():
         assert g5.lib.pg5_bh_batch(*args) == 0
         torch.cuda.synchronize()
         a = acc.cpu().numpy()
+        p = pot.cpu().numpy()
         assert g5.lib.pg5_bh_batch(*args) == 0
         torch.cuda.synchronize()
         np.testing.assert_array_equal(acc.cpu().numpy(), a)
+        P4 = d_pos.cpu().numpy().astype(np.float64)
+        Cc = d_cells.cpu().numpy().astype(np.float64)
     finally:
         DL.free()
+    # against the FP64 oracle over the same lists (the caller contract of
+    # PG5/newton_quad/source.cpp:845-867: particle part through the g5 calls,
+    # cells through the g5c calls) on the FP32 inputs the kernel reads
+    e2 = float(np.float32(eps * eps))
+    for g in np.random.default_rng(10).choice(len(L["ioff"]), 4, replace=False):
+        il = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+        pl = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+        cl = L["clist"][L["coff"][g]:L["coff"][g] + L["ccnt"][g]]
+        ap, pp = O.pg5_force(P4[il, :3], e2, P4[pl, :3], P4[pl, 3])
+        aq, pq = O.quad_force(P4[il, :3], e2, Cc[cl, :3], Cc[cl, 3], Cc[cl, 4:10])
+        assert np.percentile(rel_err(a[il], ap + aq), 99) < 1e-5
+        assert rel_err(p[il], pp + pq).max() < 1e-5
     # against the FP64 direct sum over all 2^21 particles: the tree's own
     # error at theta = 0.5 with quadrupoles
     sel = np.sort(np.random.default_rng(9).choice(n, 96, replace=False))

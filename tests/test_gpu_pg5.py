@@ -213,3 +213,51 @@ def test_n_beyond_the_addresses_written(g5):
         assert np.all(np.isfinite(a)) and np.all(np.isfinite(p))
         assert rel_err(a, a0).max() < 1e-6
         assert rel_err(p, p0).max() < 1e-6
+
+
+def test_rsqrt_bias_of_the_device_and_its_correction(g5):
+    """The reference's rsqrt-bias rule (rsqrt.c:42-81) applied to the device's
+    own rsqrt: the library's b equals the oracle's restatement of the rule on
+    the v_rsq_f32 values the force kernel produces (read back as the
+    potential of one unit mass at distance 0 with eps^2 = x); mode 1 scales
+    the g5 outputs by float(1 - 3b) and float(1 - b) as phantom_g5.c:45-60,
+    244-269 do."""
+    import ctypes as C
+
+    import torch
+    L = g5.lib
+    L.pg5_rsqrt_bias.restype = C.c_double
+    b = L.pg5_rsqrt_bias()
+    ys = []
+    for x in O.rsqrt_args():
+        n = len(x)
+        xi = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        xi[:, 3] = torch.from_numpy(x).cuda()
+        xj = torch.tensor([[0.0, 0.0, 0.0, 1.0]], dtype=torch.float32, device="cuda")
+        acc = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+        pot = torch.empty(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert L.pg5_dev_force(0, n, xi.data_ptr(), 1, xj.data_ptr(), None, acc.data_ptr(),
+                               pot.data_ptr(), None) == 0
+        torch.cuda.synchronize()
+        ys.append(pot.cpu().numpy())
+        del xi, acc, pot
+    y0, y1 = ys
+    assert np.all(y0 == y0.astype(np.float32))  # FP32 values, carried exactly
+    bo = O.rsqrt_bias_of(y0, y1)
+    print(f"device rsqrt bias {b:.3e} (oracle on the device's values {bo:.3e})")
+    assert abs(b - bo) <= 1e-9 * abs(bo) + 1e-18, (b, bo)
+    assert abs(b) < 1e-6
+    # the correction on the g5 outputs
+    m, x, _ = ics.plummer(512, 27)
+    g5.set_eps_to_all(0.01)
+    g5.set_xmj(0, x, m)
+    g5.set_n(512)
+    a0, p0 = g5.calculate_force_on_x(x[:64])
+    assert L.pg5_set_rsqrt_bias_mode(1) == 0
+    try:
+        a1, p1 = g5.calculate_force_on_x(x[:64])
+    finally:
+        L.pg5_set_rsqrt_bias_mode(0)
+    np.testing.assert_array_equal(a1, a0 * float(np.float32(1 - 3 * b)))
+    np.testing.assert_array_equal(p1, p0 * float(np.float32(1 - b)))

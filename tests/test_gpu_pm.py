@@ -70,7 +70,7 @@ def test_treepm_total_force_is_the_periodic_force(tab):
     acc = torch.zeros((i["np"], 3), dtype=torch.float64, device="cuda")
     keys = ("ioff", "icnt", "ilist", "joff", "jcnt", "jlist", "order")
     assert tab.lib.pg5t_batch_force(0, i["ngroups"], *[L.ptr(k) for k in keys],
-                                    d_pos4.data_ptr(), acc.data_ptr(), 0, None) == 0
+                                    d_pos4.data_ptr(), None, acc.data_ptr(), 0, None) == 0
     torch.cuda.synchronize()
     src = L.host()["src"]
     L.free()

@@ -113,7 +113,7 @@ def test_gpu_lists_give_identical_batch_forces():
     h_pos4 = dev(pos4)
     ng = len(h["ioff"])
     assert T.lib.pg5t_batch_force(0, ng, *[dh[k].data_ptr() for k in keys], h_pos4.data_ptr(),
-                                  acc_h.data_ptr(), 0, None) == 0
+                                  None, acc_h.data_ptr(), 0, None) == 0
     torch.cuda.synchronize()
     # GPU lists, packed on the GPU
     d_x = dev(x)
@@ -128,7 +128,7 @@ def test_gpu_lists_give_identical_batch_forces():
     acc_d = torch.zeros((i["np"], 3), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     assert T.lib.pg5t_batch_force(0, i["ngroups"], *[L.ptr(k) for k in keys], d_pos4.data_ptr(),
-                                  acc_d.data_ptr(), 0, None) == 0
+                                  None, acc_d.data_ptr(), 0, None) == 0
     torch.cuda.synchronize()
     L.free()
     np.testing.assert_array_equal(acc_d.cpu().numpy(), acc_h.cpu().numpy())

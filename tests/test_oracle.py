@@ -295,6 +295,31 @@ def test_quad_oracle_matches_reference_fixture():
     np.testing.assert_allclose(g["pot"], -p, rtol=3e-3)
 
 
+# ------------------------------------------------------------ rsqrt bias
+def test_rsqrt_bias_restatement():
+    """or_rsqrt_bias_of restates rsqrt.c:42-81.  With the reference's own
+    rsqrt (SSE rsqrtss) it gives the compiled reference's rsqrt_bias() (that
+    build sums under -ffast-math, so to 1e-11 relative, not bit for bit); with
+    correctly rounded FP32 values the bias vanishes; and values planted 1 + d
+    too large give b = d to first order (a dropped factor of the rule fails)."""
+    import ctypes as C
+    y0, y1 = O.rsqrtss_table()
+    b = O.rsqrt_bias_of(y0, y1)
+    if O.ref_available("pg5"):
+        R = C.CDLL(O._refpath("pg5").decode(), mode=C.RTLD_LOCAL)
+        R.rsqrt_bias.restype = C.c_double
+        br = R.rsqrt_bias()
+        assert abs(b - br) < 1e-11 * abs(br), (b, br)
+    assert 0 < abs(b) < 1e-5
+    x0, x1 = O.rsqrt_args()
+    e0 = (1 / np.sqrt(x0.astype(np.float64))).astype(np.float32)
+    e1 = (1 / np.sqrt(x1.astype(np.float64))).astype(np.float32)
+    assert abs(O.rsqrt_bias_of(e0, e1)) < 1e-10
+    d = 3e-4
+    bp = O.rsqrt_bias_of((e0 * (1 + d)).astype(np.float32), (e1 * (1 + d)).astype(np.float32))
+    assert abs(bp - d) < 2 * d * d, (bp, d)
+
+
 # --------------------------------------------------------- learned cut-off
 def test_mlp_oracle_gradient_matches_finite_differences():
     import mlp_oracle as M
@@ -313,6 +338,48 @@ def test_mlp_oracle_gradient_matches_finite_differences():
     # forward agrees with the loss' own forward
     y = M.forward(p, v)
     assert abs(M.loss_and_grad(p, v, y, wt)[0]) < 1e-28
+
+
+def test_mlp_oracle_jacobian_matches_finite_differences():
+    """Every column of the oracle Jacobian against central differences of the
+    forward pass, and the normal equations against their definition (the
+    gradient of the loss is 2 J^T r / n)."""
+    import mlp_oracle as M
+    rng = np.random.default_rng(6)
+    p = rng.normal(0, 0.7, M.NP)
+    v = rng.uniform(0, 1, 48)
+    t = rng.normal(0, 1, 48)
+    wt = rng.uniform(0.5, 2, 48)
+    y, J = M.jacobian(p, v)
+    np.testing.assert_allclose(y, M.forward(p, v), rtol=0, atol=1e-14)
+    h = 1e-6
+    for k in range(M.NP):
+        e = np.zeros(M.NP)
+        e[k] = h
+        fd = (M.forward(p + e, v) - M.forward(p - e, v)) / (2 * h)
+        assert np.max(np.abs(fd - J[:, k])) < 1e-7 * max(1.0, np.abs(J[:, k]).max()), k
+    jtj, jtr, rtr = M.normal_eq(p, v, t, wt)
+    loss, g = M.loss_and_grad(p, v, t, wt)
+    np.testing.assert_allclose(2 * jtr / len(v), g, rtol=1e-10, atol=1e-14)
+    assert abs(rtr / len(v) - loss) < 1e-12 * loss
+    np.testing.assert_allclose(jtj, jtj.T, rtol=0, atol=0)
+
+
+def test_s2_unit_shape_gives_the_cutoff_force():
+    """The S2 form of the learned target: with H(u) = R(u, 1), the short-range
+    shape is f(r) = [r >= eps_PP] / r^3 + [r < eps_PP] H(r / eps_PP) / eps_PP^3
+    - H(r / eps_PM) / eps_PM^3 below eps_PM.  Checked against the oracle's
+    closed form of pg5_table.c:27-29 on a grid through the core, both joints
+    of the polynomial pieces and the cut-off; H(0) = 12.8, H(1) = 1."""
+    import mlp_oracle as M
+    pp, pm = 0.1 / 128, 3.0 / 128
+    H = lambda u: M.s2_unit(u, O.s2_reff)
+    np.testing.assert_allclose(H([0.0, 1.0]), [12.8, 1.0], rtol=1e-15)
+    r = np.concatenate([np.geomspace(1e-3 * pp, pm, 3000)[:-1], [pp / 2, pp, pm / 2]])
+    core = r < pp
+    f = np.where(core, H(np.minimum(r / pp, 1.0)) / pp ** 3, 1.0 / r ** 3) - H(r / pm) / pm ** 3
+    # relative to the pair's Newtonian 1/r^3 (f itself vanishes at the cut-off)
+    assert np.max(np.abs(f - O.s2_force(pp, pm, r)) * r ** 3) < 1e-12
 
 
 def test_mlp_target_is_bounded_and_smooth():
