@@ -112,9 +112,10 @@ int pg5_mlp_set_params(const double *p321, int form, double rc, double r0);
 int pg5_mlp_get_params(double *p321, int *form, double *rc, double *r0);
 /* Levenberg-Marquardt on the GPU: `starts` seeded starts of `iters` trial
  * steps each (FP64 Jacobians and normal equations from the training kernels,
- * the damped 321 x 321 solve on the host); keeps the best.  *loss = weighted
- * mean squared error of the kept network. */
-int pg5_mlp_train(int iters, int starts, unsigned seed, double *loss);
+ * the damped 321 x 321 solve on the host), then `refine` more trial steps for
+ * the best of them, which is kept.  *loss = weighted mean squared error of
+ * the kept network. */
+int pg5_mlp_train(int iters, int starts, int refine, unsigned seed, double *loss);
 /* Normal equations of the weighted fit at p321 to n host samples, from the
  * FP64 training kernels: jtj[321*321] (nullable), jtr[321], *rtr, with
  * residuals sqrt(wt) (y(v) - t).  Synchronous; for checking the kernels. */

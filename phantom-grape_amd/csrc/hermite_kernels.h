@@ -118,14 +118,12 @@ struct Plan {
   int ntiles, ns, jper, stride;
 };
 
-__host__ __device__ inline Plan pg6h_plan(int nact, int nj, int target, int min_jper,
-                                          int max_ns = 1 << 30) {
+__host__ __device__ inline Plan pg6h_plan(int nact, int nj, int target, int min_jper) {
   Plan p;
   p.ntiles = (nact + 63) / 64;
   const int blocks = target / kForceWaves;
   int ns = p.ntiles > 0 ? (blocks + p.ntiles - 1) / p.ntiles : 1;
   if (ns > nj / (kForceWaves * min_jper)) ns = nj / (kForceWaves * min_jper);
-  if (ns > max_ns) ns = max_ns;
   if (ns < 1) ns = 1;
   const int q = 16 * kForceWaves;
   p.jper = ((nj + ns - 1) / ns + q - 1) / q * q;
@@ -407,99 +405,6 @@ __global__ __launch_bounds__(256) void pg6h_finish(int n, Params P, Ctl* ctl, in
     // for it)
     ctl->tmin = atomicExch(&ctl->tmin_next, kInfBits);
     ctl->done = 0;
-  }
-}
-
-// Force and finish in one launch: pg6h_force's (i-tile, j-range) items, each
-// writing its FP64 partials, and the last item of each i-tile to finish
-// (counted on tile_done[tile], release/acquire fences around the count)
-// sums that tile's partials in a fixed order (eight threads per slot take
-// ranges q, q + 8, ..., then the eight sums in q order), applies correct_one
-// to its 64 slots and folds the new t + dt into Ctl::tmin_next; the last tile
-// to finish closes the step in Ctl as pg6h_finish's last block does.  The
-// j-records the corrector writes (store_j) are not read in this kernel: the
-// pair loop reads the predicted set that pg6h_select wrote.  At most max_ns
-// j-ranges per tile, so that a tile's finish reads few partials.
-// Block = kForceWaves wavefronts.
-__global__ __launch_bounds__(64 * kForceWaves) void pg6h_force_finish(
-    int n, float eps2, Params P, Ctl* ctl, int target, int min_jper, int max_ns,
-    const int* __restrict__ act, State S, const double* __restrict__ ipos,
-    const double* __restrict__ ivel, const int* __restrict__ iid, pg6::JPred jp,
-    pg6::JState js, pg6::Partials part, int* __restrict__ tile_done) {
-  if (pg6h_skip(ctl, P.dtmax)) return;
-  const int nact = ctl->nact;
-  const double tnext = __longlong_as_double((long long)ctl->tmin);
-  const Plan pl = pg6h_plan(nact, n, target, min_jper, max_ns);
-  const int items = pl.ntiles * pl.ns;
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  const int piece = pl.jper / kForceWaves;
-  double* dst[7] = {part.ax, part.ay, part.az, part.pot, part.jx, part.jy, part.jz};
-  __shared__ double red[7][kForceWaves][64];
-  __shared__ int s_last;
-  for (int it = blockIdx.x; it < items; it += gridDim.x) {
-    const int tile = it % pl.ntiles, sp = it / pl.ntiles;
-    const int i = tile * 64 + lane;
-    const int ic = i < nact ? i : nact - 1;  // padding lanes repeat the last slot
-    const int j0 = min(n, sp * pl.jper + w * piece);
-    const pg6::RangeSums r = pg6::pg6_range_h<false>(
-        ipos[3 * ic], ipos[3 * ic + 1], ipos[3 * ic + 2], (float)ivel[3 * ic],
-        (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], iid[ic], eps2, j0, min(n, j0 + piece),
-        jp);
-    red[0][w][lane] = r.ax;
-    red[1][w][lane] = r.ay;
-    red[2][w][lane] = r.az;
-    red[3][w][lane] = r.pot;
-    red[4][w][lane] = r.jx;
-    red[5][w][lane] = r.jy;
-    red[6][w][lane] = r.jz;
-    __syncthreads();
-    if (threadIdx.x < 7 * 64) {
-      const int c = threadIdx.x / 64;
-      double sum = 0.0;
-      for (int q = 0; q < kForceWaves; ++q) sum += red[c][q][lane];
-      const int k = tile * 64 + lane;
-      if (k < nact) dst[c][(size_t)sp * pl.stride + k] = sum;
-    }
-    // release the partials, then count this range done for the tile
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&tile_done[tile], 1) == pl.ns - 1;
-    __syncthreads();
-    if (!s_last) continue;
-    __threadfence();  // acquire the tile's other partials
-    const int k = tile * 64 + lane;
-    double f[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (k < nact)
-      for (int q = w; q < pl.ns; q += kForceWaves)
-        for (int c = 0; c < 7; ++c) f[c] += dst[c][(size_t)q * pl.stride + k];
-    for (int c = 0; c < 7; ++c) red[c][w][lane] = f[c];
-    __syncthreads();
-    double tmin = 1e300;
-    if (w == 0 && k < nact) {
-      for (int q = 1; q < kForceWaves; ++q)
-        for (int c = 0; c < 7; ++c) f[c] += red[c][q][lane];
-      tmin = correct_one(k, act[k], S, P, ipos, ivel, f, js, &ctl->bad);
-    }
-    fold_min_block(tmin, &ctl->tmin_next);  // thread 0 waits for its atomic
-    if (threadIdx.x == 0) {
-      tile_done[tile] = 0;
-      if (atomicAdd(&ctl->done, 1) == pl.ntiles - 1) {  // the last tile closes the step
-        ctl->last_nact = nact;
-        ctl->nsteps += nact;
-        ctl->nblocks += 1;
-        if (tnext == P.dtmax) {  // end of a window: every time restarts at 0
-          ctl->t_gl = 0.0;
-          ctl->t_mod += 1;
-        } else {
-          ctl->t_gl = tnext;
-        }
-        ctl->nact = 0;
-        ctl->tmin = atomicExch(&ctl->tmin_next, kInfBits);
-        ctl->done = 0;
-      }
-    }
-    __syncthreads();
   }
 }
 

@@ -78,7 +78,9 @@ __device__ inline void eval_k(cfloat* w, const float (&v)[K], float (&y)[K]) {
   }
 #pragma unroll
   for (int u = 0; u < K; ++u) y[u] = w[kB3];
-#pragma unroll
+  // one second-layer unit per iteration: its 18 parameters fit the scalar
+  // registers (unrolled, all 288 would be live at once and spill)
+#pragma unroll 1
   for (int k = 0; k < kH; ++k) {
     float z[K];
 #pragma unroll

@@ -926,37 +926,47 @@ bool chol_solve(std::vector<double>& A, std::vector<double>& b, int P) {
   return true;
 }
 
-// Levenberg-Marquardt (Marquardt's diagonal scaling) from x for at most
-// `iters` trial steps; returns r^T r at the final x.
-double lm_fit(Fit& f, std::vector<double>& x, int iters, hipStream_t st) {
-  constexpr int P = mlp::kParams, E = mlp::kExt;
-  std::vector<double> M, Mn, A((size_t)P * P), d(P), xn(P);
-  fit_normal(f, x.data(), M, st);
-  double F = M[(size_t)P * E + P];
+// Levenberg-Marquardt state (Marquardt's diagonal scaling): parameters x,
+// their normal equations M (kExt x kExt, the last column J^T r, the corner
+// r^T r), the loss F = r^T r and the damping lam.
+struct Lm {
+  std::vector<double> x, M;
+  double F = HUGE_VAL;
   double lam = 1e-3;
-  for (int it = 0; it < iters && lam < 1e12; ++it) {
+};
+
+void lm_start(Fit& f, Lm& s, hipStream_t st) {
+  fit_normal(f, s.x.data(), s.M, st);
+  s.F = s.M[(size_t)mlp::kParams * mlp::kExt + mlp::kParams];
+  s.lam = 1e-3;
+}
+
+// At most `iters` trial steps from s (fewer once the damping exceeds 1e12).
+void lm_steps(Fit& f, Lm& s, int iters, hipStream_t st) {
+  constexpr int P = mlp::kParams, E = mlp::kExt;
+  std::vector<double> Mn, A((size_t)P * P), d(P), xn(P);
+  for (int it = 0; it < iters && s.lam < 1e12; ++it) {
     for (int a = 0; a < P; ++a) {
-      for (int b = 0; b < P; ++b) A[(size_t)a * P + b] = M[(size_t)a * E + b];
-      A[(size_t)a * P + a] += lam * M[(size_t)a * E + a] + 1e-300;
-      d[a] = -M[(size_t)a * E + P];
+      for (int b = 0; b < P; ++b) A[(size_t)a * P + b] = s.M[(size_t)a * E + b];
+      A[(size_t)a * P + a] += s.lam * s.M[(size_t)a * E + a] + 1e-300;
+      d[a] = -s.M[(size_t)a * E + P];
     }
     if (!chol_solve(A, d, P)) {
-      lam *= 4.0;
+      s.lam *= 4.0;
       continue;
     }
-    for (int a = 0; a < P; ++a) xn[a] = x[a] + d[a];
+    for (int a = 0; a < P; ++a) xn[a] = s.x[a] + d[a];
     fit_normal(f, xn.data(), Mn, st);
     const double Fn = Mn[(size_t)P * E + P];
-    if (std::isfinite(Fn) && Fn < F) {
-      x.swap(xn);
-      M.swap(Mn);
-      F = Fn;
-      lam = std::max(lam / 3.0, 1e-15);
+    if (std::isfinite(Fn) && Fn < s.F) {
+      s.x.swap(xn);
+      s.M.swap(Mn);
+      s.F = Fn;
+      s.lam = std::max(s.lam / 3.0, 1e-15);
     } else {
-      lam *= 4.0;
+      s.lam *= 4.0;
     }
   }
-  return F;
 }
 
 // Seeded initial parameters (tanh form): spread first-layer units, Glorot
@@ -1020,14 +1030,19 @@ PGAMD_API int pg5_mlp_normal_eq(int n, const double* v, const double* t, const d
 // weights 1 / (|Y| + 0.05)^2.  Targets are evaluated in FP64.  `starts`
 // seeded initialisations each run `iters` Levenberg-Marquardt trial steps
 // (FP64 residuals, Jacobians and normal equations from the GPU kernels, the
-// damped 321 x 321 solve on the host); the one with the smallest loss is
-// kept.  Deterministic for a given seed.  *loss_out = sum wt (y - t)^2 / n.
-PGAMD_API int pg5_mlp_train(int iters, int starts, unsigned seed, double* loss_out) {
+// damped 321 x 321 solve on the host); the one with the smallest loss then
+// runs `refine` more (many starts stall on a plateau for hundreds of steps;
+// the best one after a short run is the one that converges).  Deterministic
+// for a given seed.  *loss_out = sum wt (y - t)^2 / n.
+PGAMD_API int pg5_mlp_train(int iters, int starts, int refine, unsigned seed,
+                            double* loss_out) {
   Board* b = board(0, "pg5_mlp_train");
   if (!b) return -1;
-  if (iters <= 0 || starts <= 0 || starts > 64) {
-    api_error(3, "pg5_mlp_train: iters=%d starts=%d (need iters > 0, 0 < starts <= 64)", iters,
-              starts);
+  if (iters <= 0 || starts <= 0 || starts > 64 || refine < 0) {
+    api_error(3,
+              "pg5_mlp_train: iters=%d starts=%d refine=%d (need iters > 0, 0 < starts <= 64, "
+              "refine >= 0)",
+              iters, starts, refine);
     return -1;
   }
   std::lock_guard<std::mutex> lk(g_mutex);
@@ -1080,24 +1095,23 @@ PGAMD_API int pg5_mlp_train(int iters, int starts, unsigned seed, double* loss_o
   Fit f;
   f.alloc(n);
   f.upload(hv.data(), ht.data(), hsw.data(), b->stream);
-  std::vector<double> best, x;
-  double bestF = HUGE_VAL;
+  Lm best;
   for (int s = 0; s < starts; ++s) {
-    mlp_init(rng, tmean, x);
-    const double F = lm_fit(f, x, iters, b->stream);
-    if (F < bestF) {
-      bestF = F;
-      best = x;
-    }
+    Lm c;
+    mlp_init(rng, tmean, c.x);
+    lm_start(f, c, b->stream);
+    lm_steps(f, c, iters, b->stream);
+    if (c.F < best.F) best = std::move(c);
   }
+  lm_steps(f, best, refine, b->stream);
   f.release();
-  g_mlp.params = best;
+  g_mlp.params = best.x;
   g_mlp.form = s2 ? 1 : 0;
   g_mlp.rc = rc;
   g_mlp.r0 = r0;
   g_mlp.valid = true;
   g_mlp.dirty = true;
-  if (loss_out) *loss_out = bestF / n;
+  if (loss_out) *loss_out = best.F / n;
   return 0;
 }
 

@@ -76,7 +76,7 @@ inline int ni_bucket(int ni) {
 // kernel for large ones, and many j-ranges so that the last wave of blocks is
 // a small fraction of the launch.
 inline LaunchChoice default_choice(int ni) {
-  if (ni <= 256) return {3, 64, 4};
+  if (ni <= 128) return {0, 64, 32};
   if (ni <= 4096) return {0, 64, 8};
   if (ni <= 8192) return {2, 128, 16};
   if (ni <= 32768) return {2, 128, 32};
@@ -120,12 +120,13 @@ struct Board {
   DevBuf d_in, d_part;
   DevBuf d_fcount;  // pg6_force_f's per-i-block completion counters (kept at zero)
   // Sorted j layout of pg6_force_o (pg6::JSort): built from the predicted
-  // positions when missing or older than kSortAge predictions, its records
-  // rewritten by every prediction made while it is in use (srec_ok: they hold
-  // the current prediction).
+  // positions when missing or stale (kSortAge), its origins, flags and
+  // records rewritten by every prediction made while it is in use (srec_ok:
+  // they hold the current prediction).
   DevBuf d_sort, d_sort_tmp;
   pg6::JSort so{};
   int sort_nj = 0, sort_age = 0;
+  long long sort_writes = 0;  // j-records written since the build
   bool srec_ok = false;
   // neighbour lists of the last lasthalf*n call
   DevBuf d_nb;
@@ -344,6 +345,7 @@ void board_flush_j(Board& b) {
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
                      b.npend, mapped<JRecord>(b.pend), b.js);
   PGAMD_HIP_CHECK(hipGetLastError());
+  b.sort_writes += b.npend;
   // No wait here: the next set_j_particle waits for this event before it
   // writes the pinned queue again (by then lasthalf has usually synchronised).
   PGAMD_HIP_CHECK(hipEventRecord(b.pend_ev, b.stream));
@@ -358,32 +360,40 @@ void board_predict(Board& b, int nj, hipStream_t st, bool sorted) {
   b.pred_armed = false;
   if (nj <= 0) return;
   board_use_stream(b, st);
+  hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
+                     b.js, b.jp);
+  PGAMD_HIP_CHECK(hipGetLastError());
   // the sorted records are rewritten only while the layout fits this nj
   const bool srt = sorted && b.sort_nj == nj;
-  pg6::JSort so = b.so;
-  if (!srt) so.rec = nullptr;
-  hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
-                     b.js, b.jp, so);
-  PGAMD_HIP_CHECK(hipGetLastError());
+  if (srt) {
+    hipLaunchKernelGGL(pg6::pg6_sort_records, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj,
+                       b.jp, b.so, false);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    ++b.sort_age;
+  }
   board_done_on(b, st);
   b.nj_pred = nj;
   b.srec_ok = srt;
-  if (srt) ++b.sort_age;
 }
 
-constexpr int kSortAge = 64;  // predictions between rebuilds of the sorted layout
+// The sorted layout is rebuilt (new Morton order) after kSortAge predictions
+// or once as many j-records as it holds have been written since its build:
+// between rebuilds each prediction recenters the chunk origins, and what ages
+// is how compact the chunks stay as their members move apart.
+constexpr int kSortAge = 64;
 
 // (Re)builds the sorted layout of the first nj predicted j-particles on st
 // (asynchronously): box of the positions, Morton keys, radix sort, then the
-// inverse map, chunk origins and records (pg6_sort_finish).
+// inverse map, chunk origins, flags and records (pg6_sort_records).
 void board_build_sort(Board& b, int nj, hipStream_t st) {
   const int nc = ceil_div(nj, pg6::kChunk);
   const int nbox = std::min(256, ceil_div(nj, 256));
-  // perm, inv (nj each), keys (2 nj u64), vals (2 nj), rec (2 nj float4), org (3 nc),
-  // flags, box partials (6 nbox)
-  const size_t bytes = sizeof(int) * 2 * (size_t)nj + sizeof(unsigned long long) * 2 * (size_t)nj +
-                       sizeof(int) * 2 * (size_t)nj + sizeof(float4) * 2 * (size_t)nj +
-                       sizeof(double) * (3 * (size_t)nc + 6 * (size_t)nbox + 4) + 8 * 256;
+  // keys (2 nj u64), vals (2 nj), perm, inv (nj each), rec (2 nj float4),
+  // org (3 nc), box partials (6 nbox), box (4), cflag (nc)
+  const size_t bytes = sizeof(unsigned long long) * 2 * (size_t)nj + sizeof(int) * 4 * (size_t)nj +
+                       sizeof(float4) * 2 * (size_t)nj +
+                       sizeof(double) * (3 * (size_t)nc + 6 * (size_t)nbox + 4) +
+                       sizeof(int) * (size_t)nc + 9 * 256;
   b.d_sort.reserve(bytes);
   char* base = static_cast<char*>(b.d_sort.ptr);
   auto take = [&](size_t n) {
@@ -399,7 +409,7 @@ void board_build_sort(Board& b, int nj, hipStream_t st) {
   b.so.org = reinterpret_cast<double*>(take(sizeof(double) * 3 * (size_t)nc));
   double* boxp = reinterpret_cast<double*>(take(sizeof(double) * 6 * (size_t)nbox));
   double* box = reinterpret_cast<double*>(take(sizeof(double) * 4));
-  b.so.flags = reinterpret_cast<int*>(take(sizeof(int)));
+  b.so.cflag = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nc));
   hipLaunchKernelGGL(pg6::pg6_box_part, dim3(nbox), dim3(256), 0, st, nj, b.jp, boxp);
   hipLaunchKernelGGL(pg6::pg6_box_final, dim3(1), dim3(256), 0, st, nbox, boxp, box);
   hipLaunchKernelGGL(pg6::pg6_sort_keys, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.jp,
@@ -411,12 +421,12 @@ void board_build_sort(Board& b, int nj, hipStream_t st) {
   b.d_sort_tmp.reserve(tmp);
   PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.d_sort_tmp.ptr, tmp, keys, keys + nj,
                                                      vals, b.so.perm, nj, 0, 63, st));
-  PGAMD_HIP_CHECK(hipMemsetAsync(b.so.flags, 0, sizeof(int), st));
-  hipLaunchKernelGGL(pg6::pg6_sort_finish, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.jp,
-                     b.so);
+  hipLaunchKernelGGL(pg6::pg6_sort_records, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj,
+                     b.jp, b.so, true);
   PGAMD_HIP_CHECK(hipGetLastError());
   b.sort_nj = nj;
   b.sort_age = 0;
+  b.sort_writes = 0;
   b.srec_ok = true;
 }
 
@@ -501,7 +511,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   const bool pred = board_needs_pred(b, nj);
   if (p.variant == 4) {
     // the sorted layout of the current prediction
-    const bool fresh = b.sort_nj == nj && b.sort_age < kSortAge;
+    const bool fresh = b.sort_nj == nj && b.sort_age < kSortAge && b.sort_writes < nj;
     if (pred || !b.srec_ok) {
       if (fresh) {
         board_predict(b, nj, st, true);
@@ -556,6 +566,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
       b.nj_pred = nj;
       b.predict_flag = 0;
       b.pred_armed = false;
+      b.srec_ok = false;  // the sorted records hold an older prediction
     }
     board_done_on(b, st);
     return;
@@ -1393,9 +1404,8 @@ struct Hermite {
   int* iid = nullptr;
   double* epart = nullptr;  // energy partial sums, 2 per block
   Partials part{};
-  int target = 0, min_jper = 16, max_ns = 32;  // force split (pg6h_plan)
-  int min_blocks = 0, force_blocks = 0;
-  int* tile_done = nullptr;  // per i-tile range counters of pg6h_force_finish (kept at 0)
+  int target = 0, min_jper = 16;  // force split (pg6h_plan)
+  int min_blocks = 0, force_blocks = 0, finish_blocks = 0;
   HostBuf host;  // pinned: a Ctl image, then t_end
   // a captured batch of kGraphSteps steps, and the board arrays it was captured with
   hipGraphExec_t graph = nullptr;
@@ -1423,16 +1433,17 @@ void hermite_force(Board& b, Hermite& h, int ni) {
   board_force(b, ni, h.n, h.ipos, h.ivel, h.iid, h.eps2, o, b.stream);
 }
 
-// Queues one block step (two kernels, no host synchronisation).
+// Queues one block step (three kernels, no host synchronisation).
 void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
   const int n = h.n;
   board_use_stream(b, st);
   hipLaunchKernelGGL(pg6h::pg6h_select, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, h.S, h.P,
                      h.ctl, b.js, b.jp, h.act, h.ipos, h.ivel, h.iid);
-  hipLaunchKernelGGL(pg6h::pg6h_force_finish, dim3(h.force_blocks),
-                     dim3(64 * pg6h::kForceWaves), 0, st, n, (float)h.eps2, h.P, h.ctl,
-                     h.target, h.min_jper, h.max_ns, h.act, h.S, h.ipos, h.ivel, h.iid, b.jp,
-                     b.js, h.part, h.tile_done);
+  hipLaunchKernelGGL(pg6h::pg6h_force, dim3(h.force_blocks), dim3(64 * pg6h::kForceWaves), 0, st,
+                     n, (float)h.eps2,
+                     h.P, h.ctl, h.target, h.min_jper, h.ipos, h.ivel, h.iid, b.jp, h.part);
+  hipLaunchKernelGGL(pg6h::pg6h_finish, dim3(h.finish_blocks), dim3(256), 0, st, n, h.P, h.ctl,
+                     h.target, h.min_jper, h.act, h.S, h.ipos, h.ivel, h.part, b.js);
   PGAMD_HIP_CHECK(hipGetLastError());
   b.nj_pred = -1;  // the board's predictions now belong to the integrator
   b.predict_flag = 1;
@@ -1515,7 +1526,9 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   h.target = 16 * cus;  // wavefront items per force launch: 4 per SIMD
   h.min_blocks = std::min(ceil_div(n, 256), 4 * cus);
   h.force_blocks = 4 * cus * 4 / pg6h::kForceWaves;  // 4 wavefronts per SIMD
-  if (const char* e = std::getenv("PGAMD_HERMITE_MAX_NS")) h.max_ns = std::max(1, std::atoi(e));
+  // blocks of 256 threads: a wavefront per active particle of a typical block
+  // step (a few hundred), and few blocks to count in Ctl::done
+  h.finish_blocks = std::max(1, std::min(ceil_div(n, 256), cus));
   // largest force split: ns * ntiles <= target / kForceWaves + ntiles (pg6h_plan)
   const size_t nsplit_i = ((size_t)h.target / pg6h::kForceWaves + ceil_div(n, 64)) * 64;
   // state: x, v, a, j (3n each), pot, t, dt, m (n each)
@@ -1533,7 +1546,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   // nsplit_i), Ctl; act, iid (n each)
   const size_t dbytes =
       sizeof(double) * ((size_t)n * 13 + 2 * kHermBlocks + 7 * nsplit_i) + 256;
-  const size_t ibytes = sizeof(int) * ((size_t)n * 2 + ceil_div(n, 64));
+  const size_t ibytes = sizeof(int) * ((size_t)n * 2);
   h.work.reserve(dbytes + ibytes);
   double* w = h.work.as<double>();
   h.ipos = w;
@@ -1553,7 +1566,6 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   int* wi = reinterpret_cast<int*>(reinterpret_cast<char*>(h.ctl) + 256);
   h.act = wi;
   h.iid = wi + n;
-  h.tile_done = wi + 2 * (size_t)n;
   h.host.reserve(512);
   hipStream_t st = b.stream;
   // upload x, v, m; t = 0, a = j = 0
@@ -1563,7 +1575,6 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.a, 0, sizeof(double) * 6 * (size_t)n, st));
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.t, 0, sizeof(double) * (size_t)n, st));
   PGAMD_HIP_CHECK(hipMemsetAsync(h.S.dt, 0, sizeof(double) * (size_t)n, st));
-  PGAMD_HIP_CHECK(hipMemsetAsync(h.tile_done, 0, sizeof(int) * (size_t)ceil_div(n, 64), st));
   // j-memory = the particles at t = 0 (set_particle_on_grape, banana01.c:206-236):
   // the start-up kernel below writes it; the i-set is every particle as is.
   PGAMD_HIP_CHECK(hipMemcpyAsync(h.ipos, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));

@@ -152,38 +152,25 @@ __device__ inline void pg6_predict_j(int j, bool valid, double ti, const JState&
 // Block = 256 threads, one j per thread.
 // Sorted j layout for pg6_force_o: the predicted j-set in the order of a
 // Morton sort of their positions (taken when the layout was built), in
-// chunks of kChunk; chunk c carries an FP64 origin org[3c..3c+2] (the centre
-// of its members' box when built) and member k (sorted position) the record
-// rec[2k] = {x - o, y - o, z - o (FP32), m}, rec[2k+1] = {vx, vy, vz, id bits}.
-// perm[k] = j address of sorted position k, inv[j] = k; flags != 0 when some
-// id differs from its address (then pg6_force_o tests the self pair by id on
-// every chunk).
+// chunks of kChunk.  Rewritten after every prediction by pg6_sort_records:
+// chunk c carries an FP64 origin org[3c..3c+2] (the centre of its members'
+// current box), a flag cflag[c] != 0 when some member's id differs from its
+// address (then pg6_force_o tests the self pair by id on that chunk), and
+// member k (sorted position) the record rec[2k] = {x - o, y - o, z - o
+// (FP32), m}, rec[2k+1] = {vx, vy, vz, id bits}.  perm[k] = j address of
+// sorted position k, inv[j] = k.
 struct JSort {
   int* perm;
   int* inv;
   float4* rec;
   double* org;
-  int* flags;
+  int* cflag;
 };
 
-__device__ inline void pg6_sorted_rec(const JSort& so, int k, double x, double y, double z,
-                                      float4 vm, int id, const double* o) {
-  so.rec[2 * (size_t)k] = make_float4((float)(x - o[0]), (float)(y - o[1]), (float)(z - o[2]),
-                                      vm.w);
-  so.rec[2 * (size_t)k + 1] = make_float4(vm.x, vm.y, vm.z, __int_as_float(id));
-}
-
-// Block = 256 threads, one j per thread.  so.rec != null: the sorted records
-// of the predicted j are written too (through so.inv).
-__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp,
-                                                   JSort so) {
+// Block = 256 threads, one j per thread.
+__global__ __launch_bounds__(256) void pg6_predict(int nj, double ti, JState js, JPred jp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   pg6_predict_j(j, j < nj, ti, js, jp);
-  if (so.rec && j < nj) {
-    const int k = so.inv[j];
-    pg6_sorted_rec(so, k, jp.x[j], jp.y[j], jp.z[j], jp.vm[j], jp.id[j],
-                   so.org + 3 * (size_t)(k / kChunk));
-  }
 }
 
 // ---------------------------------------------------------- sorted layout
@@ -265,22 +252,24 @@ __global__ void pg6_sort_keys(int nj, JPred jp, const double* __restrict__ box,
   vals[j] = j;
 }
 
-// After the sort (perm = addresses in key order): inverse map, chunk origins
-// (centre of each chunk's box, one wavefront per chunk), the id check, and
-// the sorted records of the current prediction.  Block = 256 threads.
-__global__ __launch_bounds__(256) void pg6_sort_finish(int nj, JPred jp, JSort so) {
+// Chunk origins, id flags and records of the sorted layout from the current
+// prediction (after the sort, also the inverse map: with_inv).  One thread
+// per sorted position k, one wavefront per chunk; block = 256 threads.
+__global__ __launch_bounds__(256) void pg6_sort_records(int nj, JPred jp, JSort so,
+                                                        bool with_inv) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool v = k < nj;
   const int j = v ? so.perm[k] : 0;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   double p[3] = {0.0, 0.0, 0.0};
+  int id = 0;
   if (v) {
-    so.inv[j] = k;
+    if (with_inv) so.inv[j] = k;
     p[0] = jp.x[j];
     p[1] = jp.y[j];
     p[2] = jp.z[j];
+    id = jp.id[j];
     for (int d = 0; d < 3; ++d) lo[d] = hi[d] = p[d];
-    if (jp.id[j] != j) atomicOr(so.flags, 1);
   }
   for (int o = 32; o > 0; o >>= 1)
     for (int d = 0; d < 3; ++d) {
@@ -290,9 +279,17 @@ __global__ __launch_bounds__(256) void pg6_sort_finish(int nj, JPred jp, JSort s
   // blocks of 256 threads hold four whole chunks of kChunk = 64; after the
   // butterfly every lane has its chunk's box
   const double o[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
-  if ((k % kChunk) == 0 && v)
+  const bool mixed = __builtin_amdgcn_ballot_w64(v && id != j) != 0ull;
+  if ((k % kChunk) == 0 && v) {
     for (int d = 0; d < 3; ++d) so.org[3 * (size_t)(k / kChunk) + d] = o[d];
-  if (v) pg6_sorted_rec(so, k, p[0], p[1], p[2], jp.vm[j], jp.id[j], o);
+    so.cflag[k / kChunk] = mixed ? 1 : 0;
+  }
+  if (v) {
+    const float4 vm = jp.vm[j];
+    so.rec[2 * (size_t)k] = make_float4((float)(p[0] - o[0]), (float)(p[1] - o[1]),
+                                        (float)(p[2] - o[2]), vm.w);
+    so.rec[2 * (size_t)k + 1] = make_float4(vm.x, vm.y, vm.z, __int_as_float(id));
+  }
 }
 
 
@@ -758,23 +755,24 @@ __global__ __launch_bounds__(kThreads) void pg6_force_h(
 
 // Pair loop of pg6_force_o for the i-particle of this lane against the
 // sorted positions [k0, k1) of the sorted layout (JSort):
-//  * dx = (u_j - d_i) - l_i, two FP32 operations: u_j = x_j - o_c (FP32,
-//    stored), d_i = FP32(x_i - o_c) and l_i = FP32((x_i - o_c) - d_i) formed
-//    in FP64 once per chunk and lane.  The i offset is exact to FP32 order
-//    in l_i; the j offset's rounding is at most 2^-24 |x_j - o_c|, half an ulp
-//    of the chunk's size, so a close pair keeps its digits when the chunk is
-//    small (the Morton order keeps it so);
-//  * the self pair is tested by id only on chunks that hold this wavefront's
-//    i itself (self_k = the i's sorted position, from inv when ids are the
-//    addresses; -1 with flags set: every chunk is tested).
-// FP32 sums over tiles of kJTile positions, folded into FP64.  rmax/nn as
-// pg6_range_h, nn converted to the j address.  Every lane of the wavefront
-// calls it with the same k0, k1 (the records are read by scalar loads).
+//  * dx = u_j - d_i, one FP32 operation: u_j = x_j - o_c (FP32, stored) and
+//    d_i = FP32(x_i - o_c), formed in FP64 once per chunk and lane.  The two
+//    roundings are at most 2^-24 |x_j - o_c| and 2^-24 |x_i - o_c|, with
+//    |x_i - o_c| <= |dx| + |x_j - o_c|: an FP32 rounding of dx itself plus
+//    ulps of the chunk's size, so a close pair keeps its digits when the
+//    chunk is small (the Morton order keeps it so);
+//  * the self pair is tested by id only on chunks that hold the sorted
+//    position of some lane's address idi (self_k = inv[idi], -1 when idi is
+//    no address) or a member whose id differs from its address (cflag); the
+//    j whose id is idi sits in one of them.
+// FP32 sums over each chunk (at most kChunk positions), folded into FP64.
+// rmax/nn as pg6_range_h per tile of kJTile positions, nn converted to the j
+// address.  Every lane of the wavefront calls it with the same k0, k1 (the
+// records are read by scalar loads).
 template <bool NN>
 __device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi, float vxi,
                                                  float vyi, float vzi, int idi, int self_k,
-                                                 bool test_all, float eps2, int k0, int k1,
-                                                 const JSort& so) {
+                                                 float eps2, int k0, int k1, const JSort& so) {
   cdouble* porg = (cdouble*)so.org;
   RangeSums r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f, -1};
   const unsigned kmask = 0xFFFFFF00u;
@@ -786,18 +784,15 @@ __device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi
       const int ch = (kt + c) / kChunk;
       ce = min(nt, (ch + 1) * kChunk - kt);  // to the end of the chunk or the tile
       const double ox = porg[3 * ch], oy = porg[3 * ch + 1], oz = porg[3 * ch + 2];
-      const double ex = xi - ox, ey = yi - oy, ez = zi - oz;
-      const float dxi = (float)ex, dyi = (float)ey, dzi = (float)ez;
-      const float lxi = (float)(ex - (double)dxi);
-      const float lyi = (float)(ey - (double)dyi);
-      const float lzi = (float)(ez - (double)dzi);
-      const bool self_here = test_all || __builtin_amdgcn_ballot_w64(self_c == ch) != 0ull;
+      const float dxi = (float)(xi - ox), dyi = (float)(yi - oy), dzi = (float)(zi - oz);
+      const bool self_here =
+          __builtin_amdgcn_ballot_w64(self_c == ch) != 0ull || so.cflag[ch] != 0;
       cfloat_s* q = (cfloat_s*)(so.rec + 2 * (size_t)(kt + c));
 #define PG6_PAIR_O(SELF)                                                                   \
   {                                                                                        \
-    const float dx = (q[0] - dxi) - lxi;                                                   \
-    const float dy = (q[1] - dyi) - lyi;                                                   \
-    const float dz = (q[2] - dzi) - lzi;                                                   \
+    const float dx = q[0] - dxi;                                                           \
+    const float dy = q[1] - dyi;                                                           \
+    const float dz = q[2] - dzi;                                                           \
     pg6_pair_h<SELF, NN>(t, dx, dy, dz, q[4] - vxi, q[5] - vyi, q[6] - vzi, q[3], eps2,    \
                          SELF && __float_as_int(q[7]) == idi, (unsigned)(255 - jj));       \
   }
@@ -809,6 +804,16 @@ __device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi
         for (int jj = c; jj < ce; ++jj, q += 8) PG6_PAIR_O(false)
       }
 #undef PG6_PAIR_O
+      // fold the chunk's FP32 sums: neighbouring chunks hold close j on
+      // opposite sides of nearby i, whose large terms partly cancel
+      r.ax += (double)t.ax;
+      r.ay += (double)t.ay;
+      r.az += (double)t.az;
+      r.pot += (double)t.pot;
+      r.jx += (double)t.jx;
+      r.jy += (double)t.jy;
+      r.jz += (double)t.jz;
+      t.ax = t.ay = t.az = t.pot = t.jx = t.jy = t.jz = 0.f;
     }
     if (NN) {
       const float rq = __uint_as_float(t.key & kmask);
@@ -817,13 +822,6 @@ __device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi
         r.nn = kt + 255 - (int)(t.key & 0xFFu);
       }
     }
-    r.ax += (double)t.ax;
-    r.ay += (double)t.ay;
-    r.az += (double)t.az;
-    r.pot += (double)t.pot;
-    r.jx += (double)t.jx;
-    r.jy += (double)t.jy;
-    r.jz += (double)t.jz;
   }
   if (NN && r.nn >= 0) r.nn = so.perm[r.nn];
   return r;
@@ -844,12 +842,11 @@ __global__ __launch_bounds__(kThreads) void pg6_force_o(
   const double yi = ipos[3 * ic + 1];
   const double zi = ipos[3 * ic + 2];
   const int idi = iid[ic];
-  const bool test_all = *(volatile const int*)so.flags != 0;
-  const int self_k = (!test_all && idi >= 0 && idi < nj) ? so.inv[idi] : -1;
+  const int self_k = (idi >= 0 && idi < nj) ? so.inv[idi] : -1;
   const int j0 = blockIdx.y * jper;
   const RangeSums r = pg6_range_o<NN>(xi, yi, zi, (float)ivel[3 * ic + 0],
                                       (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], idi,
-                                      self_k, test_all, eps2, j0, min(nj, j0 + jper), so);
+                                      self_k, eps2, j0, min(nj, j0 + jper), so);
   if (!ivalid) return;
   if (nsplit == 1) {
     out.acc[3 * i + 0] = r.ax;

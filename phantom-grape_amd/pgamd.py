@@ -413,7 +413,7 @@ class G5Table:
         _sig(L, "pg5_mlp_set_params", C.c_int, _dp, C.c_int, C.c_double, C.c_double)
         _sig(L, "pg5_mlp_get_params", C.c_int, _dp, _ip, _dp, _dp)
         _sig(L, "pg5_mlp_normal_eq", C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp)
-        _sig(L, "pg5_mlp_train", C.c_int, C.c_int, C.c_int, C.c_uint, _dp)
+        _sig(L, "pg5_mlp_train", C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _dp)
         _sig(L, "pg5_mlp_eval_dev", C.c_int, C.c_int, _vp, _vp)
         _sig(L, "pg5t_dev_eval", C.c_int, C.c_int, _vp, _vp)
         _sig(L, "pg5t_dev_force", C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp)
@@ -457,11 +457,12 @@ class G5Table:
     def mlp_set_params(self, p, form, rc, r0):
         assert self.lib.pg5_mlp_set_params(dptr(f64(p)), int(form), float(rc), float(r0)) == 0
 
-    def mlp_train(self, iters=200, starts=4, seed=1):
-        """pg5_mlp_train: Levenberg-Marquardt from `starts` seeded starts;
-        returns the final weighted MSE."""
+    def mlp_train(self, iters=200, starts=4, refine=1000, seed=1):
+        """pg5_mlp_train: Levenberg-Marquardt from `starts` seeded starts of
+        `iters` trial steps, the best refined by `refine` more; returns the
+        final weighted MSE."""
         loss = np.zeros(1)
-        rc = self.lib.pg5_mlp_train(int(iters), int(starts), int(seed), dptr(loss))
+        rc = self.lib.pg5_mlp_train(int(iters), int(starts), int(refine), int(seed), dptr(loss))
         assert rc == 0
         return float(loss[0])
 

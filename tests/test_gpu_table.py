@@ -31,6 +31,9 @@ def tl():
     sig(L, "g5_set_mj", None, C.c_int, C.c_int, dp)
     sig(L, "g5_set_n", None, C.c_int)
     sig(L, "g5_calculate_force_on_x", None, dp, dp, dp, C.c_int)
+    sig(L, "g5_set_xmjMC", None, C.c_int, C.c_int, C.c_int, dp, dp)
+    sig(L, "g5_set_nMC", None, C.c_int, C.c_int)
+    sig(L, "g5_calculate_force_on_xMC", None, C.c_int, dp, dp, dp, C.c_int)
     sig(L, "pg5t_dev_eval", C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p)
     L.pg5t_dev_eval.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
     L.g5_open()
@@ -134,17 +137,27 @@ def test_set_xj_and_set_mj(tl):
 
 
 def test_n_beyond_the_addresses_written(tl):
-    """g5_set_n past the addresses set adds zero records (no force)."""
+    """g5_set_n past the addresses set adds zero records (no force).  On a
+    board no other test writes (devid 37): addresses another call wrote keep
+    their records, as in the reference's static j-memory."""
     import pgamd
+    dev = 37
     tl.pg5_gen_s2_force_table(PP, PM)
     m, x = ics.clumpy_box(1000, 33)
-    a0 = force(tl, x[:50], x, m)
-    scale = np.linalg.norm(a0, axis=1).max()
     xi = pgamd.f64(x[:50])
-    for n in (3000, 20000):
-        tl.g5_set_n(n)
+
+    def calc(n):
+        tl.g5_set_nMC(dev, n)
         a = np.zeros((50, 3))
         p = np.zeros(50)
-        tl.g5_calculate_force_on_x(pgamd.dptr(xi), pgamd.dptr(a), pgamd.dptr(p), 50)
+        tl.g5_calculate_force_on_xMC(dev, pgamd.dptr(xi), pgamd.dptr(a), pgamd.dptr(p), 50)
+        return a
+
+    tl.g5_set_xmjMC(dev, 0, len(m), pgamd.dptr(pgamd.f64(x)), pgamd.dptr(pgamd.f64(m)))
+    a0 = calc(len(m))
+    np.testing.assert_allclose(a0, force(tl, x[:50], x, m), rtol=1e-6, atol=1e-12)
+    scale = np.linalg.norm(a0, axis=1).max()
+    for n in (3000, 20000):
+        a = calc(n)
         assert np.all(np.isfinite(a))
         assert rel_err(a, a0, floor=1e-3 * scale).max() < 1e-6
