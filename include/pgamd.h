@@ -36,6 +36,16 @@ int pg5_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
 int pg5_dev_quad(int devid, int ni, const void *xi, int nc, const void *cells,
                  double *acc, double *pot, void *stream);
 void *pg5_dev_stream(int devid);
+/* Allocation-free form of pg5_dev_force, for callers that cannot allocate
+ * device memory (HIP graph capture, serving loops): no counterpart in the
+ * reference.  The j-range partial sums live in a static device array of
+ * 16 MiB per GPU (the split is reduced until they fit: ni * nsplit * 32 bytes
+ * <= 16 MiB); no board, stream or host buffer is created, `stream` (null =
+ * the default stream) carries the work.  Every call on one GPU shares that
+ * array: calls that overlap in time on one GPU race.  Results equal
+ * pg5_dev_force's whenever its split fits. */
+int pg5_dev_force_noalloc(int ni, const void *xi, int nj, const void *xj, const float *epsj2,
+                          double *acc, double *pot, void *stream);
 /* rsqrt-bias correction of the g5_* outputs, the reference's rule
  * (PG5/newton/libpg5/rsqrt.c:42-81 measures the bias b of the rsqrt the
  * kernel uses; phantom_g5.c:45-60,244-269 scales acc by float(1 - 3b) and pot

@@ -714,6 +714,96 @@ PGAMD_API int pg5_dev_force(int devid, int ni, const void* xi, int nj, const voi
   return 0;
 }
 
+// ------------------------------------------------ allocation-free variant
+// pg5_dev_force_noalloc keeps its j-range partial sums in this file-scope
+// device array instead of memory the library allocates: every GPU the library
+// runs on holds one copy (kScratchBytes) until the process ends.  Host code
+// never passes `scratch` itself to a kernel (in host code the name is only a
+// stand-in); it uses the device address looked up per device with
+// hipGetSymbolAddress on the first call on that device.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxGpus = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxGpus] = {};
+
+// Device address of this device's scratch array (current device), or null
+// after reporting an error.
+double* scratch_ptr(const char* fn) {
+  int dev = 0;
+  PGAMD_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxGpus) {
+    api_error(4, "%s: device %d outside [0, %d)", fn, dev, kMaxGpus);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      api_error(7, "%s: no device address for the scratch array on device %d (%s)", fn, dev,
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+__global__ void zero_outputs(int ni, double* acc, double* pot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  acc[3 * i + 0] = 0.0;
+  acc[3 * i + 1] = 0.0;
+  acc[3 * i + 2] = 0.0;
+  pot[i] = 0.0;
+}
+
+}  // namespace
+
+// Work of pg5_dev_force with no memory allocated by the call: the partial
+// sums live in a static device array of 16 MiB per GPU, shared by every call
+// on that GPU (calls that overlap in time on one GPU race on it).  The j-range
+// split is the one pg5_dev_force plans, reduced until the partials fit
+// (ni * nsplit * 32 bytes <= 16 MiB; one range needs none), so the results
+// equal pg5_dev_force's whenever its plan fits.  No board, stream or host
+// buffer is created: `stream` (null = the default stream) carries the work.
+// For callers that cannot allocate (HIP graph capture, serving loops).
+PGAMD_API int pg5_dev_force_noalloc(int ni, const void* xi, int nj, const void* xj,
+                                    const float* epsj2, double* acc, double* pot,
+                                    void* stream) {
+  if (ni < 0 || nj < 0 || (ni > 0 && (!xi || !acc || !pot)) || (nj > 0 && !xj) ||
+      (kSym && nj > 0 && !epsj2)) {
+    api_error(3, "pg5_dev_force_noalloc: ni=%d nj=%d (need ni, nj >= 0 and non-null arrays)",
+              ni, nj);
+    return -1;
+  }
+  if (ni == 0) return 0;
+  ensure_device();
+  double* sp = scratch_ptr("pg5_dev_force_noalloc");
+  if (!sp) return -1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Outputs5 o{acc, pot};
+  if (nj == 0) {
+    hipLaunchKernelGGL(zero_outputs, dim3(ceil_div(ni, 256)), dim3(256), 0, st, ni, acc, pot);
+    PGAMD_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  Plan p = plan(ni, nj);
+  const size_t cap = kScratchBytes / (4 * sizeof(double) * (size_t)ni);  // splits that fit
+  if ((size_t)p.nsplit > cap) {
+    const int ns = (int)std::max<size_t>(cap, 1);
+    p.jper = (ceil_div(nj, ns) + 63) / 64 * 64;
+    p.nsplit = ceil_div(nj, p.jper);
+  }
+  launch_force(p, ni, static_cast<const float4*>(xi), nj, static_cast<const float4*>(xj), epsj2,
+               nullptr, partials_at(sp, p.nsplit, ni), o, st);
+  return 0;
+}
+
 // Cell force of nc cells (device array of 12 floats each: x, y, z, M, Qxx,
 // Qxy, Qxz, Qyy, Qyz, Qzz, 0, 0) on ni i-particles; pot = + sum (phi_p+phi_q).
 PGAMD_API int pg5_dev_quad(int devid, int ni, const void* xi, int nc, const void* cells,
