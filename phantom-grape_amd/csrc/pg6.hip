@@ -72,15 +72,15 @@ inline int ni_bucket(int ni) {
 }
 
 // Default choice, from the autotuner's picks at nj = 65536 on MI355X: the
-// LDS-tile kernel for small and medium i-sets, the two-float scalar-broadcast
-// kernel for large ones, and many j-ranges so that the last wave of blocks is
-// a small fraction of the launch.
+// LDS-tile kernel for small and medium i-sets, the sorted chunk-origin kernel
+// for large ones (the two-float kernel below kSortedMinNj j), and many
+// j-ranges so that the last wave of blocks is a small fraction of the launch.
 inline LaunchChoice default_choice(int ni) {
   if (ni <= 128) return {0, 64, 32};
-  if (ni <= 4096) return {0, 64, 8};
-  if (ni <= 8192) return {2, 128, 16};
-  if (ni <= 32768) return {2, 128, 32};
-  return {2, 128, 64};
+  if (ni <= 2048) return {0, 64, 4};
+  if (ni <= 8192) return {4, 256, 16};
+  if (ni <= 32768) return {4, 64, 32};
+  return {4, 128, 64};
 }
 
 struct Board {
@@ -1284,7 +1284,8 @@ __global__ void pg6_gather_i(int n, JPred jp, double* pos, double* vel, int* id)
 // min(65536, nj), times every (kernel, bi, blocks_per_cu) candidate on the
 // first nj predicted j-particles (i-set = the first ni of them; force, jerk,
 // potential and nearest neighbour, j-range reduction included) and keeps the
-// fastest for that bucket; plan_launch uses it for every ni in the bucket.
+// fastest for that bucket (the default shape unless one is more than 1 %
+// faster); plan_launch uses it for every ni in the bucket.
 // Call after pg6_dev_predict (or g6calc_firsthalf) so the predicted set is
 // representative.  out[4 * k .. 4k+3] = {variant, bi, bpc, microseconds} per
 // bucket k (may be NULL); returns the number of buckets tuned.  Explicit
@@ -1325,17 +1326,20 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   g_timer.on = false;
   const int save_bi = b.force_bi, save_bpc = b.force_bpc, save_var = b.force_variant;
   static const int kBpc[] = {4, 8, 16, 32, 64};
-  static const int kBpcF[] = {1, 2, 4, 8, 16};  // variant 3: 64 .. 1024 blocks
   int tuned = 0;
   for (int k = 0; k < kBuckets && (64 << k) <= nimax; ++k) {
     const int ni = 64 << k;
     float best = 1e30f;
-    LaunchChoice bc = default_choice(ni);
+    const LaunchChoice dc = default_choice(ni);
+    LaunchChoice bc = dc;
+    float t_def = 1e30f;  // the default's time, if it is a candidate here
+    // Variant 3 is not a candidate: its advantage, the prediction inside the
+    // launch, is not exercised by repeated calls on one prediction (and with
+    // it the kernel measured slower than the split path).
     for (int var = 0; var <= 4; ++var)
       for (int bi : {64, 128, 256})
-        for (int bk = 0; bk < 5; ++bk) {
-          const int bpc = var == 3 ? kBpcF[bk] : kBpc[bk];
-          if (var == 3 && (bi != 64 || ni > kFusedMaxNi)) continue;
+        for (int bpc : kBpc) {
+          if (var == 3) continue;
           if (var == 4 && nj < kSortedMinNj) continue;
           b.force_variant = var;
           b.force_bi = bi;
@@ -1347,11 +1351,18 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
           PGAMD_HIP_CHECK(hipEventSynchronize(e1));
           float ms = 0.f;
           PGAMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (var == dc.variant && bi == dc.bi && bpc == dc.bpc) t_def = ms;
           if (ms < best) {
             best = ms;
             bc = {var, bi, bpc};
           }
         }
+    // keep the default unless a candidate beats it by more than 1 % (run-to-run
+    // noise would otherwise pick a different shape on every call)
+    if (t_def <= 1.01f * best) {
+      bc = dc;
+      best = t_def;
+    }
     b.tuned[k] = bc;
     b.tuned_ok[k] = true;
     // the largest bucket also serves every larger ni

@@ -1208,14 +1208,20 @@ __global__ __launch_bounds__(kFThreads) void pg6_force_f(
     }
   }
   if (nsplit == 1) return;
-  // Release this block's partials (and, with PRED, its predicted j), then
-  // count it done; the last block of the i-block acquires everything.
-  __threadfence();
+  // Release this block's partials (and, with PRED, its predicted j) at agent
+  // scope once the block is done with them, then count it done; the last
+  // block of the i-block acquires everything.  One release per block and one
+  // acquire in the last block only: on this multi-L2 part an agent-scope
+  // release writes back the L2 and an acquire invalidates it, so a full fence
+  // in every block costs far more than the pair work of a small call.
   __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(&counter[blockIdx.x], 1) == nsplit - 1;
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    s_last = atomicAdd(&counter[blockIdx.x], 1) == nsplit - 1;
+  }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   const int q = w;  // thread (q, lane): i = lane of this i-block, splits q, q + 8, ...
   double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   float rmax = 0.f;

@@ -212,9 +212,10 @@ def test_autotune_and_launch_shapes(g6):
     tuned = g6.autotune(n, eps * eps, reps=1)
     assert sorted(tuned) == [64 << k for k in range(7)]
     for ni, (var, bi, bpc, us) in tuned.items():
-        # variant 4 (sorted chunk origins) is a candidate only from 16384 j
-        assert var in (0, 1, 2, 3) and bi in (64, 128, 256) and us > 0
-        assert bpc in ((1, 2, 4, 8, 16) if var == 3 else (4, 8, 16, 32, 64))
+        # variant 4 (sorted chunk origins) is a candidate only from 16384 j,
+        # variant 3 (one launch) not at all
+        assert var in (0, 1, 2) and bi in (64, 128, 256) and us > 0
+        assert bpc in (4, 8, 16, 32, 64)
         p = g6.plan(ni, n)
         assert (p["variant"], p["bi"], p["bpc"]) == (var, bi, bpc)
         assert p["nsplit"] * p["jper"] >= n
