@@ -1318,6 +1318,7 @@ struct PmCache {
   hipfftHandle fwd{}, inv{};
   DevBuf rho, rhok, fk, f;
   DevBuf keys, order, sort_tmp;  // particle sort by mesh cell (pm_keys)
+  DevBuf fix;  // pm_mass_part's block sums, then the fixed-point scale
 };
 PmCache g_pm;
 std::mutex g_pm_mutex;
@@ -1389,8 +1390,16 @@ PGAMD_API int pg5t_pm_force(int n, const double* pos, const double* mass, int me
     PGAMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.ptr, tmp, kin, kin + n, vin,
                                                        vin + n, n, 0, end_bit, st));
     order = vin + n;
+    c.fix.reserve(sizeof(double) * (pm::kMassBlocks + 1));
+    double* part = c.fix.as<double>();
+    double* scale = part + pm::kMassBlocks;
+    hipLaunchKernelGGL(pm::pm_mass_part, dim3(pm::kMassBlocks), dim3(256), 0, st, n, mass, part);
+    hipLaunchKernelGGL(pm::pm_fix_scale, dim3(1), dim3(64), 0, st, part, mesh, scale);
     hipLaunchKernelGGL(pm::pm_assign_sorted, dim3(ceil_div(n, pm::kAssignRun)),
-                       dim3(pm::kAssignThreads), 0, st, n, order, pos, mass, mesh, rho);
+                       dim3(pm::kAssignThreads), 0, st, n, order, pos, mass, mesh, scale,
+                       reinterpret_cast<unsigned long long*>(rho));
+    hipLaunchKernelGGL(pm::pm_fix_to_double, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0,
+                       st, nr, scale, rho);
     PGAMD_HIP_CHECK(hipGetLastError());
   }
   PGAMD_FFT_CHECK(hipfftExecD2Z(c.fwd, rho, reinterpret_cast<hipfftDoubleComplex*>(rhok)));

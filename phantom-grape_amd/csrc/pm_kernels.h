@@ -9,8 +9,11 @@
 //      at integer multiples of h = 1/M): the particles are sorted by the
 //      Morton code of their nearest mesh point, each block sums a run of
 //      kAssignRun of them into a box of the mesh held in LDS and adds the
-//      box to the mesh with FP64 atomics (a run whose box does not fit adds
-//      straight to the mesh);
+//      box to the mesh (a run whose box does not fit adds straight to the
+//      mesh).  The sums are 64-bit fixed point (pm_fix_scale): integer
+//      addition is associative, so the mesh does not depend on the order in
+//      which the atomics land and the force is the same bit for bit on
+//      every run;
 //   2. real-to-complex FFT (hipFFT, FP64);
 //   3. Green's function -4 pi S(k a)^2 / (k^2 W(k)^2), with S the S2 form
 //      factor and W the TSC window, times -i k for each axis (spectral
@@ -74,15 +77,54 @@ constexpr int kAssignPer = 1;                               // particles per thr
 constexpr int kAssignRun = kAssignThreads * kAssignPer;     // particles per block
 constexpr int kBoxCells = 6144;                             // LDS box: 48 KB of FP64
 
+// Fixed-point scale of the mass assignment.  Every mesh point receives at
+// most sum |m| M^3 (TSC weights of a particle sum to 1), so with
+// 2^e <= 2^62 / (sum |m| M^3) no point and no partial sum can overflow a
+// signed 64-bit integer; one unit is 2^-e, a relative resolution of
+// M^3 / 2^62 of the mean density (4.5e-13 at M = 128).
+constexpr int kMassBlocks = 256;
+
+// Per-block sums of |m| (FP64, fixed order): part[kMassBlocks].  Block = 256.
+__global__ __launch_bounds__(256) void pm_mass_part(int n, const double* __restrict__ mass,
+                                                    double* __restrict__ part) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += kMassBlocks * 256) v += fabs(mass[p]);
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// scale[0] = 2^e from the block sums (one thread, in block order).
+__global__ void pm_fix_scale(const double* __restrict__ part, int M, double* __restrict__ scale) {
+  if (threadIdx.x != 0) return;
+  double sum = 0.0;
+  for (int b = 0; b < kMassBlocks; ++b) sum += part[b];
+  const double bound = sum * (double)M * (double)M * (double)M;
+  int e = 0;
+  if (bound > 0.0) {
+    frexp(bound, &e);  // bound < 2^e
+    e = 62 - e;
+  }
+  scale[0] = ldexp(1.0, e);
+}
+
 // TSC assignment of the sorted run [blockIdx.x * kAssignRun, ...) of
-// particles (order[] = particle indices).  The run's stencils span the mesh
-// box [lo, hi] (unwrapped nearest points +-1); if it holds at most kBoxCells
-// points the run is summed in LDS and the box added to rho point by point,
-// else every stencil point goes straight to rho.
+// particles (order[] = particle indices) as fixed-point integers (units of
+// 1 / scale[0]) into rho (M^3 signed 64-bit integers, two's complement sums
+// through unsigned atomics).  The run's stencils span the mesh box [lo, hi]
+// (unwrapped nearest points +-1); if it holds at most kBoxCells points the
+// run is summed in LDS and the box added to rho point by point, else every
+// stencil point goes straight to rho.
 __global__ __launch_bounds__(kAssignThreads) void pm_assign_sorted(
     int n, const int* __restrict__ order, const double* __restrict__ pos,
-    const double* __restrict__ mass, int M, double* __restrict__ rho) {
-  __shared__ double box[kBoxCells];
+    const double* __restrict__ mass, int M, const double* __restrict__ scale,
+    unsigned long long* __restrict__ rho) {
+  __shared__ unsigned long long box[kBoxCells];
   __shared__ int s_lo[3], s_hi[3];
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * kAssignRun;
@@ -111,10 +153,10 @@ __global__ __launch_bounds__(kAssignThreads) void pm_assign_sorted(
   const bool in_lds = (long long)bx * by * bz <= kBoxCells;  // uniform over the block
   const int ox = s_lo[0], oy = s_lo[1], oz = s_lo[2];
   if (in_lds) {
-    for (int t = tid; t < bx * by * bz; t += kAssignThreads) box[t] = 0.0;
+    for (int t = tid; t < bx * by * bz; t += kAssignThreads) box[t] = 0ull;
     __syncthreads();
   }
-  const double m3 = (double)M * (double)M * (double)M;
+  const double m3 = (double)M * (double)M * (double)M * scale[0];
   for (int q = 0; q < kAssignPer; ++q) {
     const int k = k0 + q * kAssignThreads + tid;
     if (k >= n) break;
@@ -123,29 +165,38 @@ __global__ __launch_bounds__(kAssignThreads) void pm_assign_sorted(
     const int gx = tsc(pos[3 * p] * M, wx);
     const int gy = tsc(pos[3 * p + 1] * M, wy);
     const int gz = tsc(pos[3 * p + 2] * M, wz);
-    const double qm = mass[p] * m3;  // mass / h^3
+    const double qm = mass[p] * m3;  // mass / h^3 in fixed-point units
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) {
         const double wab = qm * wx[a] * wy[b];
         for (int c = 0; c < 3; ++c) {
+          const unsigned long long v = (unsigned long long)llrint(wab * wz[c]);
           if (in_lds)
             atomicAdd(&box[((gx + a - 1 - ox) * by + (gy + b - 1 - oy)) * bz + (gz + c - 1 - oz)],
-                      wab * wz[c]);
+                      v);
           else
             atomicAdd(&rho[((size_t)wrap(gx + a - 1, M) * M + wrap(gy + b - 1, M)) * M +
                            wrap(gz + c - 1, M)],
-                      wab * wz[c]);
+                      v);
         }
       }
   }
   if (!in_lds) return;
   __syncthreads();
   for (int t = tid; t < bx * by * bz; t += kAssignThreads) {
-    const double v = box[t];
-    if (v == 0.0) continue;
+    const unsigned long long v = box[t];
+    if (v == 0ull) continue;
     const int ix = t / (by * bz), iy = (t / bz) % by, iz = t % bz;
     atomicAdd(&rho[((size_t)wrap(ox + ix, M) * M + wrap(oy + iy, M)) * M + wrap(oz + iz, M)], v);
   }
+}
+
+// The fixed-point mesh back to FP64 in place: rho[i] = int64(bits) / scale.
+__global__ void pm_fix_to_double(size_t nr, const double* __restrict__ scale, double* rho) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const long long v = reinterpret_cast<const long long*>(rho)[i];
+  rho[i] = (double)v / scale[0];
 }
 
 // S2 form factor of a sphere of diameter a at wave number k.

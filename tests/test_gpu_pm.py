@@ -46,6 +46,20 @@ def test_pm_matches_the_mesh_restatement(tab, mesh):
     assert np.abs((m[:, None] * g).sum(0)).max() < 1e-6 * np.abs(m[:, None] * g).sum()
 
 
+def test_mesh_force_does_not_depend_on_particle_order(tab):
+    """The mass assignment sums in 64-bit fixed point, so the order in which
+    particles (and their atomics) arrive cannot change a bit: the same
+    particles in another order get the same accelerations, bit for bit, and
+    a second call repeats the first."""
+    m, x = ics.clumpy_box(20000, 77)
+    mesh, a = 64, 3.0 / 64
+    g = pm_gpu(tab, x, m, mesh, a)
+    np.testing.assert_array_equal(pm_gpu(tab, x, m, mesh, a), g)
+    perm = np.random.default_rng(5).permutation(len(m))
+    gp = pm_gpu(tab, x[perm], m[perm], mesh, a)
+    np.testing.assert_array_equal(gp, g[perm])
+
+
 def test_treepm_total_force_is_the_periodic_force(tab):
     """Table (short range) + mesh (long range) = Ewald sum of the softened
     force, to the mesh's discretisation error at r_cut = 3 cells."""
