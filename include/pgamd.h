@@ -194,9 +194,13 @@ int pg6_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 /* Force kernel: 0 = j-tiles staged in LDS, 1 = j broadcast through scalar
  * registers, 2 = as 1 with dx formed in FP32 from two-float positions and
  * the per-pair self test skipped on j-chunks whose id range cannot hold the
- * self pair, -1 = automatic (autotuned or default per i-count).  0 and 1:
- * same arithmetic, results equal up to summation order; 2: dx within about
- * one FP32 ulp of theirs. */
+ * self pair, 3 = prediction, force and j-range reduction in one launch (up
+ * to 1024 i), 4 = j in Morton order as FP32 offsets from the FP64 origin of
+ * their 64-particle chunk (dx error grows with the chunk extent: chosen
+ * automatically only from 16384 j), 5 = as 4 with the j-range of each
+ * 64-i block split over its wavefronts, -1 = automatic (autotuned or default
+ * per i-count).  0 and 1: same arithmetic, results equal up to summation
+ * order; 2 and 3: dx within about one FP32 ulp of theirs. */
 int pg6_dev_set_kernel(int variant);
 /* Launch shape for (ni, nj): out5 = {variant, bi, blocks_per_cu, nsplit,
  * jper}. */

@@ -73,14 +73,16 @@ inline int ni_bucket(int ni) {
 
 // Default choice, from the autotuner's picks at nj = 65536 on MI355X: the
 // LDS-tile kernel for small and medium i-sets, the sorted chunk-origin kernel
-// for large ones (the two-float kernel below kSortedMinNj j), and many
-// j-ranges so that the last wave of blocks is a small fraction of the launch.
+// for large ones (the two-float kernel below kSortedMinNj j; for the largest,
+// four wavefronts per 64-i block sharing each j-range, 4x fewer partial sums
+// than one wavefront per range), and many j-ranges so that the last wave of
+// blocks is a small fraction of the launch.
 inline LaunchChoice default_choice(int ni) {
   if (ni <= 128) return {0, 64, 32};
   if (ni <= 2048) return {0, 64, 4};
   if (ni <= 8192) return {4, 256, 16};
   if (ni <= 32768) return {4, 64, 32};
-  return {4, 128, 64};
+  return {5, 256, 64};
 }
 
 struct Board {
@@ -443,9 +445,12 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   p.variant = b.force_variant >= 0 ? b.force_variant : c.variant;
   p.bpc = b.force_bpc > 0 ? b.force_bpc : c.bpc;
   if (p.variant == 3 && ni > kFusedMaxNi) p.variant = 2;
-  if (p.variant == 4 && b.force_variant != 4 && nj < kSortedMinNj) p.variant = 2;
+  if ((p.variant == 4 || p.variant == 5) && b.force_variant != p.variant && nj < kSortedMinNj)
+    p.variant = 2;
   if (p.variant == 3) p.bi = 64;
-  const int nbi = ceil_div(ni, p.bi);
+  // variant 5: blocks of bi threads on 64 i each, the j-range split over the
+  // block's bi / 64 wavefronts
+  const int nbi = ceil_div(ni, p.variant == 5 ? 64 : p.bi);
   // the SGPR-broadcast kernels run bi threads per block, the LDS one 256, the
   // one-launch kernel 512 (bpc counts quarters of a block per CU there)
   const int target = p.variant == 3 ? std::max(1, device_cu_count() * p.bpc / 4)
@@ -455,7 +460,8 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   ns = std::min(ns, maxs);
   int jper = ceil_div(nj, ns);
   static_assert(pg6::kChunk == 64, "j-ranges are aligned to the self-filter chunks");
-  jper = (jper + 63) / 64 * 64;
+  const int align = p.variant == 5 ? p.bi : 64;  // whole chunks per wavefront
+  jper = (jper + align - 1) / align * align;
   ns = std::max(1, ceil_div(nj, jper));
   p.nsplit = ns;
   p.jper = jper;
@@ -468,7 +474,11 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
                     const Partials& part, const Outputs& out, hipStream_t st) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
   g_timer.before(st);
-  if (p.variant == 4)  // ranges of sorted positions, multiples of pg6::kChunk
+  if (p.variant == 5)  // 64 i per block, jper a multiple of BI sorted positions
+    hipLaunchKernelGGL((pg6::pg6_force_ow<NN, BI / 64>), dim3(ceil_div(ni, 64), p.nsplit),
+                       dim3(BI), 0, st, ni, ipos, ivel, iid, eps2, nj, p.jper, so, jp, part, out,
+                       p.nsplit);
+  else if (p.variant == 4)  // ranges of sorted positions, multiples of pg6::kChunk
     hipLaunchKernelGGL((pg6::pg6_force_o<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
                        nj, p.jper, so, jp, part, out, p.nsplit);
   else if (p.variant == 2)  // plan_launch keeps jper a multiple of pg6::kChunk
@@ -483,12 +493,12 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
   PGAMD_HIP_CHECK(hipGetLastError());
   g_timer.after(st);
   if (p.nsplit > 1) {
-    // one thread per i walks the splits; with many splits a wavefront per i
-    // (up to 1024 i: four-thread sums would leave most CUs idle) or four
-    // threads per i (more i, coalesced loads; 2-5 % faster calls at 2048 and
-    // 4096 i against N = 16384)
+    // one thread per i walks the splits; with many splits a block of 256
+    // threads per i (up to 1024 i: four-thread sums would leave most CUs
+    // idle) or four threads per i (more i, coalesced loads; 2-5 % faster
+    // calls at 2048 and 4096 i against N = 16384)
     if (p.nsplit > 16 && ni <= 1024)
-      hipLaunchKernelGGL((pg6::pg6_reduce_w<NN>), dim3(ceil_div(ni, 4)), dim3(256), 0, st, ni,
+      hipLaunchKernelGGL((pg6::pg6_reduce_w<NN>), dim3(ni), dim3(256), 0, st, ni,
                          p.nsplit, part, jp, ipos, eps2, out);
     else if (p.nsplit > 8)
       hipLaunchKernelGGL((pg6::pg6_reduce_q4<NN>), dim3(ceil_div(ni, 64)), dim3(256), 0, st, ni,
@@ -509,7 +519,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   board_use_stream(b, st);
   const LaunchPlan p = plan_launch(b, ni, nj > 0 ? nj : 1);
   const bool pred = board_needs_pred(b, nj);
-  if (p.variant == 4) {
+  if (p.variant == 4 || p.variant == 5) {
     // the sorted layout of the current prediction
     const bool fresh = b.sort_nj == nj && b.sort_age < kSortAge && b.sort_writes < nj;
     if (pred || !b.srec_ok) {
@@ -1250,13 +1260,15 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
 // to kFusedMaxNi i; more fall back to 2), 4 = j-particles in Morton order as
 // FP32 offsets from their chunk's origin (pg6_force_o; its dx error scales
 // with the chunk extent, so the automatic choice takes it only from
-// kSortedMinNj j up; set here it runs at any nj), -1 = the autotuned or
-// default choice for each i-count.  0 and 1 compute the same arithmetic; 2
-// and 3 form dx in FP32 (within about one FP32 ulp of the others).
+// kSortedMinNj j up; set here it runs at any nj), 5 = as 4 with 64 i per
+// block and the block's j-range split over its bi / 64 wavefronts (fewer
+// partial sums; pg6_force_ow), -1 = the autotuned or default choice for each
+// i-count.  0 and 1 compute the same arithmetic; 2 and 3 form dx in FP32
+// (within about one FP32 ulp of the others).
 PGAMD_API int pg6_dev_set_kernel(int variant) {
   std::lock_guard<std::mutex> lk(g_mutex);
-  if (variant < -1 || variant > 4) {
-    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1, 2, 3, 4}", variant);
+  if (variant < -1 || variant > 5) {
+    api_error(3, "pg6_dev_set_kernel: variant %d not in {-1, 0, 1, 2, 3, 4, 5}", variant);
     return -1;
   }
   g_board.force_variant = variant;
@@ -1282,8 +1294,9 @@ __global__ void pg6_gather_i(int n, JPred jp, double* pos, double* vel, int* id)
 
 // Autotuner for the direct force: for each i-count bucket ni = 64, 128, ...,
 // min(65536, nj), times every (kernel, bi, blocks_per_cu) candidate on the
-// first nj predicted j-particles (i-set = the first ni of them; force, jerk,
-// potential and nearest neighbour, j-range reduction included) and keeps the
+// first nj j-particles (i-set = the first ni of them; prediction at the
+// board's time, force, jerk, potential and nearest neighbour, j-range
+// reduction included) and keeps the
 // fastest for that bucket (the default shape unless one is more than 1 %
 // faster); plan_launch uses it for every ni in the bucket.
 // Call after pg6_dev_predict (or g6calc_firsthalf) so the predicted set is
@@ -1326,6 +1339,7 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   g_timer.on = false;
   const int save_bi = b.force_bi, save_bpc = b.force_bpc, save_var = b.force_variant;
   static const int kBpc[] = {4, 8, 16, 32, 64};
+  static const int kBpcF[] = {1, 2, 4, 8, 16};  // variant 3: 64 .. 1024 blocks
   int tuned = 0;
   for (int k = 0; k < kBuckets && (64 << k) <= nimax; ++k) {
     const int ni = 64 << k;
@@ -1333,20 +1347,25 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
     const LaunchChoice dc = default_choice(ni);
     LaunchChoice bc = dc;
     float t_def = 1e30f;  // the default's time, if it is a candidate here
-    // Variant 3 is not a candidate: its advantage, the prediction inside the
-    // launch, is not exercised by repeated calls on one prediction (and with
-    // it the kernel measured slower than the split path).
-    for (int var = 0; var <= 4; ++var)
+    // Every timed call predicts the j-set first, as a block step does: the
+    // kernels differ in what a prediction costs them (the sorted layout's
+    // records, the one-launch kernel's in-kernel prediction).
+    for (int var = 0; var <= 5; ++var)
       for (int bi : {64, 128, 256})
-        for (int bpc : kBpc) {
-          if (var == 3) continue;
-          if (var == 4 && nj < kSortedMinNj) continue;
+        for (int bk = 0; bk < 5; ++bk) {
+          const int bpc = var == 3 ? kBpcF[bk] : kBpc[bk];
+          if (var == 3 && (bi != 64 || ni > kFusedMaxNi)) continue;
+          if ((var == 4 || var == 5) && nj < kSortedMinNj) continue;
           b.force_variant = var;
           b.force_bi = bi;
           b.force_bpc = bpc;
+          b.predict_flag = 1;
           board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);  // warm-up
           PGAMD_HIP_CHECK(hipEventRecord(e0, st));
-          for (int r = 0; r < reps; ++r) board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);
+          for (int r = 0; r < reps; ++r) {
+            b.predict_flag = 1;
+            board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);
+          }
           PGAMD_HIP_CHECK(hipEventRecord(e1, st));
           PGAMD_HIP_CHECK(hipEventSynchronize(e1));
           float ms = 0.f;

@@ -883,22 +883,109 @@ __device__ inline bool nn_better(float r, int n, float rbest, int nbest) {
   return r > rbest || (r == rbest && n >= 0 && (nbest < 0 || n < nbest));
 }
 
-// Sums the nsplit partials of each i with one wavefront per i: lane l takes
-// splits l, l+64, ... in order, then a fixed xor-butterfly combines the
-// lanes (every lane ends with the same sums, so the result does not depend
-// on which lane writes).  For many splits and few i, where one thread per i
-// would walk nsplit dependent loads.  Block = 256 threads = 4 i.
+// pg6_force_o with the j-range of a block split over its W wavefronts, all on
+// the same 64 i-particles (blockIdx.x = i-block of 64, blockIdx.y = range of
+// sorted positions, jper a multiple of W * kChunk): wave w walks the w-th
+// piece, and wave 0 adds the W sums in wave order (nearest neighbour by
+// nn_better).  W times fewer partial sums than pg6_force_o for the same
+// number of wavefronts.  Block = 64 W threads.
+template <bool NN, int W>
+__global__ __launch_bounds__(64 * W) void pg6_force_ow(
+    int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const int* __restrict__ iid, float eps2, int nj, int jper, JSort so, JPred jp, Partials part,
+    Outputs out, int nsplit) {
+  __shared__ double s_v[7][W][64];
+  __shared__ float s_r[W][64];
+  __shared__ int s_n[W][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  const bool ivalid = i < ni;
+  const int ic = ivalid ? i : ni - 1;  // padding lanes repeat i = ni-1
+  const double xi = ipos[3 * ic + 0];
+  const double yi = ipos[3 * ic + 1];
+  const double zi = ipos[3 * ic + 2];
+  const int idi = iid[ic];
+  const int self_k = (idi >= 0 && idi < nj) ? so.inv[idi] : -1;
+  const int piece = jper / W;
+  const int k0 = min(nj, blockIdx.y * jper + w * piece);
+  const RangeSums r = pg6_range_o<NN>(xi, yi, zi, (float)ivel[3 * ic + 0],
+                                      (float)ivel[3 * ic + 1], (float)ivel[3 * ic + 2], idi,
+                                      self_k, eps2, k0, min(nj, k0 + piece), so);
+  s_v[0][w][lane] = r.ax;
+  s_v[1][w][lane] = r.ay;
+  s_v[2][w][lane] = r.az;
+  s_v[3][w][lane] = r.pot;
+  s_v[4][w][lane] = r.jx;
+  s_v[5][w][lane] = r.jy;
+  s_v[6][w][lane] = r.jz;
+  s_r[w][lane] = r.rmax;
+  s_n[w][lane] = r.nn;
+  __syncthreads();
+  if (w != 0 || !ivalid) return;
+  double v[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) v[k] = s_v[k][0][lane];
+  float rmax = s_r[0][lane];
+  int nn = s_n[0][lane];
+#pragma unroll
+  for (int g = 1; g < W; ++g) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] += s_v[k][g][lane];
+    if (NN && nn_better(s_r[g][lane], s_n[g][lane], rmax, nn)) {
+      rmax = s_r[g][lane];
+      nn = s_n[g][lane];
+    }
+  }
+  if (nsplit == 1) {
+    out.acc[3 * i + 0] = v[0];
+    out.acc[3 * i + 1] = v[1];
+    out.acc[3 * i + 2] = v[2];
+    out.jrk[3 * i + 0] = v[4];
+    out.jrk[3 * i + 1] = v[5];
+    out.jrk[3 * i + 2] = v[6];
+    out.pot[i] = v[3];
+    if (NN) {
+      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+    }
+  } else {
+    const size_t o = (size_t)blockIdx.y * ni + i;
+    part.ax[o] = v[0];
+    part.ay[o] = v[1];
+    part.az[o] = v[2];
+    part.pot[o] = v[3];
+    part.jx[o] = v[4];
+    part.jy[o] = v[5];
+    part.jz[o] = v[6];
+    if (NN) {
+      part.rmax[o] = rmax;
+      part.nn[o] = nn;
+    }
+  }
+}
+
+// Sums the nsplit partials of each i with a block of 256 threads per i:
+// thread t takes splits t, t+256, ... in order (all its loads issued
+// together), a fixed xor-butterfly combines each wavefront's lanes, and the
+// four wavefront sums are added in wave order.  Deterministic.  For many
+// splits and few i, where one thread per i would walk nsplit dependent loads
+// and one wavefront per i still waited for several rounds of them.
 template <bool NN>
 __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials part, JPred jp,
                                                     const double* __restrict__ ipos, float eps2,
                                                     Outputs out) {
+  __shared__ double s_v[7][4];
+  __shared__ float s_r[4];
+  __shared__ int s_n[4];
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= ni) return;  // uniform per wavefront
+  const int w = threadIdx.x >> 6;
+  const int i = blockIdx.x;
   double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   float rmax = 0.f;
   int nn = -1;
-  for (int s = lane; s < nsplit; s += 64) {
+#pragma unroll 4
+  for (int s = threadIdx.x; s < nsplit; s += 256) {
     const size_t o = (size_t)s * ni + i;
     v[0] += part.ax[o];
     v[1] += part.ay[o];
@@ -929,7 +1016,26 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
       }
     }
   }
-  if (lane != 0) return;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s_v[k][w] = v[k];
+    s_r[w] = rmax;
+    s_n[w] = nn;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) v[k] = s_v[k][0];
+  rmax = s_r[0];
+  nn = s_n[0];
+  for (int g = 1; g < 4; ++g) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] += s_v[k][g];
+    if (NN && nn_better(s_r[g], s_n[g], rmax, nn)) {
+      rmax = s_r[g];
+      nn = s_n[g];
+    }
+  }
   out.acc[3 * i + 0] = v[0];
   out.acc[3 * i + 1] = v[1];
   out.acc[3 * i + 2] = v[2];

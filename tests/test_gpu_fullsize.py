@@ -55,7 +55,7 @@ def test_pg6_step_n65536():
     assert np.percentile(rel_err(r["jrk"][sel], o["jrk"]), 99) < 1e-5
     assert rel_err(r["pot"][sel], o["pot"]).max() < 1e-6
     # each large-i kernel at full size, the sorted chunk-origin one included
-    for k in (2, 4):
+    for k in (2, 4, 5):
         assert g6.lib.pg6_dev_set_kernel(k) == 0
         rk = g6.calc(n, x, v, idx, eps * eps, variant="2p")
         g6.lib.pg6_dev_set_kernel(-1)

@@ -70,12 +70,15 @@ def test_matches_reference_fixture(g6):
 
 
 @pytest.mark.parametrize("eps", [1.0 / 256, 0.0])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("ni", [1, 7, 48, 64, 500, 4096])
 def test_force_vs_fp64_oracle(g6, ni, variant, eps):
     """Every kernel against the FP64 oracle, softened and unsoftened (the
-    reference's sample integrator runs with eps = 0)."""
+    reference's sample integrator runs with eps = 0).  Variant 5 runs with
+    four wavefronts per 64-i block (bi = 256)."""
     assert g6.lib.pg6_dev_set_kernel(variant) == 0
+    if variant == 5:
+        assert g6.lib.pg6_dev_set_launch(256, 16, 64) == 0
     n = 4096
     m, x, v, a2, j6, tj, idx = plummer_state(n, 2, eps)
     g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
@@ -96,7 +99,7 @@ def test_force_vs_fp64_oracle(g6, ni, variant, eps):
     # p99 ~1e-6 here (a CPU emulation of the layout gives 8.5e-7 over 512 i).
     # The automatic choice takes it only from 16384 j up; the 65536-particle
     # test holds it to 1e-6.
-    tol = 3e-6 if variant == 4 else 1e-6
+    tol = 3e-6 if variant in (4, 5) else 1e-6
     assert np.percentile(ea, 99) < tol, np.percentile(ea, 99)
     assert np.percentile(ej, 99) < 10 * tol, np.percentile(ej, 99)
     assert rel_err(r["pot"], o["pot"]).max() < tol
@@ -111,6 +114,7 @@ def test_force_vs_fp64_oracle(g6, ni, variant, eps):
     d = xo[r["nnb"]] - xo[isel]
     np.testing.assert_allclose(r["rnnb"], np.sqrt((d * d).sum(1) + eps * eps), rtol=1e-6)
     g6.lib.pg6_dev_set_kernel(-1)
+    assert g6.lib.pg6_dev_set_launch(0, 0, 0) == 0
 
 
 @pytest.mark.parametrize("eps", [1.0 / 256, 0.0])
@@ -120,8 +124,9 @@ def test_self_filter(g6, ids, eps):
     misses the wavefront's: i-particles taken in address order (most chunks
     skip the test when ids follow addresses, none when they are permuted),
     with eps = 0 so that a self pair that slipped through would give NaN.
-    pg6_force_o (variant 4, sorted j) tests only the chunk that holds each
-    i itself when ids are the addresses, and every chunk otherwise."""
+    pg6_force_o (variant 4, sorted j; variant 5 with two wavefronts per
+    64-i block) tests the self pair only on chunks that hold a lane's own
+    address or an id that differs from its address."""
     n = 4096
     m, x, v, a2, j6, tj, idx = plummer_state(n, 5, eps)
     if ids == "permuted":
@@ -132,9 +137,11 @@ def test_self_filter(g6, ids, eps):
     xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
     vj = vo.astype(np.float32).astype(np.float64)
     for sel, var in ((np.arange(n), 2), (np.arange(1000, 3000), 2), (np.arange(77, 141), 2),
-                     (np.arange(n), 4), (np.arange(77, 141), 4)):
+                     (np.arange(n), 4), (np.arange(77, 141), 4), (np.arange(n), 5)):
         assert g6.lib.pg6_dev_set_kernel(var) == 0
+        assert g6.lib.pg6_dev_set_launch(128 if var == 5 else 0, 0, 0) == 0
         r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+        assert g6.lib.pg6_dev_set_launch(0, 0, 0) == 0
         assert g6.lib.pg6_dev_set_kernel(1) == 0
         r1 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
         o = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo, vj, m, idx)
@@ -142,7 +149,7 @@ def test_self_filter(g6, ids, eps):
             assert np.all(np.isfinite(r[k])), k
         # the chunk-origin dx of variant 4 at 4096 particles: see
         # test_force_vs_fp64_oracle
-        tol = 3e-6 if var == 4 else 1e-6
+        tol = 3e-6 if var in (4, 5) else 1e-6
         assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < tol
         assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 10 * tol
         assert rel_err(r["pot"], o["pot"]).max() < tol
@@ -212,10 +219,10 @@ def test_autotune_and_launch_shapes(g6):
     tuned = g6.autotune(n, eps * eps, reps=1)
     assert sorted(tuned) == [64 << k for k in range(7)]
     for ni, (var, bi, bpc, us) in tuned.items():
-        # variant 4 (sorted chunk origins) is a candidate only from 16384 j,
-        # variant 3 (one launch) not at all
-        assert var in (0, 1, 2) and bi in (64, 128, 256) and us > 0
-        assert bpc in (4, 8, 16, 32, 64)
+        # variants 4 and 5 (sorted chunk origins) are candidates only from
+        # 16384 j
+        assert var in (0, 1, 2, 3) and bi in (64, 128, 256) and us > 0
+        assert bpc in ((1, 2, 4, 8, 16) if var == 3 else (4, 8, 16, 32, 64))
         p = g6.plan(ni, n)
         assert (p["variant"], p["bi"], p["bpc"]) == (var, bi, bpc)
         assert p["nsplit"] * p["jper"] >= n
