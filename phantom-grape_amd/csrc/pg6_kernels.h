@@ -830,8 +830,11 @@ __device__ __forceinline__ RangeSums pg6_range_o(double xi, double yi, double zi
 // pg6_force_h on the sorted layout (pg6_range_o): one i-particle per lane,
 // blocks of BI threads, blockIdx.y = range of sorted positions.  Same outputs
 // and partials as pg6_force_h (nearest neighbours by j address).
+// At least eight wavefronts per SIMD (64 VGPRs; left alone the compiler
+// takes 85 and five): more wavefronts to cover the scalar-load latency of
+// the j records.
 template <bool NN>
-__global__ __launch_bounds__(kThreads) void pg6_force_o(
+__global__ __launch_bounds__(kThreads, 8) void pg6_force_o(
     int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
     const int* __restrict__ iid, float eps2, int nj, int jper, JSort so, JPred jp, Partials part,
     Outputs out, int nsplit) {
@@ -889,8 +892,10 @@ __device__ inline bool nn_better(float r, int n, float rbest, int nbest) {
 // piece, and wave 0 adds the W sums in wave order (nearest neighbour by
 // nn_better).  W times fewer partial sums than pg6_force_o for the same
 // number of wavefronts.  Block = 64 W threads.
+// Seven wavefronts per SIMD (72 VGPRs; at eight the four-wave version
+// spills, left alone it takes 86 VGPRs and five): 2 % faster at N = 65536.
 template <bool NN, int W>
-__global__ __launch_bounds__(64 * W) void pg6_force_ow(
+__global__ __launch_bounds__(64 * W, 7) void pg6_force_ow(
     int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
     const int* __restrict__ iid, float eps2, int nj, int jper, JSort so, JPred jp, Partials part,
     Outputs out, int nsplit) {
