@@ -1359,6 +1359,9 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
           b.force_variant = var;
           b.force_bi = bi;
           b.force_bpc = bpc;
+          // the positions do not change here: no rebuild of the sorted
+          // layout inside a timed run
+          b.sort_age = 0;
           b.predict_flag = 1;
           board_force(b, ni, nj, ipos, ivel, iid, eps2, o, st);  // warm-up
           PGAMD_HIP_CHECK(hipEventRecord(e0, st));
