@@ -1479,6 +1479,7 @@ void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
                      h.target, h.min_jper, h.act, h.S, h.ipos, h.ivel, h.part, b.js);
   PGAMD_HIP_CHECK(hipGetLastError());
   b.nj_pred = -1;  // the board's predictions now belong to the integrator
+  b.sort_age = kSortAge;  // its corrector moves every particle: rebuild the order
   b.predict_flag = 1;
 }
 
@@ -1503,6 +1504,7 @@ void hermite_enqueue_batch(Board& b, Hermite& h, hipStream_t st) {
   }
   PGAMD_HIP_CHECK(hipGraphLaunch(h.graph, st));
   b.nj_pred = -1;
+  b.sort_age = kSortAge;
   b.predict_flag = 1;
 }
 
