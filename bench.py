@@ -112,7 +112,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", type=int, default=-1,
                     help="PG6 force kernel: 0 LDS j-tiles, 1 SGPR broadcast, 2 SGPR broadcast "
-                         "with two-float positions (default: library's)")
+                         "with two-float positions, 3 one launch (prediction and reduction "
+                         "inside), 4 Morton-sorted chunk-origin positions, 5 as 4 with the "
+                         "j-range of each 64-i block split over its wavefronts (default: "
+                         "library's)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the library's default launch shapes instead of autotuning them")
     ap.add_argument("--launch", default="",
