@@ -160,6 +160,33 @@ def test_self_filter(g6, ids, eps):
     g6.lib.pg6_dev_set_kernel(-1)
 
 
+@pytest.mark.parametrize("variant,bi", [(4, 64), (4, 256), (5, 128), (5, 256)])
+def test_sorted_kernels_on_ragged_sizes(g6, variant, bi):
+    """The sorted chunk-origin kernels with a j count that is no multiple of
+    the 64-particle chunk (the last chunk is partial) and an i count that is
+    no multiple of the block, against the FP64 oracle, softened."""
+    nj, ni, eps = 3001, 700, 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(nj, 21, eps)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    ti = 1.0 / 512
+    g6.set_ti(ti)
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.random.default_rng(22).choice(nj, ni, replace=False)
+    assert g6.lib.pg6_dev_set_kernel(variant) == 0
+    assert g6.lib.pg6_dev_set_launch(bi, 16, 64) == 0
+    try:
+        r = g6.calc(nj, xo[sel], vo[sel], idx[sel], eps * eps, variant="2p")
+    finally:
+        g6.lib.pg6_dev_set_kernel(-1)
+        g6.lib.pg6_dev_set_launch(0, 0, 0)
+    o = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo,
+                    vo.astype(np.float32).astype(np.float64), m, idx)
+    assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 3e-6
+    assert np.percentile(rel_err(r["jrk"], o["jrk"]), 99) < 3e-5
+    assert rel_err(r["pot"], o["pot"]).max() < 3e-6
+    assert np.mean(r["nnb"] == o["nnb"]) > 0.99
+
+
 def test_sorted_layout_follows_j_writes(g6):
     """Variant 4 keeps its Morton order between rebuilds and recenters the
     chunk origins and per-chunk id flags at every prediction.  A whole new
