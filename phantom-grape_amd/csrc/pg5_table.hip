@@ -1103,6 +1103,11 @@ PGAMD_API int pg5_mlp_train(int iters, int starts, int refine, unsigned seed,
     lm_steps(f, c, iters, b->stream);
     if (c.F < best.F) best = std::move(c);
   }
+  if (best.x.empty()) {  // every start's loss was not finite
+    f.release();
+    api_error(6, "pg5_mlp_train: no start reached a finite loss");
+    return -1;
+  }
   lm_steps(f, best, refine, b->stream);
   f.release();
   g_mlp.params = best.x;

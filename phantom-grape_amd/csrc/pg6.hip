@@ -519,10 +519,11 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   board_use_stream(b, st);
   const LaunchPlan p = plan_launch(b, ni, nj > 0 ? nj : 1);
   const bool pred = board_needs_pred(b, nj);
-  if (p.variant == 4 || p.variant == 5) {
+  if ((p.variant == 4 || p.variant == 5) && nj > 0) {
     // the sorted layout of the current prediction
     const bool fresh = b.sort_nj == nj && b.sort_age < kSortAge && b.sort_writes < nj;
-    if (pred || !b.srec_ok) {
+    // (a layout of another j count orders other particles: rebuild it)
+    if (pred || !b.srec_ok || b.sort_nj != nj) {
       if (fresh) {
         board_predict(b, nj, st, true);
       } else {

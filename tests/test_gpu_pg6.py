@@ -187,6 +187,31 @@ def test_sorted_kernels_on_ragged_sizes(g6, variant, bi):
     assert np.mean(r["nnb"] == o["nnb"]) > 0.99
 
 
+@pytest.mark.parametrize("variant", [4, 5])
+def test_sorted_kernels_follow_the_j_count(g6, variant):
+    """A call on fewer j than the last one, with no new prediction, uses the
+    first nj j-particles (a sorted layout of the larger set orders other
+    particles and is rebuilt), and a call on no j gives zeros."""
+    n, eps, ti = 4096, 1.0 / 256, 1.0 / 512
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 31, eps)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    g6.set_ti(ti)
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.arange(0, 3000, 7)
+    assert g6.lib.pg6_dev_set_kernel(variant) == 0
+    try:
+        g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps)
+        r = g6.calc(3000, xo[sel], vo[sel], idx[sel], eps * eps)
+        z = g6.calc(0, xo[sel], vo[sel], idx[sel], eps * eps)
+    finally:
+        g6.lib.pg6_dev_set_kernel(-1)
+    o = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo[:3000],
+                    vo[:3000].astype(np.float32).astype(np.float64), m[:3000], idx[:3000])
+    assert np.percentile(rel_err(r["acc"], o["acc"]), 99) < 3e-6
+    assert rel_err(r["pot"], o["pot"]).max() < 3e-6
+    assert np.all(z["acc"] == 0.0) and np.all(z["pot"] == 0.0)
+
+
 def test_sorted_layout_follows_j_writes(g6):
     """Variant 4 keeps its Morton order between rebuilds and recenters the
     chunk origins and per-chunk id flags at every prediction.  A whole new
