@@ -468,10 +468,13 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
   return p;
 }
 
+// pred_in: the LDS kernel predicts the j-set itself (variant 0 with one
+// i-block only, board_force decides), at time ti from js.
 template <int BI, bool NN>
 void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const double* ivel,
                     const int* iid, float eps2, int nj, const JPred& jp, const pg6::JSort& so,
-                    const Partials& part, const Outputs& out, hipStream_t st) {
+                    const Partials& part, const Outputs& out, hipStream_t st, bool pred_in,
+                    double ti, const JState& js) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
   g_timer.before(st);
   if (p.variant == 5)  // 64 i per block, jper a multiple of BI sorted positions
@@ -487,9 +490,12 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
   else if (p.variant == 1)
     hipLaunchKernelGGL((pg6::pg6_force_s<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
                        nj, p.jper, jp, part, out, p.nsplit);
+  else if (pred_in)
+    hipLaunchKernelGGL((pg6::pg6_force<BI, NN, true>), grid, dim3(pg6::kThreads), 0, st, ni,
+                       ipos, ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);
   else
-    hipLaunchKernelGGL((pg6::pg6_force<BI, NN>), grid, dim3(pg6::kThreads), 0, st, ni, ipos,
-                       ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit);
+    hipLaunchKernelGGL((pg6::pg6_force<BI, NN, false>), grid, dim3(pg6::kThreads), 0, st, ni,
+                       ipos, ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);
   PGAMD_HIP_CHECK(hipGetLastError());
   g_timer.after(st);
   if (p.nsplit > 1) {
@@ -531,9 +537,12 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
         board_build_sort(b, nj, st);
       }
     }
-  } else if (pred && p.variant != 3) {
-    board_predict(b, nj, st);
   }
+  // one i-block of the LDS kernel: every j sits in one block's range, so the
+  // blocks predict their own j (no predictor launch)
+  const bool pred_in = pred && p.variant == 0 && ceil_div(ni, p.bi) == 1;
+  if (pred && !pred_in && p.variant != 3 && p.variant != 4 && p.variant != 5)
+    board_predict(b, nj, st);
   Partials part{};
   if (p.nsplit > 1) {
     const size_t n = (size_t)p.nsplit * ni;
@@ -585,9 +594,11 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
 #define PG6_LAUNCH(BI)                                                                     \
   do {                                                                                     \
     if (nn)                                                                                \
-      launch_force_t<BI, true>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st); \
+      launch_force_t<BI, true>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st,  \
+                               pred_in, b.ti, b.js);                                       \
     else                                                                                   \
-      launch_force_t<BI, false>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st);\
+      launch_force_t<BI, false>(p, ni, ipos, ivel, iid, e2, njr, b.jp, b.so, part, out, st, \
+                                pred_in, b.ti, b.js);                                      \
   } while (0)
   switch (p.bi) {
     case 256: PG6_LAUNCH(256); break;
@@ -595,6 +606,12 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
     default: PG6_LAUNCH(64); break;
   }
 #undef PG6_LAUNCH
+  if (pred_in) {
+    b.nj_pred = nj;
+    b.predict_flag = 0;
+    b.pred_armed = false;
+    b.srec_ok = false;  // the sorted records hold an older prediction
+  }
   board_done_on(b, st);
 }
 

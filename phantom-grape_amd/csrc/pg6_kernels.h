@@ -319,11 +319,17 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 // (group g takes j = g, g+G, ...), and are combined in fixed order at the end.
 // blockIdx.y selects the j-range [y*jper, min(nj,(y+1)*jper)).
 // With nsplit == 1 the kernel writes final outputs, otherwise partials.
-template <int BI, bool NN>
+// PRED: the block predicts the j of each tile itself (pg6_predict_j at time
+// ti, writing the predicted set as pg6_predict does) instead of reading the
+// predicted set; for launches with one i-block, where every j is in exactly
+// one block's range, so that the separate predictor launch goes away.
+static_assert(kThreads == kJTile, "one thread stages one j of a tile");
+
+template <int BI, bool NN, bool PRED>
 __global__ __launch_bounds__(kThreads) void pg6_force(
     int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
     const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp,
-    Partials part, Outputs out, int nsplit) {
+    Partials part, Outputs out, int nsplit, double ti, JState js) {
   constexpr int G = kThreads / BI;
   __shared__ double s_x[kJTile], s_y[kJTile], s_z[kJTile];
   __shared__ float4 s_vm[kJTile];
@@ -362,6 +368,9 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
   for (int jt = j0; jt < j1; jt += kJTile) {
     const int nt = min(kJTile, j1 - jt);
     __syncthreads();
+    // every thread of the block takes part (the predictor's chunk id ranges
+    // combine across each wavefront)
+    if (PRED) pg6_predict_j(jt + tid, tid < nt, ti, js, jp);
     if (tid < nt) {
       const int j = jt + tid;
       s_x[tid] = jp.x[j];
