@@ -508,7 +508,7 @@ def bench_c3(args, stream, cpu):
     T = pgamd.G5Table()
     T.lib.pg5_gen_s2_force_table(pp, pm)
     t0 = time.time()
-    loss = T.mlp_train(iters=200, starts=4, refine=1000, seed=7)
+    loss = T.mlp_train(iters=200, starts=4, refine=3000, seed=7)
     ttrain = time.time() - t0
     P = L["pos"]
     mm = m[L["src"]]
@@ -521,7 +521,7 @@ def bench_c3(args, stream, cpu):
            "interactions": L["work_pp"], "list_build_host_s": round(tb, 3),
            "list_build_gpu_s": round(float(np.median(tgpu[1:])), 4),
            "mlp_train": {"s": round(ttrain, 2), "loss": loss,
-                         "method": "Levenberg-Marquardt, FP64 GPU kernels, 4 starts x 200 trial steps, best + 1000"}}
+                         "method": "Levenberg-Marquardt, FP64 GPU kernels, 4 starts x 200 trial steps, best + 3000"}}
     O = _oracle()
     sel = np.sort(np.random.default_rng(4).choice(len(x), 512, replace=False))
     f = O.table_force_f64(P[sel], P, mm, pp, pm)

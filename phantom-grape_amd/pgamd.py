@@ -457,7 +457,7 @@ class G5Table:
     def mlp_set_params(self, p, form, rc, r0):
         assert self.lib.pg5_mlp_set_params(dptr(f64(p)), int(form), float(rc), float(r0)) == 0
 
-    def mlp_train(self, iters=200, starts=4, refine=1000, seed=1):
+    def mlp_train(self, iters=200, starts=4, refine=3000, seed=1):
         """pg5_mlp_train: Levenberg-Marquardt from `starts` seeded starts of
         `iters` trial steps, the best refined by `refine` more; returns the
         final weighted MSE."""

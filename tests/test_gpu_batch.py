@@ -199,7 +199,7 @@ def learned_f(tab, r):
 @pytest.fixture(scope="module")
 def trained(tab):
     """The S2 network trained on the GPU (Levenberg-Marquardt, FP64)."""
-    loss = tab.mlp_train(iters=150, starts=3, refine=1000, seed=7)
+    loss = tab.mlp_train(iters=150, starts=3, refine=2500, seed=7)
     return loss, tab.mlp_params()
 
 
@@ -214,7 +214,7 @@ def test_mlp_training_and_learned_force(tab, treepm_case, trained):
     # Newtonian force is u^3 times the network's error outside the core)
     e = np.max(np.abs(M.forward(p, u) - H) * (u ** 3 + 0.2))
     print(f"trained: loss {loss:.3e} max |y - H| (u^3 + 0.2) {e:.3e}")
-    assert e < 2e-4 and loss < 1e-8
+    assert e < 3e-6 and loss < 1e-12
     # the GPU evaluator decodes it into the cut-off shape f(r): compared with
     # the closed form relative to the pair's Newtonian 1/r^3
     r = np.geomspace(PP / 20, 0.999 * PM, 3000)
@@ -233,7 +233,7 @@ def test_mlp_training_and_learned_force(tab, treepm_case, trained):
     fa = O.table_force_f64(L["pos"][sel], L["pos"], c["m"], PP, PM)
     e = p99(a[sel], fa)
     print(f"learned force vs closed form: p99 {e:.3e}")
-    assert e < 1e-4, e
+    assert e < 2e-5, e
 
 
 def test_trained_network_gains_over_the_table_and_its_start(tab, treepm_case, trained):
