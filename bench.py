@@ -546,6 +546,8 @@ def bench_c3(args, stream, cpu):
                  d["order"], d_pos.data_ptr(), d_pos64 if p64 else None, acc.data_ptr(), mode)
         best = np.zeros(1, np.int32)
         T.lib.pg5t_batch_autotune(*args_, 2, pgamd.iptr(best), None)
+        shape = np.zeros(2, np.int32)
+        T.lib.pg5t_batch_launch_shape(mode, pgamd.iptr(shape))
         h = T.lib.pg5t_batch_graph(*args_)
         ms = _time(lambda: T.lib.pg5t_batch_graph_launch(h, sp), stream, 5)
         T.lib.pg5t_batch_graph_destroy(h)
@@ -553,7 +555,8 @@ def bench_c3(args, stream, cpu):
         res[name] = {"ms": round(ms, 3), "ginteractions_s": round(L["work_pp"] / ms / 1e6, 1),
                      "pct_fp32_peak": round(100 * 38 * L["work_pp"] / (ms * 1e-3) / 1e12
                                             / FP32_PEAK_TFLOPS, 2),
-                     "blocks_per_cu": int(best[0]), "table": f"frc_bit {frc}, order {order}",
+                     "block": int(shape[0]), "blocks_per_cu": int(shape[1]),
+                     "table": f"frc_bit {frc}, order {order}",
                      "positions": "FP64 group origins" if p64 else "FP32 scaled",
                      "accuracy_vs_closed_form_s2": _errs(a, f),
                      "accuracy_vs_reference_table": _errs(a, fref)}
@@ -643,6 +646,8 @@ def bench_c4(args, stream, cpu):
              eps * eps, acc.data_ptr(), pot.data_ptr())
     best = np.zeros(1, np.int32)
     g5.lib.pg5_bh_autotune(*args_, 2, pgamd.iptr(best), None)
+    shape = np.zeros(2, np.int32)
+    g5.lib.pg5_bh_launch_shape(pgamd.iptr(shape))
     h = g5.lib.pg5_bh_graph(*args_)
     ms = _time(lambda: g5.lib.pg5_bh_graph_launch(h, stream.cuda_stream), stream, 5)
     g5.lib.pg5_bh_graph_destroy(h)
@@ -653,7 +658,8 @@ def bench_c4(args, stream, cpu):
            "list_build_gpu_s": round(float(np.median(tgpu[1:])), 4), "ms": round(ms, 3),
            "ginteractions_s": round((L["work_pp"] + L["work_pc"]) / ms / 1e6, 1),
            "pct_fp32_peak": round(100 * flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
-           "flops": "38/particle + 71/cell interaction", "blocks_per_cu": int(best[0])}
+           "flops": "38/particle + 71/cell interaction", "block": int(shape[0]),
+           "blocks_per_cu": int(shape[1])}
     O = _oracle()
     sel = np.sort(np.random.default_rng(5).choice(n, 256, replace=False))
     a = acc.cpu().numpy()[sel]
@@ -705,6 +711,8 @@ def bench_c5(args, stream, cpu):
              jrk.data_ptr(), pot.data_ptr())
     best = np.zeros(1, np.int32)
     g6.lib.pg6_cluster_autotune(*args_, 2, pgamd.iptr(best), None)
+    shape = np.zeros(2, np.int32)
+    g6.lib.pg6_cluster_launch_shape(pgamd.iptr(shape))
     h = g6.lib.pg6_cluster_graph(*args_)
     ms = _time(lambda: g6.lib.pg6_cluster_graph_launch(h, stream.cuda_stream), stream, 5)
     g6.lib.pg6_cluster_graph_destroy(h)
@@ -713,7 +721,7 @@ def bench_c5(args, stream, cpu):
            "interactions": inter, "ms": round(ms, 3),
            "ginteractions_s": round(inter / ms / 1e6, 1),
            "pct_fp32_peak": round(100 * 60 * inter / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 2),
-           "blocks_per_cu": int(best[0])}
+           "j_tile": int(shape[0]), "blocks_per_cu": int(shape[1])}
     O = _oracle()
     a, j = acc.cpu().numpy(), jrk.cpu().numpy()
     ea, ej, oa, oj = [], [], [], []

@@ -78,12 +78,20 @@ void *pg5_bh_graph(int devid, int ngroups, const int *ioff, const int *icnt, con
                    const void *cells, double eps2, double *acc, double *pot);
 int pg5_bh_graph_launch(void *graph, void *stream);
 int pg5_bh_graph_destroy(void *graph);
-/* Autotuner over persistent blocks per CU {1,2,3,4,6,8,12,16}. */
+/* Autotuner over blocks of {256, 512, 1024} threads (an LDS tile holds that
+ * many particles or half as many cells) x persistent blocks per CU
+ * {1,2,3,4,6,8,12,16}; keeps the fastest pair.  *best = its blocks per CU,
+ * ms[8] (nullable) = the timings at the block kept. */
 int pg5_bh_autotune(int devid, int ngroups, const int *ioff, const int *icnt, const int *ilist,
                     const int *poff, const int *pcnt, const int *plist, const int *coff,
                     const int *ccnt, const int *clist, const int *order, const void *pos,
                     const void *cells, double eps2, double *acc, double *pot, int reps,
                     int *best, float *ms);
+/* Launch shape of the tree-list batch: block in {0 = default 1024, 256, 512,
+ * 1024}, blocks_per_cu (0 = default 2); a negative value keeps that part.
+ * pg5_bh_launch_shape: out2 = {block, blocks_per_cu} in use. */
+int pg5_bh_set_launch(int block, int blocks_per_cu);
+int pg5_bh_launch_shape(int *out2);
 
 /* ------------------------------------------------------------------ libpg5_table.so
  * pgamd-library: pg5_table */
@@ -151,11 +159,20 @@ void *pg5t_batch_graph(int devid, int ngroups, const int *ioff, const int *icnt,
                        int mode);
 int pg5t_batch_graph_launch(void *graph, void *stream);
 int pg5t_batch_graph_destroy(void *graph);
+/* Autotuner over the table kernel's block of {256, 512, 1024} threads (each
+ * also its LDS j-tile; the network kernel runs 256) x persistent blocks per
+ * CU {1,2,3,4,6,8,12,16}; keeps the fastest pair for this mode.  *best = its
+ * blocks per CU, ms[8] (nullable) = the timings at the block kept. */
 int pg5t_batch_autotune(int devid, int ngroups, const int *ioff, const int *icnt,
                         const int *ilist, const int *joff, const int *jcnt, const int *jlist,
                         const int *order, const void *pos, const double *pos64, double *acc,
                         int mode, int reps, int *best, float *ms);
 int pg5t_batch_set_blocks_per_cu(int mode, int bpc);
+/* Table-mode block in {0 = default 1024, 256, 512, 1024}. */
+int pg5t_batch_set_block(int block);
+/* out2 = {threads per block, blocks per CU} the batched calls of this mode
+ * use now. */
+int pg5t_batch_launch_shape(int mode, int *out2);
 /* Long-range (particle-mesh) side of the TreePM force, which the reference
  * does not ship (PG5/table/libpg5/pppm.h:4-6 names its mesh): the periodic
  * force of the S2 shape R(r, eps_pm) that the table subtracts, by TSC
@@ -255,9 +272,18 @@ void *pg6_cluster_graph(int nitems, const int *items, const int *coff, const dou
                         double *jrk, double *pot);
 int pg6_cluster_graph_launch(void *graph, void *stream);
 int pg6_cluster_graph_destroy(void *graph);
+/* Autotuner over j-tiles of {128, 256, 512} particles x persistent blocks
+ * per CU {1,2,3,4,6,8,12,16}; keeps the fastest pair.  *best = its blocks
+ * per CU, ms[8] (nullable) = the timings at the tile kept. */
 int pg6_cluster_autotune(int nitems, const int *items, const int *coff, const double *pos,
                          const double *vel, const double *mass, const float *eps2, double *acc,
                          double *jrk, double *pot, int reps, int *best, float *ms);
+/* Launch shape of the cluster batch: tile in {0 = default 256, 128, 256,
+ * 512} j-particles per LDS tile, blocks_per_cu (0 = default 4); a negative
+ * value keeps that part.  pg6_cluster_launch_shape: out2 = {tile,
+ * blocks_per_cu} in use. */
+int pg6_cluster_set_launch(int tile, int blocks_per_cu);
+int pg6_cluster_launch_shape(int *out2);
 
 #if defined(__cplusplus)
 }

@@ -20,9 +20,7 @@
 
 namespace pgbh {
 
-constexpr int kThreads = 1024;  // block
-constexpr int kPTile = kThreads;     // particles staged per LDS tile
-constexpr int kCTile = kThreads / 2; // cells staged per LDS tile
+constexpr int kThreads = 1024;  // default block (256 and 512 are also built)
 
 __device__ inline int take_item(int* counter, int* s_item) {
   __syncthreads();
@@ -31,18 +29,22 @@ __device__ inline int take_item(int* counter, int* s_item) {
   return *s_item;
 }
 
-// Lanes are packed by phases: a group of n <= kThreads i-particles takes
-// p = kThreads / n copies of the i-set; lane t holds i-particle t % n and
+// Lanes are packed by phases: a group of n <= BLOCK i-particles takes
+// p = BLOCK / n copies of the i-set; lane t holds i-particle t % n and
 // walks the particles and cells of each LDS tile with jj = t / n, t / n + p,
 // ...; the p phase sums of each i are added in phase order at the end.
-// Groups larger than kThreads run in passes.
-__global__ __launch_bounds__(kThreads) void bh_batch(
+// Groups larger than BLOCK run in passes.  An LDS tile holds BLOCK particles
+// or BLOCK / 2 cells.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void bh_batch(
     int ngroups, const int* __restrict__ order, const int* __restrict__ ioff,
     const int* __restrict__ icnt, const int* __restrict__ ilist, const int* __restrict__ poff,
     const int* __restrict__ pcnt, const int* __restrict__ plist, const int* __restrict__ coff,
     const int* __restrict__ ccnt, const int* __restrict__ clist, const float4* __restrict__ pos,
     const pg5::Cell* __restrict__ cells, float eps2, double* __restrict__ acc,
     double* __restrict__ pot, int* counter) {
+  constexpr int kPTile = BLOCK;      // particles staged per LDS tile
+  constexpr int kCTile = BLOCK / 2;  // cells staged per LDS tile
   // one LDS buffer holds either a particle tile or a cell tile
   constexpr size_t kStage = sizeof(float4) * kPTile > sizeof(pg5::Cell) * kCTile
                                 ? sizeof(float4) * kPTile
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(kThreads) void bh_batch(
   __shared__ __align__(16) char s_stage[kStage];
   float4* s_p = reinterpret_cast<float4*>(s_stage);
   pg5::Cell* s_c = reinterpret_cast<pg5::Cell*>(s_stage);
-  __shared__ double s_red[4][kThreads];
+  __shared__ double s_red[4][BLOCK];
   __shared__ int s_item;
   const int tid = threadIdx.x;
   for (;;) {
@@ -60,9 +62,9 @@ __global__ __launch_bounds__(kThreads) void bh_batch(
     const int i0 = ioff[grp], ni = icnt[grp];
     const int p0 = poff[grp], np = pcnt[grp];
     const int c0 = coff[grp], nc = ccnt[grp];
-    for (int ib = 0; ib < ni; ib += kThreads) {
-      const int nib = min(kThreads, ni - ib);
-      const int nph = kThreads / nib;  // phases
+    for (int ib = 0; ib < ni; ib += BLOCK) {
+      const int nib = min(BLOCK, ni - ib);
+      const int nph = BLOCK / nib;  // phases
       const int il = tid % nib;
       const int phase = tid / nib;
       const bool live = phase < nph;

@@ -8,17 +8,19 @@
 // (FP64), mass[n] (FP64), cluster c = particles [coff[c], coff[c+1]),
 // eps2[c] per cluster, and work items (cluster, first i within it): a block
 // takes an item from the counter and computes up to 256 i of that cluster
-// against all of its members.  Arithmetic as pg6_force: FP64 dx rounded to
-// FP32, FP32 pair math with v_rsq_f32, FP32 sums per 256-particle LDS tile
-// folded into FP64; self pair excluded by index.  Output (FP64):
+// against all of its members, staged TILE j at a time in LDS (TILE = 128,
+// 256 or 512, chosen by the autotuner).  Arithmetic as pg6_force: FP64 dx
+// rounded to FP32, FP32 pair math with v_rsq_f32, FP32 sums over at most 256
+// j folded into FP64 (tiles of 256 and 512 give identical results); self
+// pair excluded by index.  Output (FP64):
 // acc[i][3], jrk[i][3], pot[i] = - sum m/r.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace pgc {
 
-constexpr int kThreads = 256;
-constexpr int kTile = 256;
+constexpr int kThreads = 256;  // block = i-particles per work item
+constexpr int kTile = 256;     // default j-tile (128 and 512 are also built)
 
 __device__ inline int take_item(int* counter, int* s_item) {
   __syncthreads();
@@ -27,13 +29,15 @@ __device__ inline int take_item(int* counter, int* s_item) {
   return *s_item;
 }
 
+template <int TILE>
 __global__ __launch_bounds__(kThreads) void cluster_force(
     int nitems, const int2* __restrict__ items, const int* __restrict__ coff,
     const double* __restrict__ pos, const double* __restrict__ vel,
     const double* __restrict__ mass, const float* __restrict__ eps2,
     double* __restrict__ acc, double* __restrict__ jrk, double* __restrict__ pot, int* counter) {
-  __shared__ double s_x[kTile], s_y[kTile], s_z[kTile];
-  __shared__ float4 s_vm[kTile];
+  constexpr int kFold = TILE < 256 ? TILE : 256;
+  __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+  __shared__ float4 s_vm[TILE];
   __shared__ double s_red[7][kThreads];
   __shared__ int s_item;
   const int tid = threadIdx.x;
@@ -59,51 +63,56 @@ __global__ __launch_bounds__(kThreads) void cluster_force(
     const float vyi = (float)vel[3 * (size_t)i + 1];
     const float vzi = (float)vel[3 * (size_t)i + 2];
     double ax = 0.0, ay = 0.0, az = 0.0, pp = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
-    for (int jt = 0; jt < n; jt += kTile) {
-      const int nt = min(kTile, n - jt);
+    for (int jt = 0; jt < n; jt += TILE) {
+      const int nt = min(TILE, n - jt);
       __syncthreads();
-      if (tid < nt) {
-        const size_t j = (size_t)start + jt + tid;
-        s_x[tid] = pos[3 * j];
-        s_y[tid] = pos[3 * j + 1];
-        s_z[tid] = pos[3 * j + 2];
-        s_vm[tid] = make_float4((float)vel[3 * j], (float)vel[3 * j + 1], (float)vel[3 * j + 2],
-                                (float)mass[j]);
+      for (int k = tid; k < nt; k += kThreads) {
+        const size_t j = (size_t)start + jt + k;
+        s_x[k] = pos[3 * j];
+        s_y[k] = pos[3 * j + 1];
+        s_z[k] = pos[3 * j + 2];
+        s_vm[k] = make_float4((float)vel[3 * j], (float)vel[3 * j + 1], (float)vel[3 * j + 2],
+                              (float)mass[j]);
       }
       __syncthreads();
-      float fax = 0.f, fay = 0.f, faz = 0.f, fp = 0.f, fjx = 0.f, fjy = 0.f, fjz = 0.f;
       const int self = i - start - jt;  // tile slot of the i-particle itself
+      // FP32 sums over at most kFold j, then folded into FP64: a tile of 512
+      // sums exactly as two tiles of 256
+      for (int sub = 0; sub < nt; sub += kFold) {
+        const int ne = min(sub + kFold, nt);
+        float fax = 0.f, fay = 0.f, faz = 0.f, fp = 0.f, fjx = 0.f, fjy = 0.f, fjz = 0.f;
 #pragma unroll 4
-      for (int jj = g; jj < nt; jj += G) {
-        const float dx = (float)(s_x[jj] - xi);
-        const float dy = (float)(s_y[jj] - yi);
-        const float dz = (float)(s_z[jj] - zi);
-        const float4 vm = s_vm[jj];
-        const float dvx = vm.x - vxi;
-        const float dvy = vm.y - vyi;
-        const float dvz = vm.z - vzi;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
-        const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
-        const float rinv = jj == self ? 0.f : __builtin_amdgcn_rsqf(r2);
-        const float rinv2 = rinv * rinv;
-        const float mr = vm.w * rinv;
-        const float mr3 = mr * rinv2;
-        const float beta = -3.f * rv * rinv2;
-        fp -= mr;
-        fax = fmaf(mr3, dx, fax);
-        fay = fmaf(mr3, dy, fay);
-        faz = fmaf(mr3, dz, faz);
-        fjx = fmaf(mr3, fmaf(beta, dx, dvx), fjx);
-        fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
-        fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
+        for (int jj = sub + g; jj < ne; jj += G) {
+          const float dx = (float)(s_x[jj] - xi);
+          const float dy = (float)(s_y[jj] - yi);
+          const float dz = (float)(s_z[jj] - zi);
+          const float4 vm = s_vm[jj];
+          const float dvx = vm.x - vxi;
+          const float dvy = vm.y - vyi;
+          const float dvz = vm.z - vzi;
+          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+          const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+          const float rinv = jj == self ? 0.f : __builtin_amdgcn_rsqf(r2);
+          const float rinv2 = rinv * rinv;
+          const float mr = vm.w * rinv;
+          const float mr3 = mr * rinv2;
+          const float beta = -3.f * rv * rinv2;
+          fp -= mr;
+          fax = fmaf(mr3, dx, fax);
+          fay = fmaf(mr3, dy, fay);
+          faz = fmaf(mr3, dz, faz);
+          fjx = fmaf(mr3, fmaf(beta, dx, dvx), fjx);
+          fjy = fmaf(mr3, fmaf(beta, dy, dvy), fjy);
+          fjz = fmaf(mr3, fmaf(beta, dz, dvz), fjz);
+        }
+        ax += (double)fax;
+        ay += (double)fay;
+        az += (double)faz;
+        pp += (double)fp;
+        jx += (double)fjx;
+        jy += (double)fjy;
+        jz += (double)fjz;
       }
-      ax += (double)fax;
-      ay += (double)fay;
-      az += (double)faz;
-      pp += (double)fp;
-      jx += (double)fjx;
-      jy += (double)fjy;
-      jz += (double)fjz;
     }
     if (G > 1) {
       __syncthreads();

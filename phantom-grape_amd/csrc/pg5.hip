@@ -859,7 +859,9 @@ PGAMD_API int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper) {
 // --------------------------------------------------- batched tree-list force
 
 namespace {
-int g_bh_bpc = 0;  // persistent blocks per CU (0 = default 4)
+int g_bh_bpc = 0;    // persistent blocks per CU (0 = default 2)
+int g_bh_block = 0;  // threads per block (0 = default 1024)
+constexpr int kBhBlocks[3] = {256, 512, 1024};
 
 struct BhArgs {
   int ngroups;
@@ -870,18 +872,32 @@ struct BhArgs {
   double *acc, *pot;
 };
 
-void launch_bh(const BhArgs& a, int* counter, int bpc, hipStream_t st) {
+template <int BLOCK>
+void launch_bh_t(const BhArgs& a, int* counter, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(pgbh::bh_batch<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st, a.ngroups, a.order,
+                     a.ioff, a.icnt, a.ilist, a.poff, a.pcnt, a.plist, a.coff, a.ccnt, a.clist,
+                     a.pos, a.cells, a.eps2, a.acc, a.pot, counter);
+}
+
+// Counter reset + persistent kernel on st; block in kBhBlocks.
+void launch_bh(const BhArgs& a, int* counter, int bpc, int block, hipStream_t st) {
   hipLaunchKernelGGL(pgbh::zero_counter, dim3(1), dim3(64), 0, st, counter);
   PGAMD_HIP_CHECK(hipGetLastError());
   if (a.ngroups <= 0) return;
   const int blocks = std::max(1, std::min(a.ngroups, device_cu_count() * bpc));
-  hipLaunchKernelGGL(pgbh::bh_batch, dim3(blocks), dim3(pgbh::kThreads), 0, st, a.ngroups,
-                     a.order, a.ioff, a.icnt, a.ilist, a.poff, a.pcnt, a.plist, a.coff, a.ccnt,
-                     a.clist, a.pos, a.cells, a.eps2, a.acc, a.pot, counter);
+  switch (block) {
+    case 256: launch_bh_t<256>(a, counter, blocks, st); break;
+    case 512: launch_bh_t<512>(a, counter, blocks, st); break;
+    case 1024: launch_bh_t<1024>(a, counter, blocks, st); break;
+    default:
+      api_error(3, "tree-list batch: block %d not in {256, 512, 1024}", block);
+      return;
+  }
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
 int bh_bpc() { return g_bh_bpc > 0 ? g_bh_bpc : 2; }
+int bh_block() { return g_bh_block > 0 ? g_bh_block : pgbh::kThreads; }
 
 struct BhGraph {
   hipGraph_t graph = nullptr;
@@ -920,7 +936,7 @@ PGAMD_API int pg5_bh_batch(int devid, int ngroups, const int* ioff, const int* i
   b->d_counter.reserve(256);
   launch_bh(bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist, order, pos,
                     cells, eps2, acc, pot),
-            b->d_counter.as<int>(), bh_bpc(), stream ? (hipStream_t)stream : b->stream);
+            b->d_counter.as<int>(), bh_bpc(), bh_block(), stream ? (hipStream_t)stream : b->stream);
   return 0;
 }
 
@@ -940,7 +956,7 @@ PGAMD_API void* pg5_bh_graph(int devid, int ngroups, const int* ioff, const int*
   PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   launch_bh(bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist, order, pos,
                     cells, eps2, acc, pot),
-            g->counter.as<int>(), bh_bpc(), cs);
+            g->counter.as<int>(), bh_bpc(), bh_block(), cs);
   PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
   PGAMD_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   PGAMD_HIP_CHECK(hipStreamDestroy(cs));
@@ -969,9 +985,13 @@ PGAMD_API int pg5_bh_graph_destroy(void* h) {
   return 0;
 }
 
-// Autotuner: times the launch for {1, 2, 3, 4, 6, 8, 12, 16} persistent
-// blocks per CU (one warm-up, mean of reps), keeps the fastest; ms[8]
-// (nullable) receives the timings.
+// Autotuner: times the launch on these inputs for blocks of 256, 512 and
+// 1024 threads (an LDS tile holds that many particles, or half as many
+// cells) and {1, 2, 3, 4, 6, 8, 12, 16} persistent blocks per CU (one
+// warm-up, mean of reps), keeps the fastest pair and returns its blocks per
+// CU through *best; ms[8] (nullable) receives the timings at the block size
+// kept, which pg5_bh_launch_shape reports.  Results depend on the block size
+// only through the order of the FP32 partial sums.
 PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int* icnt,
                               const int* ilist, const int* poff, const int* pcnt,
                               const int* plist, const int* coff, const int* ccnt,
@@ -980,6 +1000,10 @@ PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int
                               int* best, float* ms) {
   Board* b = board(devid, "pg5_bh_autotune");
   if (!b) return -1;
+  if (ngroups < 0) {
+    api_error(3, "pg5_bh_autotune: ngroups=%d", ngroups);
+    return -1;
+  }
   const BhArgs a = bh_args(ngroups, ioff, icnt, ilist, poff, pcnt, plist, coff, ccnt, clist,
                            order, pos, cells, eps2, acc, pot);
   b->d_counter.reserve(256);
@@ -987,28 +1011,64 @@ PGAMD_API int pg5_bh_autotune(int devid, int ngroups, const int* ioff, const int
   hipEvent_t e0, e1;
   PGAMD_HIP_CHECK(hipEventCreate(&e0));
   PGAMD_HIP_CHECK(hipEventCreate(&e1));
-  float bestt = 1e30f;
-  int bestc = bh_bpc();
+  float bestt = 1e30f, t_all[3][8];
+  int bestc = bh_bpc(), bestkb = 2;
+  while (bestkb > 0 && kBhBlocks[bestkb] != bh_block()) --bestkb;
   reps = std::max(1, reps);
-  for (int k = 0; k < 8; ++k) {
-    launch_bh(a, b->d_counter.as<int>(), cand[k], b->stream);
-    PGAMD_HIP_CHECK(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < reps; ++r) launch_bh(a, b->d_counter.as<int>(), cand[k], b->stream);
-    PGAMD_HIP_CHECK(hipEventRecord(e1, b->stream));
-    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
-    float t = 0.f;
-    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-    t /= reps;
-    if (ms) ms[k] = t;
-    if (t < bestt) {
-      bestt = t;
-      bestc = cand[k];
+  for (int kb = 0; kb < 3; ++kb)
+    for (int k = 0; k < 8; ++k) {
+      launch_bh(a, b->d_counter.as<int>(), cand[k], kBhBlocks[kb], b->stream);
+      PGAMD_HIP_CHECK(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < reps; ++r)
+        launch_bh(a, b->d_counter.as<int>(), cand[k], kBhBlocks[kb], b->stream);
+      PGAMD_HIP_CHECK(hipEventRecord(e1, b->stream));
+      PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+      float t = 0.f;
+      PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+      t /= reps;
+      t_all[kb][k] = t;
+      if (t < bestt) {
+        bestt = t;
+        bestc = cand[k];
+        bestkb = kb;
+      }
     }
-  }
   PGAMD_HIP_CHECK(hipEventDestroy(e0));
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
   g_bh_bpc = bestc;
+  g_bh_block = kBhBlocks[bestkb];
+  if (ms)
+    for (int k = 0; k < 8; ++k) ms[k] = t_all[bestkb][k];
   if (best) *best = bestc;
+  return 0;
+}
+
+// Launch shape of the tree-list batch: block = threads per block (256, 512
+// or 1024; 0 = default 1024), blocks_per_cu (0 = default 2); a negative value
+// leaves that part as it is.
+PGAMD_API int pg5_bh_set_launch(int block, int blocks_per_cu) {
+  if (block > 0 && block != 256 && block != 512 && block != 1024) {
+    api_error(3, "pg5_bh_set_launch: block %d not in {0, 256, 512, 1024}", block);
+    return -1;
+  }
+  if (blocks_per_cu > 64) {
+    api_error(3, "pg5_bh_set_launch: blocks_per_cu %d > 64", blocks_per_cu);
+    return -1;
+  }
+  if (block >= 0) g_bh_block = block;
+  if (blocks_per_cu >= 0) g_bh_bpc = blocks_per_cu;
+  return 0;
+}
+
+// Launch shape the tree-list batch uses now: out2 = {threads per block,
+// persistent blocks per CU}.
+PGAMD_API int pg5_bh_launch_shape(int* out2) {
+  if (!out2) {
+    api_error(3, "pg5_bh_launch_shape: null output");
+    return -1;
+  }
+  out2[0] = bh_block();
+  out2[1] = bh_bpc();
   return 0;
 }
 

@@ -97,6 +97,7 @@ Table g_tab;
 Mlp g_mlp;
 int g_cut_mode = 0;  // 0: table, 1: learned MLP
 int g_batch_bpc[2] = {0, 0};  // persistent blocks per CU per mode (0 = default)
+int g_batch_block = 0;        // table-mode block (= j-tile) size (0 = default 1024)
 Board g_boards[kMaxDev];
 std::mutex g_mutex;  // guards the table and board creation
 int g_force_bi = 0;
@@ -459,12 +460,34 @@ mlp::Cut mlp_cut(hipStream_t st) {
   return c;
 }
 
+template <int BLOCK, bool QUAD>
+void launch_table_batch(int blocks, size_t lds, hipStream_t st, int ngroups, const int* order,
+                        const int* ioff, const int* icnt, const int* ilist, const int* joff,
+                        const int* jcnt, const int* jlist, const float4* pos,
+                        const pgb::Origin& og, const pgb::TableCut& tc, double ascale,
+                        double* acc, int* counter) {
+  hipLaunchKernelGGL((pgb::treepm_batch_table<BLOCK, QUAD>), dim3(blocks), dim3(BLOCK), lds, st,
+                     ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, pos, og, tc, ascale,
+                     acc, counter);
+}
+
+// Table-mode block sizes (each also the j-tile staged in LDS per step).
+constexpr int kTableBlocks[3] = {256, 512, 1024};
+
+bool valid_table_block(int block) {
+  for (int b : kTableBlocks)
+    if (b == block) return true;
+  return false;
+}
+
 // Launches the persistent batch kernel (counter reset + kernel) on st.
-// pos64 (nullable): FP64 positions for group-origin separations.
+// pos64 (nullable): FP64 positions for group-origin separations.  block:
+// threads per block of the table kernel (256, 512 or 1024); the network
+// kernel always runs 256.
 void launch_batch(int mode, int ngroups, const int* order, const int* ioff, const int* icnt,
                   const int* ilist, const int* joff, const int* jcnt, const int* jlist,
                   const float4* pos, const double* pos64, double* acc, int* counter, int bpc,
-                  hipStream_t st) {
+                  int block, hipStream_t st) {
   hipLaunchKernelGGL(pgb::zero_counter, dim3(1), dim3(64), 0, st, counter);
   PGAMD_HIP_CHECK(hipGetLastError());
   if (ngroups <= 0) return;
@@ -477,16 +500,27 @@ void launch_batch(int mode, int ngroups, const int* order, const int* ioff, cons
     hipLaunchKernelGGL(pgb::treepm_batch_mlp, dim3(blocks), dim3(pgb::kThreads), 0, st, ngroups,
                        order, ioff, icnt, ilist, joff, jcnt, jlist, pos, og, mc, ascale, acc,
                        counter);
-  } else if (g_tab.order == 2) {
-    const size_t lds = sizeof(float4) * (size_t)g_tab.ntab;
-    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock, true>), dim3(blocks),
-                       dim3(pgb::kTableBlock), lds, st, ngroups, order, ioff, icnt, ilist, joff,
-                       jcnt, jlist, pos, og, tc, ascale, acc, counter);
   } else {
-    const size_t lds = sizeof(float2) * (size_t)g_tab.ntab;
-    hipLaunchKernelGGL((pgb::treepm_batch_table<pgb::kTableBlock, false>), dim3(blocks),
-                       dim3(pgb::kTableBlock), lds, st, ngroups, order, ioff, icnt, ilist, joff,
-                       jcnt, jlist, pos, og, tc, ascale, acc, counter);
+    const bool quad = g_tab.order == 2;
+    const size_t lds = (quad ? sizeof(float4) : sizeof(float2)) * (size_t)g_tab.ntab;
+#define PGB_TABLE_LAUNCH(B, Q)                                                                 \
+  launch_table_batch<B, Q>(blocks, lds, st, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, \
+                           pos, og, tc, ascale, acc, counter)
+    switch (block) {
+      case 256:
+        if (quad) PGB_TABLE_LAUNCH(256, true); else PGB_TABLE_LAUNCH(256, false);
+        break;
+      case 512:
+        if (quad) PGB_TABLE_LAUNCH(512, true); else PGB_TABLE_LAUNCH(512, false);
+        break;
+      case 1024:
+        if (quad) PGB_TABLE_LAUNCH(1024, true); else PGB_TABLE_LAUNCH(1024, false);
+        break;
+      default:
+        api_error(3, "treepm batch: block %d not in {256, 512, 1024}", block);
+        return;
+    }
+#undef PGB_TABLE_LAUNCH
   }
   PGAMD_HIP_CHECK(hipGetLastError());
 }
@@ -494,6 +528,11 @@ void launch_batch(int mode, int ngroups, const int* order, const int* ioff, cons
 int default_bpc(int mode) {
   if (g_batch_bpc[mode] > 0) return g_batch_bpc[mode];
   return mode == 1 ? 4 : 2;
+}
+
+int default_block(int mode) {
+  if (mode == 1) return pgb::kThreads;
+  return g_batch_block > 0 ? g_batch_block : pgb::kTableBlock;
 }
 
 // Drop-in force through the batch kernel (used in MLP mode): device i and j
@@ -528,7 +567,7 @@ void run_mlp_dropin(DevBuf& scratch, int ni, const float4* xi, int nj, const flo
   }
   PGAMD_HIP_CHECK(hipMemcpyAsync(ilist, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
   launch_batch(1, ng, nullptr, ioff, icnt, ilist, joff, jcnt, jlist, pos, nullptr, accw, counter,
-               default_bpc(1), st);
+               default_bpc(1), default_block(1), st);
   PGAMD_HIP_CHECK(hipMemcpyAsync(acc, accw + 3 * (size_t)nj, sizeof(double) * 3 * (size_t)ni,
                                  hipMemcpyDeviceToDevice, st));
   PGAMD_HIP_CHECK(hipStreamSynchronize(st));
@@ -1187,7 +1226,7 @@ PGAMD_API int pg5t_batch_force(int devid, int ngroups, const int* ioff, const in
   b->d_counter.reserve(256);
   launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
                static_cast<const float4*>(pos), pos64, acc, b->d_counter.as<int>(),
-               default_bpc(mode), st);
+               default_bpc(mode), default_block(mode), st);
   return 0;
 }
 
@@ -1213,7 +1252,7 @@ PGAMD_API void* pg5t_batch_graph(int devid, int ngroups, const int* ioff, const 
   PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
                static_cast<const float4*>(pos), pos64, acc, bg->counter.as<int>(),
-               default_bpc(mode), cs);
+               default_bpc(mode), default_block(mode), cs);
   PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &bg->graph));
   PGAMD_HIP_CHECK(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
   PGAMD_HIP_CHECK(hipStreamDestroy(cs));
@@ -1242,10 +1281,14 @@ PGAMD_API int pg5t_batch_graph_destroy(void* h) {
   return 0;
 }
 
-// Autotuner: times the batch launch for persistent-grid sizes of
-// {1, 2, 3, 4, 6, 8, 12, 16} blocks per CU on these inputs (one warm-up, then
-// the mean of `reps`), keeps the fastest for this mode and returns it through
-// *best; ms[8] (nullable) receives the timings.
+// Autotuner: times the batch launch on these inputs for every block size
+// (table mode: 256, 512 and 1024 threads, each also the j-tile staged in
+// LDS; network mode: 256) and persistent-grid size of {1, 2, 3, 4, 6, 8, 12,
+// 16} blocks per CU (one warm-up, then the mean of `reps`), keeps the fastest
+// pair for this mode and returns its blocks per CU through *best; ms[8]
+// (nullable) receives the timings at the block size kept, which
+// pg5t_batch_launch_shape reports.  Results depend on the block size only
+// through the order of the FP32 partial sums.
 PGAMD_API int pg5t_batch_autotune(int devid, int ngroups, const int* ioff, const int* icnt,
                                   const int* ilist, const int* joff, const int* jcnt,
                                   const int* jlist, const int* order, const void* pos,
@@ -1257,38 +1300,72 @@ PGAMD_API int pg5t_batch_autotune(int devid, int ngroups, const int* ioff, const
   if (!batch_check("pg5t_batch_autotune", mode, ngroups)) return -1;
   ensure_table();
   const int cand[8] = {1, 2, 3, 4, 6, 8, 12, 16};
+  const int nblk = mode == 1 ? 1 : 3;
   hipStream_t st = b->stream;
   b->d_counter.reserve(256);
   hipEvent_t e0, e1;
   PGAMD_HIP_CHECK(hipEventCreate(&e0));
   PGAMD_HIP_CHECK(hipEventCreate(&e1));
-  float bestt = 1e30f;
-  int bestc = default_bpc(mode);
+  float bestt = 1e30f, t_all[3][8];
+  int bestc = default_bpc(mode), bestb = default_block(mode);
   reps = std::max(1, reps);
-  for (int k = 0; k < 8; ++k) {
-    auto go = [&] {
-      launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
-                   static_cast<const float4*>(pos), pos64, acc, b->d_counter.as<int>(), cand[k],
-                   st);
-    };
-    go();
-    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) go();
-    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
-    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
-    float t = 0.f;
-    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-    t /= reps;
-    if (ms) ms[k] = t;
-    if (t < bestt) {
-      bestt = t;
-      bestc = cand[k];
+  for (int kb = 0; kb < nblk; ++kb) {
+    const int blk = mode == 1 ? pgb::kThreads : kTableBlocks[kb];
+    for (int k = 0; k < 8; ++k) {
+      auto go = [&] {
+        launch_batch(mode, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist,
+                     static_cast<const float4*>(pos), pos64, acc, b->d_counter.as<int>(), cand[k],
+                     blk, st);
+      };
+      go();
+      PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) go();
+      PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+      PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+      float t = 0.f;
+      PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+      t /= reps;
+      t_all[kb][k] = t;
+      if (t < bestt) {
+        bestt = t;
+        bestc = cand[k];
+        bestb = blk;
+      }
     }
   }
   PGAMD_HIP_CHECK(hipEventDestroy(e0));
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
   g_batch_bpc[mode] = bestc;
+  if (mode == 0) g_batch_block = bestb;
+  if (ms) {
+    int kb = 0;
+    while (mode == 0 && kTableBlocks[kb] != bestb) ++kb;
+    for (int k = 0; k < 8; ++k) ms[k] = t_all[kb][k];
+  }
   if (best) *best = bestc;
+  return 0;
+}
+
+// Table-mode block size for the batched calls: 256, 512 or 1024 threads
+// (each also the j-tile staged in LDS per step), 0 = default (1024).
+PGAMD_API int pg5t_batch_set_block(int block) {
+  if (block != 0 && !valid_table_block(block)) {
+    api_error(3, "pg5t_batch_set_block: block %d not in {0, 256, 512, 1024}", block);
+    return -1;
+  }
+  g_batch_block = block;
+  return 0;
+}
+
+// Launch shape the batched calls of this mode use now: out2 = {threads per
+// block, persistent blocks per CU}.
+PGAMD_API int pg5t_batch_launch_shape(int mode, int* out2) {
+  if ((mode != 0 && mode != 1) || !out2) {
+    api_error(3, "pg5t_batch_launch_shape: mode=%d out=%p", mode, (void*)out2);
+    return -1;
+  }
+  out2[0] = default_block(mode);
+  out2[1] = default_bpc(mode);
   return 0;
 }
 

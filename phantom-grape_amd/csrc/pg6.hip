@@ -1113,7 +1113,9 @@ PGAMD_API int pg6_dev_timing_read(double* ms, int* count) {
 // ------------------------------------------------------ batched many clusters
 
 namespace {
-int g_cl_bpc = 0;  // persistent blocks per CU (0 = default 4)
+int g_cl_bpc = 0;   // persistent blocks per CU (0 = default 4)
+int g_cl_tile = 0;  // j-particles per LDS tile (0 = default 256)
+constexpr int kClTiles[3] = {128, 256, 512};
 DevBuf g_cl_counter;
 
 struct ClArgs {
@@ -1125,17 +1127,32 @@ struct ClArgs {
   double *acc, *jrk, *pot;
 };
 
-void launch_cl(const ClArgs& a, int* counter, int bpc, hipStream_t st) {
+template <int TILE>
+void launch_cl_t(const ClArgs& a, int* counter, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(pgc::cluster_force<TILE>, dim3(blocks), dim3(pgc::kThreads), 0, st,
+                     a.nitems, a.items, a.coff, a.pos, a.vel, a.mass, a.eps2, a.acc, a.jrk, a.pot,
+                     counter);
+}
+
+// Counter reset + persistent kernel on st; tile in kClTiles.
+void launch_cl(const ClArgs& a, int* counter, int bpc, int tile, hipStream_t st) {
   hipLaunchKernelGGL(pgc::zero_counter, dim3(1), dim3(64), 0, st, counter);
   PGAMD_HIP_CHECK(hipGetLastError());
   if (a.nitems <= 0) return;
   const int blocks = std::max(1, std::min(a.nitems, device_cu_count() * bpc));
-  hipLaunchKernelGGL(pgc::cluster_force, dim3(blocks), dim3(pgc::kThreads), 0, st, a.nitems,
-                     a.items, a.coff, a.pos, a.vel, a.mass, a.eps2, a.acc, a.jrk, a.pot, counter);
+  switch (tile) {
+    case 128: launch_cl_t<128>(a, counter, blocks, st); break;
+    case 256: launch_cl_t<256>(a, counter, blocks, st); break;
+    case 512: launch_cl_t<512>(a, counter, blocks, st); break;
+    default:
+      api_error(3, "cluster batch: tile %d not in {128, 256, 512}", tile);
+      return;
+  }
   PGAMD_HIP_CHECK(hipGetLastError());
 }
 
 int cl_bpc() { return g_cl_bpc > 0 ? g_cl_bpc : 4; }
+int cl_tile() { return g_cl_tile > 0 ? g_cl_tile : pgc::kTile; }
 
 struct ClGraph {
   hipGraph_t graph = nullptr;
@@ -1187,7 +1204,8 @@ PGAMD_API int pg6_cluster_force(int nitems, const int* items, const int* coff, c
   g_cl_counter.reserve(256);
   launch_cl(ClArgs{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
                    jrk, pot},
-            g_cl_counter.as<int>(), cl_bpc(), stream ? (hipStream_t)stream : g_board.stream);
+            g_cl_counter.as<int>(), cl_bpc(), cl_tile(),
+            stream ? (hipStream_t)stream : g_board.stream);
   return 0;
 }
 
@@ -1203,7 +1221,7 @@ PGAMD_API void* pg6_cluster_graph(int nitems, const int* items, const int* coff,
   PGAMD_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   launch_cl(ClArgs{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
                    jrk, pot},
-            g->counter.as<int>(), cl_bpc(), cs);
+            g->counter.as<int>(), cl_bpc(), cl_tile(), cs);
   PGAMD_HIP_CHECK(hipStreamEndCapture(cs, &g->graph));
   PGAMD_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   PGAMD_HIP_CHECK(hipStreamDestroy(cs));
@@ -1231,13 +1249,22 @@ PGAMD_API int pg6_cluster_graph_destroy(void* h) {
   return 0;
 }
 
-// Autotuner over {1, 2, 3, 4, 6, 8, 12, 16} persistent blocks per CU.
+// Autotuner over j-tiles of 128, 256 and 512 particles and {1, 2, 3, 4, 6,
+// 8, 12, 16} persistent blocks per CU on these inputs (one warm-up, mean of
+// reps): keeps the fastest pair and returns its blocks per CU through
+// *best; ms[8] (nullable) receives the timings at the tile kept, which
+// pg6_cluster_launch_shape reports.  Results depend on the tile only through
+// the order of the FP32 partial sums.
 PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff,
                                    const double* pos, const double* vel, const double* mass,
                                    const float* eps2, double* acc, double* jrk, double* pot,
                                    int reps, int* best, float* ms) {
   std::lock_guard<std::mutex> lk(g_mutex);
   if (!board_check_open("pg6_cluster_autotune")) return -1;
+  if (nitems < 0) {
+    api_error(3, "pg6_cluster_autotune: nitems=%d", nitems);
+    return -1;
+  }
   const ClArgs a{nitems, reinterpret_cast<const int2*>(items), coff, pos, vel, mass, eps2, acc,
                  jrk, pot};
   g_cl_counter.reserve(256);
@@ -1246,28 +1273,66 @@ PGAMD_API int pg6_cluster_autotune(int nitems, const int* items, const int* coff
   hipEvent_t e0, e1;
   PGAMD_HIP_CHECK(hipEventCreate(&e0));
   PGAMD_HIP_CHECK(hipEventCreate(&e1));
-  float bestt = 1e30f;
-  int bestc = cl_bpc();
+  float bestt = 1e30f, t_all[3][8];
+  int bestc = cl_bpc(), bestkt = 1;
+  while (bestkt < 2 && kClTiles[bestkt] != cl_tile()) ++bestkt;
   reps = std::max(1, reps);
-  for (int k = 0; k < 8; ++k) {
-    launch_cl(a, g_cl_counter.as<int>(), cand[k], st);
-    PGAMD_HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) launch_cl(a, g_cl_counter.as<int>(), cand[k], st);
-    PGAMD_HIP_CHECK(hipEventRecord(e1, st));
-    PGAMD_HIP_CHECK(hipEventSynchronize(e1));
-    float t = 0.f;
-    PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-    t /= reps;
-    if (ms) ms[k] = t;
-    if (t < bestt) {
-      bestt = t;
-      bestc = cand[k];
+  for (int kt = 0; kt < 3; ++kt)
+    for (int k = 0; k < 8; ++k) {
+      launch_cl(a, g_cl_counter.as<int>(), cand[k], kClTiles[kt], st);
+      PGAMD_HIP_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r)
+        launch_cl(a, g_cl_counter.as<int>(), cand[k], kClTiles[kt], st);
+      PGAMD_HIP_CHECK(hipEventRecord(e1, st));
+      PGAMD_HIP_CHECK(hipEventSynchronize(e1));
+      float t = 0.f;
+      PGAMD_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+      t /= reps;
+      t_all[kt][k] = t;
+      if (t < bestt) {
+        bestt = t;
+        bestc = cand[k];
+        bestkt = kt;
+      }
     }
-  }
   PGAMD_HIP_CHECK(hipEventDestroy(e0));
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
   g_cl_bpc = bestc;
+  g_cl_tile = kClTiles[bestkt];
+  if (ms)
+    for (int k = 0; k < 8; ++k) ms[k] = t_all[bestkt][k];
   if (best) *best = bestc;
+  return 0;
+}
+
+// Launch shape of the cluster batch: tile = j-particles per LDS tile (128,
+// 256 or 512; 0 = default 256), blocks_per_cu (0 = default 4); a negative
+// value leaves that part as it is.
+PGAMD_API int pg6_cluster_set_launch(int tile, int blocks_per_cu) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (tile > 0 && tile != 128 && tile != 256 && tile != 512) {
+    api_error(3, "pg6_cluster_set_launch: tile %d not in {0, 128, 256, 512}", tile);
+    return -1;
+  }
+  if (blocks_per_cu > 64) {
+    api_error(3, "pg6_cluster_set_launch: blocks_per_cu %d > 64", blocks_per_cu);
+    return -1;
+  }
+  if (tile >= 0) g_cl_tile = tile;
+  if (blocks_per_cu >= 0) g_cl_bpc = blocks_per_cu;
+  return 0;
+}
+
+// Launch shape the cluster batch uses now: out2 = {j-tile, persistent blocks
+// per CU}.
+PGAMD_API int pg6_cluster_launch_shape(int* out2) {
+  if (!out2) {
+    api_error(3, "pg6_cluster_launch_shape: null output");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mutex);
+  out2[0] = cl_tile();
+  out2[1] = cl_bpc();
   return 0;
 }
 

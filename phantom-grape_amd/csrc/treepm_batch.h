@@ -32,7 +32,7 @@ namespace pgb {
 
 constexpr int kThreads = 256;    // treepm_batch_mlp block
 constexpr int kJTile = 256;
-constexpr int kTableBlock = 1024;  // treepm_batch_table block
+constexpr int kTableBlock = 1024;  // treepm_batch_table default block (256, 512 also built)
 
 // Table evaluator: s = 2 + |dx_s|^2, f from the LDS table (linear: the
 // reference's {value, slope}; quadratic: {c0, c1, c2, 0}).

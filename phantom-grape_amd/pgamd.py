@@ -154,6 +154,8 @@ class G5:
         _sig(L, "pg5_bh_graph_launch", C.c_int, _vp, _vp)
         _sig(L, "pg5_bh_graph_destroy", C.c_int, _vp)
         _sig(L, "pg5_bh_autotune", C.c_int, C.c_int, *bh, C.c_int, _ip, _fp)
+        _sig(L, "pg5_bh_set_launch", C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg5_bh_launch_shape", C.c_int, _ip)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
         L.g5_open()
@@ -260,6 +262,8 @@ class G6:
         _sig(L, "pg6_cluster_graph_launch", C.c_int, _vp, _vp)
         _sig(L, "pg6_cluster_graph_destroy", C.c_int, _vp)
         _sig(L, "pg6_cluster_autotune", C.c_int, *cl, C.c_int, _ip, _fp)
+        _sig(L, "pg6_cluster_set_launch", C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg6_cluster_launch_shape", C.c_int, _ip)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)
         L.g6_open(0)
@@ -424,6 +428,8 @@ class G5Table:
         _sig(L, "pg5t_batch_graph_destroy", C.c_int, _vp)
         _sig(L, "pg5t_batch_autotune", C.c_int, C.c_int, *bt, C.c_int, _ip, _fp)
         _sig(L, "pg5t_batch_set_blocks_per_cu", C.c_int, C.c_int, C.c_int)
+        _sig(L, "pg5t_batch_set_block", C.c_int, C.c_int)
+        _sig(L, "pg5t_batch_launch_shape", C.c_int, C.c_int, _ip)
         _sig(L, "pg5t_pm_force", C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_double, _vp, _vp)
         _sig(L, "pgamd_set_abort_on_error", None, C.c_int)
         _sig(L, "pgamd_last_error", C.c_int, C.c_char_p, C.c_int)

@@ -139,7 +139,7 @@ def test_treepm_batch_table_matches_dropin_and_closed_form(tab, treepm_case):
     assert np.median(e) < 3e-3
 
 
-def test_treepm_graph_replay_and_autotune(tab, treepm_case):
+def test_treepm_graph_replay(tab, treepm_case):
     import torch
     c = treepm_case
     d = c["d"]
@@ -159,12 +159,57 @@ def test_treepm_graph_replay_and_autotune(tab, treepm_case):
             run_treepm(tab, c, 0)  # eager work between replays
     finally:
         tab.lib.pg5t_batch_graph_destroy(h)
-    best = np.zeros(1, np.int32)
-    ms = np.zeros(8, np.float32)
+
+
+def test_treepm_block_sizes_and_autotune(tab, treepm_case):
+    """Each block size of the table kernel (256, 512 or 1024 threads, each
+    also the j-tile staged in LDS) matches the oracle's bit-exact emulation
+    of the reference's pair terms up to summation order; the autotuner
+    searches block x blocks-per-CU, reports the pair it kept, and a call at
+    that shape equals a call at the same shape set by hand."""
     import pgamd
-    assert tab.lib.pg5t_batch_autotune(*args, 2, pgamd.iptr(best), pgamd.fptr(ms)) == 0
-    assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16) and np.all(ms > 0)
-    np.testing.assert_array_equal(run_treepm(tab, c, 0), eager)
+    c = treepm_case
+    L, P = c["L"], c["L"]["pos"]
+    t, xs = tab.table()
+    out = {}
+    shape = np.zeros(2, np.int32)
+    try:
+        for blk in (256, 512, 1024):
+            assert tab.lib.pg5t_batch_set_block(blk) == 0
+            assert tab.lib.pg5t_batch_launch_shape(0, pgamd.iptr(shape)) == 0
+            assert shape[0] == blk
+            out[blk] = a = run_treepm(tab, c, 0, p64=True)
+            for g in np.random.default_rng(blk).choice(len(L["ioff"]), 6, replace=False):
+                il, jl = group_lists(L, g)
+                o = P[il[0]]
+                ref = O.table_force_emul(P[il] - o, P[jl] - o, c["m"][jl], t, xs, 4, 5)
+                scale = np.linalg.norm(ref, axis=1).max()
+                if scale == 0:  # no j inside the cut-off
+                    assert np.all(a[il] == 0), (blk, int(g))
+                    continue
+                e = rel_err(a[il], ref, floor=1e-3 * scale).max()
+                assert e < 1e-6, (blk, int(g), len(il), e)
+        d = c["d"]
+        import torch
+        acc = torch.zeros((len(c["pos4"]), 3), dtype=torch.float64, device="cuda")
+        args = (0, len(L["ioff"]), ptr(d["ioff"]), ptr(d["icnt"]), ptr(d["ilist"]),
+                ptr(d["joff"]), ptr(d["jcnt"]), ptr(d["jlist"]), ptr(d["order"]), ptr(d["pos"]),
+                ptr(d["pos64"]), ptr(acc), 0)
+        best = np.zeros(1, np.int32)
+        ms = np.zeros(8, np.float32)
+        assert tab.lib.pg5t_batch_autotune(*args, 2, pgamd.iptr(best), pgamd.fptr(ms)) == 0
+        assert tab.lib.pg5t_batch_launch_shape(0, pgamd.iptr(shape)) == 0
+        print(f"treepm batch autotune: block {shape[0]}, {shape[1]} blocks per CU, "
+              f"ms {np.round(ms, 3).tolist()}")
+        assert shape[0] in out and shape[1] == best[0] and np.all(ms > 0)
+        np.testing.assert_array_equal(run_treepm(tab, c, 0, p64=True), out[int(shape[0])])
+        tab.lib.pgamd_set_abort_on_error(0)
+        assert tab.lib.pg5t_batch_set_block(300) == -1
+        assert tab.lib.pg5t_batch_launch_shape(2, pgamd.iptr(shape)) == -1
+    finally:
+        tab.lib.pgamd_set_abort_on_error(1)
+        tab.lib.pg5t_batch_set_block(0)
+        tab.lib.pg5t_batch_set_blocks_per_cu(0, 0)
 
 
 def test_mlp_normal_equations_match_oracle(tab):
@@ -335,6 +380,67 @@ def test_bh_batch_matches_oracle_and_graph(bh_case):
     best = np.zeros(1, np.int32)
     assert g5.lib.pg5_bh_autotune(*args, 2, pgamd.iptr(best), None) == 0
     assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16)
+    g5.lib.pg5_bh_set_launch(0, 0)
+
+
+def test_bh_block_sizes_and_autotune(bh_case):
+    """Each block size of the tree-list kernel (256, 512 or 1024 threads; an
+    LDS tile holds that many particles or half as many cells; groups of up to
+    512 i run in passes at 256) matches the oracle over the same lists; the
+    autotuner searches block x blocks-per-CU, reports the pair it kept, and a
+    call at that shape equals a call at the same shape set by hand."""
+    import torch
+
+    import pgamd
+    c = bh_case
+    L = c["L"]
+    g5 = c["g5"]
+    eps = 1.0 / 128
+    args, acc, pot = run_bh(c, eps)
+    f32 = lambda z: np.asarray(z, np.float32).astype(np.float64)  # noqa: E731
+    P4 = c["pos4"].astype(np.float64)
+    Cc = c["cells"].astype(np.float64)
+    groups = np.random.default_rng(4).choice(len(L["ioff"]), 4, replace=False)
+    big = np.argmax(L["icnt"])
+    assert L["icnt"][big] > 256  # a group that takes two passes at block 256
+    groups = np.append(groups, big)
+    out = {}
+    shape = np.zeros(2, np.int32)
+    try:
+        for blk in (256, 512, 1024):
+            assert g5.lib.pg5_bh_set_launch(blk, -1) == 0
+            assert g5.lib.pg5_bh_launch_shape(pgamd.iptr(shape)) == 0
+            assert shape[0] == blk
+            assert g5.lib.pg5_bh_batch(*args, None) == 0
+            torch.cuda.synchronize()
+            a, p = acc.cpu().numpy(), pot.cpu().numpy()
+            out[blk] = (a, p)
+            for g in groups:
+                il = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
+                pl = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
+                cl = L["clist"][L["coff"][g]:L["coff"][g] + L["ccnt"][g]]
+                ap, pp = O.pg5_force(P4[il, :3], f32(eps * eps), P4[pl, :3], P4[pl, 3])
+                aq, pq = O.quad_force(P4[il, :3], f32(eps * eps), Cc[cl, :3], Cc[cl, 3],
+                                      Cc[cl, 4:10])
+                assert np.percentile(rel_err(a[il], ap + aq), 99) < 1e-5, blk
+                assert rel_err(p[il], pp + pq).max() < 1e-5, blk
+        best = np.zeros(1, np.int32)
+        ms = np.zeros(8, np.float32)
+        assert g5.lib.pg5_bh_autotune(*args, 2, pgamd.iptr(best), pgamd.fptr(ms)) == 0
+        assert g5.lib.pg5_bh_launch_shape(pgamd.iptr(shape)) == 0
+        print(f"tree-list batch autotune: block {shape[0]}, {shape[1]} blocks per CU, "
+              f"ms {np.round(ms, 3).tolist()}")
+        assert shape[0] in out and shape[1] == best[0] and np.all(ms > 0)
+        assert g5.lib.pg5_bh_batch(*args, None) == 0
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(acc.cpu().numpy(), out[int(shape[0])][0])
+        np.testing.assert_array_equal(pot.cpu().numpy(), out[int(shape[0])][1])
+        g5.lib.pgamd_set_abort_on_error(0)
+        assert g5.lib.pg5_bh_set_launch(300, -1) == -1
+        assert g5.lib.pg5_bh_set_launch(-1, 65) == -1
+    finally:
+        g5.lib.pgamd_set_abort_on_error(1)
+        g5.lib.pg5_bh_set_launch(0, 0)
 
 
 def test_clusters_match_oracle_and_graph():
@@ -385,6 +491,45 @@ def test_clusters_match_oracle_and_graph():
     best = np.zeros(1, np.int32)
     assert g6.lib.pg6_cluster_autotune(*args, 2, pgamd.iptr(best), None) == 0
     assert best[0] in (1, 2, 3, 4, 6, 8, 12, 16)
+    # every j-tile against the oracle (clusters of 48 to 1024 particles: one
+    # partial tile up to eight whole ones), then the autotuner's pick
+    out = {}
+    shape = np.zeros(2, np.int32)
+    try:
+        for tile in (128, 256, 512):
+            assert g6.lib.pg6_cluster_set_launch(tile, -1) == 0
+            assert g6.lib.pg6_cluster_launch_shape(pgamd.iptr(shape)) == 0
+            assert shape[0] == tile
+            assert g6.lib.pg6_cluster_force(*args, None) == 0
+            torch.cuda.synchronize()
+            out[tile] = [t.cpu().numpy() for t in (acc, jrk, pot)]
+            a, j, p = out[tile]
+            for c in list(range(1, len(sizes), 5)) + [int(np.argmax(sizes))]:
+                s = slice(coff[c], coff[c + 1])
+                idx = np.arange(coff[c + 1] - coff[c], dtype=np.int32)
+                vf = V[s].astype(np.float32).astype(np.float64)
+                o = O.pg6_force(X[s], vf, idx, float(np.float32(e2[c])), X[s], vf, Mm[s], idx)
+                assert np.percentile(rel_err(a[s], o["acc"]), 99) < 1e-6, tile
+                assert np.percentile(rel_err(j[s], o["jrk"]), 99) < 1e-5, tile
+                assert rel_err(p[s], o["pot"]).max() < 1e-6, tile
+        # FP32 sums fold into FP64 every 256 j, so tiles of 256 and 512 agree
+        for r256, r512 in zip(out[256], out[512]):
+            np.testing.assert_array_equal(r512, r256)
+        ms = np.zeros(8, np.float32)
+        assert g6.lib.pg6_cluster_autotune(*args, 2, pgamd.iptr(best), pgamd.fptr(ms)) == 0
+        assert g6.lib.pg6_cluster_launch_shape(pgamd.iptr(shape)) == 0
+        print(f"cluster batch autotune: tile {shape[0]}, {shape[1]} blocks per CU, "
+              f"ms {np.round(ms, 3).tolist()}")
+        assert shape[0] in out and shape[1] == best[0] and np.all(ms > 0)
+        assert g6.lib.pg6_cluster_force(*args, None) == 0
+        torch.cuda.synchronize()
+        for t, r in zip((acc, jrk, pot), out[int(shape[0])]):
+            np.testing.assert_array_equal(t.cpu().numpy(), r)
+        g6.lib.pgamd_set_abort_on_error(0)
+        assert g6.lib.pg6_cluster_set_launch(64, -1) == -1
+    finally:
+        g6.lib.pgamd_set_abort_on_error(1)
+        g6.lib.pg6_cluster_set_launch(0, 0)
 
 
 def test_quadratic_table_with_group_origins(tab, treepm_case):
