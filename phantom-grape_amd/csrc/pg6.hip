@@ -76,9 +76,13 @@ inline int ni_bucket(int ni) {
 // for large ones (the two-float kernel below kSortedMinNj j; for the largest,
 // four wavefronts per 64-i block sharing each j-range, 4x fewer partial sums
 // than one wavefront per range), and many j-ranges so that the last wave of
-// blocks is a small fraction of the launch.
+// blocks is a small fraction of the launch.  Up to 128 i, one block per CU
+// (one i-block of 64 or 128): the call is latency bound, and more j-ranges
+// only lengthen the sum over them (64 i: 19.9 us per predict + force call
+// against 24.1 us with twice the ranges).
 inline LaunchChoice default_choice(int ni) {
-  if (ni <= 128) return {0, 64, 32};
+  if (ni <= 64) return {0, 64, 1};
+  if (ni <= 128) return {0, 128, 1};
   if (ni <= 2048) return {0, 64, 4};
   if (ni <= 8192) return {4, 256, 16};
   if (ni <= 32768) return {4, 64, 32};
@@ -116,6 +120,13 @@ struct Board {
   int predict_flag = 0;
   bool pred_armed = false;
   int pred_nj = 0;
+  // pred_virtual: the last force predicted the first virt_nj j-particles to
+  // ti inside its own launch and wrote no predicted set (predict_flag stays
+  // set, so later calls at ti predict the same j-state again).  j-records
+  // flushed before the next g6_set_ti must not enter this prediction, so the
+  // flush writes it out first (board_predict); a new ti drops it.
+  bool pred_virtual = false;
+  int virt_nj = 0;
   int nj_pred = -1;  // j count of the last prediction
   // i-set and outputs of the host API
   HostBuf h_in, h_out;
@@ -341,7 +352,10 @@ void board_predict(Board& b, int nj, hipStream_t st, bool sorted = false);
 
 void board_flush_j(Board& b) {
   if (b.npend == 0) return;
-  if (b.predict_flag && b.pred_armed) board_predict(b, b.pred_nj, b.stream);
+  if (b.predict_flag && b.pred_armed)
+    board_predict(b, b.pred_nj, b.stream);
+  else if (b.pred_virtual)
+    board_predict(b, b.virt_nj, b.stream);
   board_use_stream(b, b.stream);
   // the scatter kernel reads the pinned queue through its device mapping
   hipLaunchKernelGGL(pg6::pg6_scatter_j, dim3(ceil_div(b.npend, 256)), dim3(256), 0, b.stream,
@@ -360,6 +374,7 @@ void board_flush_j(Board& b) {
 void board_predict(Board& b, int nj, hipStream_t st, bool sorted) {
   b.predict_flag = 0;
   b.pred_armed = false;
+  b.pred_virtual = false;
   if (nj <= 0) return;
   board_use_stream(b, st);
   hipLaunchKernelGGL(pg6::pg6_predict, dim3(ceil_div(nj, 256)), dim3(256), 0, st, nj, b.ti,
@@ -469,7 +484,8 @@ LaunchPlan plan_launch(const Board& b, int ni, int nj) {
 }
 
 // pred_in: the LDS kernel predicts the j-set itself (variant 0 with one
-// i-block only, board_force decides), at time ti from js.
+// i-block only, board_force decides), at time ti from js, without writing
+// the predicted set.
 template <int BI, bool NN>
 void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const double* ivel,
                     const int* iid, float eps2, int nj, const JPred& jp, const pg6::JSort& so,
@@ -502,8 +518,13 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
     // one thread per i walks the splits; with many splits a block of 256
     // threads per i (up to 1024 i: four-thread sums would leave most CUs
     // idle) or four threads per i (more i, coalesced loads; 2-5 % faster
-    // calls at 2048 and 4096 i against N = 16384)
-    if (p.nsplit > 16 && ni <= 1024)
+    // calls at 2048 and 4096 i against N = 16384).  After an in-kernel
+    // prediction (one i-block) the neighbour's id and distance come with the
+    // partials: always the block-per-i sum.
+    if (pred_in)
+      hipLaunchKernelGGL((pg6::pg6_reduce_w<NN, true>), dim3(ni), dim3(256), 0, st, ni,
+                         p.nsplit, part, jp, ipos, eps2, out);
+    else if (p.nsplit > 16 && ni <= 1024)
       hipLaunchKernelGGL((pg6::pg6_reduce_w<NN>), dim3(ni), dim3(256), 0, st, ni,
                          p.nsplit, part, jp, ipos, eps2, out);
     else if (p.nsplit > 8)
@@ -546,7 +567,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   Partials part{};
   if (p.nsplit > 1) {
     const size_t n = (size_t)p.nsplit * ni;
-    b.d_part.reserve(n * (7 * sizeof(double) + sizeof(float) + sizeof(int)));
+    b.d_part.reserve(n * (7 * sizeof(double) + 2 * sizeof(float) + 2 * sizeof(int)));
     double* d = b.d_part.as<double>();
     part.ax = d;
     part.ay = d + n;
@@ -557,6 +578,8 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
     part.jz = d + 6 * n;
     part.rmax = reinterpret_cast<float*>(d + 7 * n);
     part.nn = reinterpret_cast<int*>(part.rmax + n);
+    part.nid = part.nn + n;
+    part.rdist = reinterpret_cast<float*>(part.nid + n);
   }
   const bool nn = out.nnb != nullptr;
   const float e2 = (float)eps2;
@@ -586,6 +609,7 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
       b.nj_pred = nj;
       b.predict_flag = 0;
       b.pred_armed = false;
+      b.pred_virtual = false;
       b.srec_ok = false;  // the sorted records hold an older prediction
     }
     board_done_on(b, st);
@@ -607,10 +631,12 @@ void board_force(Board& b, int ni, int nj, const double* ipos, const double* ive
   }
 #undef PG6_LAUNCH
   if (pred_in) {
-    b.nj_pred = nj;
-    b.predict_flag = 0;
+    // the prediction lived in the kernel's LDS only: calls at this ti predict
+    // again (same j-state, same bits) until a flush of new j-records writes
+    // it out first (board_flush_j)
     b.pred_armed = false;
-    b.srec_ok = false;  // the sorted records hold an older prediction
+    b.pred_virtual = true;
+    b.virt_nj = nj;
   }
   board_done_on(b, st);
 }
@@ -667,6 +693,8 @@ int lasthalf_impl(const char* fn, int nj, int ni, const int* pidx, const double 
   board_force(b, ni, nj, dpos, dvel, did, eps2, o, b.stream);
 
   if (want_nblist) {
+    // the list kernel reads the predicted set
+    if (board_needs_pred(b, nj)) board_predict(b, nj, b.stream);
     b.d_nb.reserve((size_t)ni * (kMaxLen + 1) * sizeof(int));
     int* dnbl = b.d_nb.as<int>();
     int* dlen = dnbl + (size_t)ni * kMaxLen;
@@ -765,6 +793,7 @@ PGAMD_API void g6_set_ti(int gpid, double ti) {
   std::lock_guard<std::mutex> lk(g_mutex);
   g_board.ti = ti;
   g_board.predict_flag = 1;
+  g_board.pred_virtual = false;
 }
 PGAMD_API void g6_set_ti_(int* gpid, double* ti) { g6_set_ti(*gpid, *ti); }
 
@@ -969,6 +998,7 @@ PGAMD_API int pg6_dev_predict(int nj, double ti, void* stream) {
   // Lazy: the first force after this (or a flush of new j-records) predicts;
   // pg6_force_f does it inside its own launch.  `stream` orders nothing here.
   (void)stream;
+  g_board.pred_virtual = false;  // superseded: pending records enter the new one
   board_flush_j(g_board);
   g_board.ti = ti;
   g_board.predict_flag = 1;
@@ -1421,7 +1451,7 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   const bool timer_was_on = g_timer.on;
   g_timer.on = false;
   const int save_bi = b.force_bi, save_bpc = b.force_bpc, save_var = b.force_variant;
-  static const int kBpc[] = {4, 8, 16, 32, 64};
+  static const int kBpc[] = {1, 2, 4, 8, 16, 32, 64};
   static const int kBpcF[] = {1, 2, 4, 8, 16};  // variant 3: 64 .. 1024 blocks
   int tuned = 0;
   for (int k = 0; k < kBuckets && (64 << k) <= nimax; ++k) {
@@ -1435,9 +1465,9 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
     // records, the one-launch kernel's in-kernel prediction).
     for (int var = 0; var <= 5; ++var)
       for (int bi : {64, 128, 256})
-        for (int bk = 0; bk < 5; ++bk) {
+        for (int bk = 0; bk < 7; ++bk) {
+          if (var == 3 && (bi != 64 || ni > kFusedMaxNi || bk >= 5)) continue;
           const int bpc = var == 3 ? kBpcF[bk] : kBpc[bk];
-          if (var == 3 && (bi != 64 || ni > kFusedMaxNi)) continue;
           if ((var == 4 || var == 5) && nj < kSortedMinNj) continue;
           b.force_variant = var;
           b.force_bi = bi;
@@ -1486,6 +1516,13 @@ PGAMD_API int pg6_dev_autotune(int nj, double eps2, int reps, int* out) {
   b.force_bi = save_bi;
   b.force_bpc = save_bpc;
   b.force_variant = save_var;
+  // the timed calls predicted the same j-state to the same ti again (some
+  // into the predicted set, some inside their launch): the set predicted
+  // before them still stands
+  b.predict_flag = 0;
+  b.pred_armed = false;
+  b.pred_virtual = false;
+  b.nj_pred = std::max(b.nj_pred, nj);
   g_timer.on = timer_was_on;
   PGAMD_HIP_CHECK(hipEventDestroy(e0));
   PGAMD_HIP_CHECK(hipEventDestroy(e1));
@@ -1564,6 +1601,7 @@ void hermite_enqueue(Board& b, Hermite& h, hipStream_t st) {
   b.nj_pred = -1;  // the board's predictions now belong to the integrator
   b.sort_age = kSortAge;  // its corrector moves every particle: rebuild the order
   b.predict_flag = 1;
+  b.pred_virtual = false;
 }
 
 // Queues kGraphSteps block steps as one captured graph (captured on first
@@ -1589,6 +1627,7 @@ void hermite_enqueue_batch(Board& b, Hermite& h, hipStream_t st) {
   b.nj_pred = -1;
   b.sort_age = kSortAge;
   b.predict_flag = 1;
+  b.pred_virtual = false;
 }
 
 // Sets Ctl::t_end (through the pinned buffer; the caller synchronises before
@@ -1725,6 +1764,7 @@ PGAMD_API int pg6_hermite_init(int n, const double* x, const double* v, const do
   PGAMD_HIP_CHECK(hipGetLastError());
   const pg6h::Ctl r = hermite_read(h, st);
   b.predict_flag = 1;
+  b.pred_virtual = false;
   b.nj_pred = -1;
   h.on = true;
   if (r.bad) {

@@ -64,6 +64,10 @@ struct Partials {
   double *ax, *ay, *az, *pot, *jx, *jy, *jz;
   float* rmax;
   int* nn;  // j address of the nearest neighbour, -1 if none
+  // pg6_force with in-kernel prediction (which leaves the JPred arrays
+  // unwritten): that neighbour's id and distance
+  int* nid;
+  float* rdist;
 };
 
 // Final outputs in the API's layout: acc[i][3], jrk[i][3], pot[i], nnb[i], rnnb[i].
@@ -319,10 +323,13 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 // (group g takes j = g, g+G, ...), and are combined in fixed order at the end.
 // blockIdx.y selects the j-range [y*jper, min(nj,(y+1)*jper)).
 // With nsplit == 1 the kernel writes final outputs, otherwise partials.
-// PRED: the block predicts the j of each tile itself (pg6_predict_j at time
-// ti, writing the predicted set as pg6_predict does) instead of reading the
-// predicted set; for launches with one i-block, where every j is in exactly
-// one block's range, so that the separate predictor launch goes away.
+// PRED: the block predicts the j of each tile itself (the arithmetic of
+// pg6_predict_j at time ti) straight into LDS instead of reading the
+// predicted set, and writes no JPred array; for launches with one i-block,
+// where every j is in exactly one block's range, so that the separate
+// predictor launch and the predicted set's round trip through memory go
+// away.  The nearest neighbour's id and distance then come from the tile in
+// LDS (Partials::nid / rdist, or the outputs directly).
 static_assert(kThreads == kJTile, "one thread stages one j of a tile");
 
 template <int BI, bool NN, bool PRED>
@@ -363,21 +370,34 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
   // the lowest j address wins.  The self pair has 1/r = 0 and never wins.
   float rmax = 0.f;  // quantised 1/r of the nearest neighbour so far
   int nn = -1;
+  int nid = -1;       // PRED: its id
+  float rdist = 0.f;  // PRED: its distance
   const unsigned kmask = 0xFFFFFF00u;
 
   for (int jt = j0; jt < j1; jt += kJTile) {
     const int nt = min(kJTile, j1 - jt);
     __syncthreads();
-    // every thread of the block takes part (the predictor's chunk id ranges
-    // combine across each wavefront)
-    if (PRED) pg6_predict_j(jt + tid, tid < nt, ti, js, jp);
     if (tid < nt) {
       const int j = jt + tid;
-      s_x[tid] = jp.x[j];
-      s_y[tid] = jp.y[j];
-      s_z[tid] = jp.z[j];
-      s_vm[tid] = jp.vm[j];
-      s_id[tid] = jp.id[j];
+      if (PRED) {  // pg6_predict_j's arithmetic, into LDS only
+        const double dt = ti - js.t[j];
+        s_x[tid] = pg6_pred_pos(dt, js.x[j], js.vx[j], js.ax[j], js.kx[j]);
+        s_y[tid] = pg6_pred_pos(dt, js.y[j], js.vy[j], js.ay[j], js.ky[j]);
+        s_z[tid] = pg6_pred_pos(dt, js.z[j], js.vz[j], js.az[j], js.kz[j]);
+        float4 vm;
+        vm.x = (float)pg6_pred_vel(dt, js.vx[j], js.ax[j], js.kx[j]);
+        vm.y = (float)pg6_pred_vel(dt, js.vy[j], js.ay[j], js.ky[j]);
+        vm.z = (float)pg6_pred_vel(dt, js.vz[j], js.az[j], js.kz[j]);
+        vm.w = (float)js.m[j];
+        s_vm[tid] = vm;
+        s_id[tid] = js.idx[j];
+      } else {
+        s_x[tid] = jp.x[j];
+        s_y[tid] = jp.y[j];
+        s_z[tid] = jp.z[j];
+        s_vm[tid] = jp.vm[j];
+        s_id[tid] = jp.id[j];
+      }
     }
     __syncthreads();
 
@@ -416,7 +436,15 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
       const float rq = __uint_as_float(tkey & kmask);
       if (rq > rmax) {  // strict: an earlier tile keeps a tie
         rmax = rq;
-        nn = jt + 255 - (int)(tkey & 0xFFu);
+        const int slot = 255 - (int)(tkey & 0xFFu);
+        nn = jt + slot;
+        if (PRED) {  // nn_distance on the tile's copy of the predicted position
+          nid = s_id[slot];
+          const float dx = (float)(s_x[slot] - xi);
+          const float dy = (float)(s_y[slot] - yi);
+          const float dz = (float)(s_z[slot] - zi);
+          rdist = sqrtf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+        }
       }
     }
     ax += (double)fax;
@@ -433,11 +461,17 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
     __shared__ double r_d[7][kThreads];
     __shared__ float r_f[kThreads];
     __shared__ int r_n[kThreads];
+    __shared__ int r_id[PRED ? kThreads : 1];
+    __shared__ float r_dist[PRED ? kThreads : 1];
     __syncthreads();
     r_d[0][tid] = ax; r_d[1][tid] = ay; r_d[2][tid] = az; r_d[3][tid] = pot;
     r_d[4][tid] = jx; r_d[5][tid] = jy; r_d[6][tid] = jz;
     r_f[tid] = rmax;
     r_n[tid] = nn;
+    if (PRED) {
+      r_id[tid] = nid;
+      r_dist[tid] = rdist;
+    }
     __syncthreads();
     if (g == 0) {
       for (int q = 1; q < G; ++q) {
@@ -451,6 +485,10 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
           if (rq > rmax || (rq == rmax && nq >= 0 && (nn < 0 || nq < nn))) {
             rmax = rq;
             nn = nq;
+            if (PRED) {
+              nid = r_id[t];
+              rdist = r_dist[t];
+            }
           }
         }
       }
@@ -467,8 +505,13 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
     out.jrk[3 * i + 2] = jz;
     out.pot[i] = pot;
     if (NN) {
-      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
-      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+      if (PRED) {
+        out.nnb[i] = nn >= 0 ? nid : -1;
+        out.rnnb[i] = nn >= 0 ? rdist : 0.f;
+      } else {
+        out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+        out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, xi, yi, zi, eps2) : 0.f;
+      }
     }
   } else {
     const size_t o = (size_t)blockIdx.y * ni + i;
@@ -482,6 +525,10 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
     if (NN) {
       part.rmax[o] = rmax;
       part.nn[o] = nn;
+      if (PRED) {
+        part.nid[o] = nid;
+        part.rdist[o] = rdist;
+      }
     }
   }
 }
@@ -986,20 +1033,26 @@ __global__ __launch_bounds__(64 * W, 7) void pg6_force_ow(
 // together), a fixed xor-butterfly combines each wavefront's lanes, and the
 // four wavefront sums are added in wave order.  Deterministic.  For many
 // splits and few i, where one thread per i would walk nsplit dependent loads
-// and one wavefront per i still waited for several rounds of them.
-template <bool NN>
+// and one wavefront per i still waited for several rounds of them.  NNP: the
+// partials carry the neighbour's id and distance (pg6_force with in-kernel
+// prediction), which then need no read of the predicted set.
+template <bool NN, bool NNP = false>
 __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials part, JPred jp,
                                                     const double* __restrict__ ipos, float eps2,
                                                     Outputs out) {
   __shared__ double s_v[7][4];
   __shared__ float s_r[4];
   __shared__ int s_n[4];
+  __shared__ int s_id[4];
+  __shared__ float s_d[4];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int i = blockIdx.x;
   double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   float rmax = 0.f;
   int nn = -1;
+  int nid = -1;       // NNP: the neighbour's id and distance ride with it
+  float rdist = 0.f;
 #pragma unroll 4
   for (int s = threadIdx.x; s < nsplit; s += 256) {
     const size_t o = (size_t)s * ni + i;
@@ -1016,6 +1069,10 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
       if (nn_better(r, n, rmax, nn)) {
         rmax = r;
         nn = n;
+        if (NNP) {
+          nid = part.nid[o];
+          rdist = part.rdist[o];
+        }
       }
     }
   }
@@ -1026,9 +1083,13 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
     if (NN) {
       const float r = __shfl_xor(rmax, off, 64);
       const int n = __shfl_xor(nn, off, 64);
+      const int d = NNP ? __shfl_xor(nid, off, 64) : 0;
+      const float e = NNP ? __shfl_xor(rdist, off, 64) : 0.f;
       if (nn_better(r, n, rmax, nn)) {
         rmax = r;
         nn = n;
+        nid = d;
+        rdist = e;
       }
     }
   }
@@ -1037,6 +1098,8 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
     for (int k = 0; k < 7; ++k) s_v[k][w] = v[k];
     s_r[w] = rmax;
     s_n[w] = nn;
+    s_id[w] = nid;
+    s_d[w] = rdist;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -1044,12 +1107,16 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
   for (int k = 0; k < 7; ++k) v[k] = s_v[k][0];
   rmax = s_r[0];
   nn = s_n[0];
+  nid = s_id[0];
+  rdist = s_d[0];
   for (int g = 1; g < 4; ++g) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) v[k] += s_v[k][g];
     if (NN && nn_better(s_r[g], s_n[g], rmax, nn)) {
       rmax = s_r[g];
       nn = s_n[g];
+      nid = s_id[g];
+      rdist = s_d[g];
     }
   }
   out.acc[3 * i + 0] = v[0];
@@ -1060,10 +1127,15 @@ __global__ __launch_bounds__(256) void pg6_reduce_w(int ni, int nsplit, Partials
   out.jrk[3 * i + 2] = v[6];
   out.pot[i] = v[3];
   if (NN) {
-    out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
-    out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
-                                        eps2)
-                          : 0.f;
+    if (NNP) {
+      out.nnb[i] = nn >= 0 ? nid : -1;
+      out.rnnb[i] = nn >= 0 ? rdist : 0.f;
+    } else {
+      out.nnb[i] = nn >= 0 ? jp.id[nn] : -1;
+      out.rnnb[i] = nn >= 0 ? nn_distance(jp, nn, ipos[3 * i], ipos[3 * i + 1], ipos[3 * i + 2],
+                                          eps2)
+                            : 0.f;
+    }
   }
 }
 

@@ -274,7 +274,7 @@ def test_autotune_and_launch_shapes(g6):
         # variants 4 and 5 (sorted chunk origins) are candidates only from
         # 16384 j
         assert var in (0, 1, 2, 3) and bi in (64, 128, 256) and us > 0
-        assert bpc in ((1, 2, 4, 8, 16) if var == 3 else (4, 8, 16, 32, 64))
+        assert bpc in ((1, 2, 4, 8, 16) if var == 3 else (1, 2, 4, 8, 16, 32, 64))
         p = g6.plan(ni, n)
         assert (p["variant"], p["bi"], p["bpc"]) == (var, bi, bpc)
         assert p["nsplit"] * p["jper"] >= n
@@ -507,3 +507,78 @@ def test_j_updates_after_firsthalf_do_not_enter_its_prediction(g6):
                                       pgamd.dptr(acc), pgamd.dptr(jrk), pgamd.dptr(pot)) == 0
         assert np.percentile(rel_err(acc, ref["acc"]), 99) < 1e-6, k
     g6.lib.pg6_dev_set_kernel(-1)
+
+
+@pytest.mark.parametrize("nj,ni", [(8192, 1), (8192, 48), (8192, 64), (100, 40)])
+def test_in_kernel_prediction_matches_the_predictor_path(g6, nj, ni):
+    """A one-block call of the LDS kernel predicts its j inside its own
+    launch, into LDS only.  The same call after a separate prediction (the
+    predicted set read back from memory) gives the same bits, nearest
+    neighbours and their distances included, with many j-ranges and with one
+    (outputs written by the force kernel itself); the predicted set read
+    afterwards is the predictor's."""
+    eps = 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(nj, 11, eps)
+    ti = 1.0 / 512
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.random.default_rng(ni).choice(nj, ni, replace=False)
+    n2 = min(nj, 128)  # two i-blocks of 64: the separate predictor runs
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    assert g6.lib.pg6_dev_set_kernel(0) == 0 and g6.lib.pg6_dev_set_launch(64, 1, 64) == 0
+    try:
+        p = g6.plan(ni, nj)
+        assert p["variant"] == 0 and (p["nsplit"] > 1) == (nj > 100), p
+        ref = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo,
+                          vo.astype(np.float32).astype(np.float64), m, idx)
+        for variant in ("2p", ""):
+            g6.set_ti(ti)
+            a = g6.calc(nj, xo[sel], vo[sel], idx[sel], eps * eps, variant=variant)
+            g6.set_ti(ti)
+            g6.calc(nj, xo[:n2], vo[:n2], idx[:n2], eps * eps, variant=variant)
+            b = g6.calc(nj, xo[sel], vo[sel], idx[sel], eps * eps, variant=variant)
+            for f in ("acc", "jrk", "pot", "nnb", "rnnb"):
+                np.testing.assert_array_equal(a[f], b[f], err_msg=f"{variant} {f}")
+            assert np.percentile(rel_err(a["acc"], ref["acc"]), 99) < 1e-6
+            assert np.percentile(rel_err(a["jrk"], ref["jrk"]), 99) < 1e-5
+            if variant and ni >= 40:
+                assert np.mean(a["nnb"] == ref["nnb"]) > 0.95
+        g6.set_ti(ti)
+        g6.calc(nj, xo[sel], vo[sel], idx[sel], eps * eps)
+        xp, vp, mp = g6.read_predicted(nj)
+        np.testing.assert_array_equal(xp, xo)
+        np.testing.assert_array_equal(vp, vo.astype(np.float32))
+    finally:
+        g6.lib.pg6_dev_set_kernel(-1)
+        g6.lib.pg6_dev_set_launch(0, 0, 0)
+
+
+def test_j_updates_at_the_same_ti_after_an_in_kernel_prediction(g6):
+    """The reference predicts once per g6_set_ti (PG6/libpg6/phantom_g6.c:
+    137-164): j-records written after a force at ti enter only the
+    prediction of the next g6_set_ti, also when that force predicted its j
+    inside its own launch and kept no predicted set."""
+    n, eps = 2048, 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 9, eps)
+    ti = 1.0 / 512
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.arange(0, n, 41)  # 50 i: one block of the LDS kernel
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    assert g6.lib.pg6_dev_set_kernel(0) == 0
+    try:
+        assert g6.plan(len(sel), n)["variant"] == 0 and g6.plan(len(sel), n)["bi"] >= len(sel)
+        g6.set_ti(ti)
+        r1 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps)
+        ref = O.pg6_force(xo[sel], vo[sel], idx[sel], eps * eps, xo,
+                          vo.astype(np.float32).astype(np.float64), m, idx)
+        assert np.percentile(rel_err(r1["acc"], ref["acc"]), 99) < 1e-6
+        g6.set_j_particles(x + 100.0, v, a2, j6, m, tj, idx)  # every j far away
+        r2 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps)  # same ti
+        for f in ("acc", "jrk", "pot", "nnb", "rnnb"):
+            np.testing.assert_array_equal(r2[f], r1[f], err_msg=f)
+        g6.set_ti(ti)  # a new prediction: the moved j enter
+        r3 = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps)
+        a1 = np.linalg.norm(r1["acc"], axis=1)
+        assert np.all(np.linalg.norm(r3["acc"], axis=1) < 1e-2 * a1)
+    finally:
+        g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+        g6.lib.pg6_dev_set_kernel(-1)
