@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--launch", default="",
                     help="fixed PG6 launch 'variant,bi,blocks_per_cu' (no autotuning), e.g. 1,256,16")
     ap.add_argument("--no-sweep", action="store_true", help="skip the i-batch size sweep")
+    ap.add_argument("--sweep-sizes", default="",
+                    help="comma list of i-batch sizes for the sweep (default 64, 128, ..., 65536)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -253,7 +255,9 @@ def main():
     # i-batch sweep (block-step sizes) against all j
     sweep = []
     rng = np.random.default_rng(SEED + 1)
-    for ni in ([] if args.no_sweep else [64 << k for k in range(11)]):
+    sizes = ([int(t) for t in args.sweep_sizes.split(",") if t] if args.sweep_sizes
+             else [64 << k for k in range(11)])
+    for ni in ([] if args.no_sweep else sizes):
         sel = np.sort(rng.choice(n, ni, replace=False))
         ip = d_xi[torch.from_numpy(sel).to(dev)].contiguous()
         iv = d_vi[torch.from_numpy(sel).to(dev)].contiguous()
