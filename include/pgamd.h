@@ -190,7 +190,11 @@ int pg5t_pm_force(int n, const double *pos, const double *mass, int mesh, double
 int pg6_dev_set_j(int nj, const double *pos, const double *vel, const double *acc2,
                   const double *jrk6, const double *mass, const double *tj, const int *idx);
 /* Work of g6_set_ti + g6calc_firsthalf (phantom_g6.c:138,153): predict
- * j-particles 0..nj-1 to ti.
+ * j-particles 0..nj-1 to ti.  The prediction runs with the first force that
+ * needs it; a force on one i-block predicts inside its own launch without
+ * storing the predicted set (pg6_dev_read_predicted, the neighbour-list
+ * calls and j-records written before the next ti store it first), so the
+ * results are those of a stored prediction, bit for bit.
  * Ordering: every call that reads or writes the board's j-memory or its
  * partial sums (pg6_dev_set_j, _predict, _force, the g6calc_* calls) runs on
  * the device after the previous such call, whichever streams they were
