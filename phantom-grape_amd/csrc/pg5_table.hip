@@ -502,7 +502,7 @@ void launch_batch(int mode, int ngroups, const int* order, const int* ioff, cons
                        counter);
   } else {
     const bool quad = g_tab.order == 2;
-    const size_t lds = (quad ? sizeof(float4) : sizeof(float2)) * (size_t)g_tab.ntab;
+    const size_t lds = (quad ? sizeof(float2) + sizeof(float) : sizeof(float2)) * (size_t)g_tab.ntab;
 #define PGB_TABLE_LAUNCH(B, Q)                                                                 \
   launch_table_batch<B, Q>(blocks, lds, st, ngroups, order, ioff, icnt, ilist, joff, jcnt, jlist, \
                            pos, og, tc, ascale, acc, counter)

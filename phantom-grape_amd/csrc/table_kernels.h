@@ -32,15 +32,21 @@ __device__ inline float table_f(const float2* __restrict__ tab, float s, float s
   return fmaf(frac, t.y, t.x);
 }
 
-// The same index with the quadratic entries {c0, c1, c2, 0}.
-__device__ inline float table_f_quad(const float4* __restrict__ tab, float s, float scut,
+// The same index with the quadratic entries {c0, c1} (q01) and c2 (q2),
+// kept apart in LDS: a gather of 8 + 4 bytes per lane instead of one of 16
+// for {c0, c1, c2, 0} (the quadratic kernel's time is spent in its LDS
+// gathers).
+__device__ inline float table_f_quad(const float2* __restrict__ q01,
+                                     const float* __restrict__ q2, float s, float scut,
                                      int shift, int base) {
   const float sc = fminf(s, scut);
   const unsigned bits = __float_as_uint(sc);
   const unsigned hi = bits >> shift;
-  const float4 t = tab[(int)hi - base];
+  const int k = (int)hi - base;
+  const float2 t = q01[k];
+  const float c2 = q2[k];
   const float d = sc - __uint_as_float(hi << shift);
-  return fmaf(d, fmaf(d, t.z, t.y), t.x);
+  return fmaf(d, fmaf(d, c2, t.y), t.x);
 }
 
 // Block: 256 threads, BI i-particles, G = 256/BI groups share each j-tile.

@@ -221,17 +221,22 @@ __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
     const int* __restrict__ jcnt, const int* __restrict__ jlist,
     const float4* __restrict__ pos, Origin og, TableCut tc, double ascale,
     double* __restrict__ acc, int* counter) {
+  // linear: {value, slope} per entry; quadratic: {c0, c1} per entry, then c2
   extern __shared__ float4 s_tab4[];
   float2* s_tab = reinterpret_cast<float2*>(s_tab4);
+  float* s_c2 = reinterpret_cast<float*>(s_tab + tc.ntab);
   __shared__ float4 s_j[BLOCK];
   __shared__ double s_red[3][BLOCK];
   __shared__ int s_item;
   const int tid = threadIdx.x;
   for (int k = tid; k < tc.ntab; k += BLOCK) {
-    if (QUAD)
-      s_tab4[k] = tc.gquad[k];
-    else
+    if (QUAD) {
+      const float4 q = tc.gquad[k];
+      s_tab[k] = make_float2(q.x, q.y);
+      s_c2[k] = q.z;
+    } else {
       s_tab[k] = tc.gtab[k];
+    }
   }
   const int shift = 23 - tc.frc_bit;
   const int base = 1 << (30 - shift);
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) void treepm_batch_table(
             const float dy = q.y - ip.y;
             const float dz = q.z - ip.z;
             const float s = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 2.0f)));
-            const float f = QUAD ? pgt::table_f_quad(s_tab4, s, scut, shift, base)
+            const float f = QUAD ? pgt::table_f_quad(s_tab, s_c2, s, scut, shift, base)
                                  : pgt::table_f(s_tab, s, scut, shift, base);
             fax = fmaf(f, q.w * dx, fax);
             fay = fmaf(f, q.w * dy, fay);
