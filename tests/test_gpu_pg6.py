@@ -582,3 +582,32 @@ def test_j_updates_at_the_same_ti_after_an_in_kernel_prediction(g6):
     finally:
         g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
         g6.lib.pg6_dev_set_kernel(-1)
+
+
+def test_neighbour_list_after_an_in_kernel_prediction(g6):
+    """A one-block lasthalf with neighbour lists as the first call after
+    g6_set_ti: the force predicts its j inside its own launch, and the list
+    kernel, which reads the predicted set, finds it written out first."""
+    n, eps = 4096, 1.0 / 256
+    m, x, v, a2, j6, tj, idx = plummer_state(n, 13, eps)
+    ti = 1.0 / 512
+    xo, vo = O.pg6_predict(ti, x, v, a2, j6, tj)
+    sel = np.arange(5, n, 97)[:40]
+    h2 = np.full(len(sel), 0.02)
+    g6.set_j_bulk(x, v, a2, j6, m, tj, idx)
+    assert g6.lib.pg6_dev_set_kernel(0) == 0
+    try:
+        g6.set_ti(ti)
+        r = g6.calc(n, xo[sel], vo[sel], idx[sel], eps * eps, h2=h2, variant="n")
+        assert r["rc"] == 0 and g6.lib.g6_read_neighbour_list(0) == 0
+        for q, i in enumerate(sel):
+            rc, got = g6.neighbour_list(q)
+            assert rc == 0
+            d2 = ((xo - xo[i]) ** 2).sum(1) + eps * eps
+            d2[i] = np.inf
+            sure = set(np.nonzero(d2 < 0.999 * h2[q])[0])
+            maybe = set(np.nonzero(d2 <= 1.001 * h2[q])[0])
+            assert sure <= set(got.tolist()) <= maybe, q
+            assert np.all(np.diff(got) > 0)  # j order
+    finally:
+        g6.lib.pg6_dev_set_kernel(-1)
