@@ -9,7 +9,7 @@ One step = one Hermite predictor pass over all j plus one force + jerk +
 potential + nearest-neighbour evaluation of all N i-particles against all N
 j-particles (N (N-1) interactions), inputs resident in HBM.
 
-Prints one JSON line.  Fields beyond the driver's contract:
+Prints one JSON line.  Fields beyond the metric, value and configuration:
   roofline      dominant kernel (pg6_force) against the FP32 vector peak,
                 kernel time from HIP events recorded around each launch on
                 its stream; 60 flops per interaction (PG6/banana/banana01.h:9)

@@ -533,7 +533,7 @@ def test_clusters_match_oracle_and_graph():
 
 
 def test_quadratic_table_with_group_origins(tab, treepm_case):
-    """The table configuration at the north star's accuracy: a quadratic
+    """The table configuration at a force error of 1e-5: a quadratic
     through each interval of the 1024-entry table (order 2) and FP64 group
     origins keep the per-particle error against the closed-form S2 below
     1e-5 at the 99th percentile; the linear table of the same size does not."""

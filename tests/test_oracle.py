@@ -414,7 +414,7 @@ def test_cpu_treepm_driver_matches_per_group_calls():
         ii = L["ilist"][L["ioff"][g]:L["ioff"][g] + L["icnt"][g]]
         jj = L["jlist"][L["joff"][g]:L["joff"][g] + L["jcnt"][g]]
         xj, mj = P[jj], mm[jj]
-        if len(jj) % 2:  # the driver pads odd lists with one massless j
+        if len(jj) % 2:  # ref_treepm pads odd lists with one massless j
             xj, mj = np.vstack([xj, np.zeros((1, 3))]), np.append(mj, 0.0)
         a = R.force(P[ii], xj, mj, devid=0)
         np.testing.assert_array_equal(acc[ii], a)
