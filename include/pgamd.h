@@ -55,7 +55,9 @@ int pg5_set_rsqrt_bias_mode(int mode);
 /* b of the device's rsqrt by that rule (measured on the first call). */
 double pg5_rsqrt_bias(void);
 /* bi in {0 = automatic, 64, 128, 256} i-particles per block; blocks_per_cu and
- * min_jper (> 0) set the j-range split. */
+ * min_jper (> 0) set the j-range split (defaults: particles 64 i per block up
+ * to 32768 i and 256 beyond, 32 blocks per CU, 256 j per range at least;
+ * cells 64 / 128 / 256 i by ni, 8 blocks per CU, 128 j). */
 int pg5_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 /* Particle force kernel: 0 = j-tiles staged in LDS (default), 1 = j
  * broadcast through scalar registers.  Same pair arithmetic; speed only. */

@@ -97,10 +97,10 @@ bool g_bias_ok = false;
 double g_bias = 0.0;
 std::mutex g_bias_mutex;
 
-// launch tuning (autotuner overrides)
+// launch tuning (pg5_dev_set_launch; 0 = the default of plan())
 int g_force_bi = 0;
-int g_blocks_per_cu = 8;
-int g_min_jper = 128;
+int g_blocks_per_cu = 0;
+int g_min_jper = 0;
 
 int npipes() {
   if (g_npipes == 0) {
@@ -189,15 +189,28 @@ struct Plan {
 // scalar registers (pg5_force_s).  Cells always use the LDS kernel.
 int g_variant = 0;
 
+// Default particle shape, measured on MI355X (LDS kernel, Plummer N x N,
+// % of the FP32 peak against the earlier bi 256 / 128 / 64 by ni, 8 blocks
+// per CU, 128 j per range): 64 i per block up to 32768 i, 256 beyond, 32
+// blocks per CU, at least 256 j per range.  N = 4096: 40 % (30 %); 16384:
+// 75 % (69 %); 32768: 81 % (78 %); 65536: 83 % (80 %); 131072: 84 % (81 %).
+// Cells keep the earlier shape.
 Plan plan(int ni, int nj, bool cells = false) {
   Plan p;
-  p.bi = g_force_bi > 0 ? g_force_bi : (ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64));
+  if (g_force_bi > 0)
+    p.bi = g_force_bi;
+  else if (cells)
+    p.bi = ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64);
+  else
+    p.bi = ni > 32768 ? 256 : 64;
+  const int bpc = g_blocks_per_cu > 0 ? g_blocks_per_cu : (cells ? 8 : 32);
+  const int min_jper = g_min_jper > 0 ? g_min_jper : (cells ? 128 : 256);
   p.variant = cells ? 0 : g_variant;
   const int nbi = ceil_div(ni, p.bi);
   // the scalar-broadcast kernel runs bi threads per block, the LDS one 256
-  const int target = device_cu_count() * g_blocks_per_cu * (p.variant == 1 ? 256 / p.bi : 1);
+  const int target = device_cu_count() * bpc * (p.variant == 1 ? 256 / p.bi : 1);
   int ns = std::max(1, ceil_div(target, nbi));
-  ns = std::min(ns, std::max(1, nj / std::max(1, g_min_jper)));
+  ns = std::min(ns, std::max(1, nj / min_jper));
   int jper = ceil_div(std::max(nj, 1), ns);
   jper = (jper + 63) / 64 * 64;
   p.nsplit = std::max(1, ceil_div(std::max(nj, 1), jper));
