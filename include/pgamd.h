@@ -115,6 +115,9 @@ int pg5t_dev_force(int devid, int ni, const void *xi, int nj, const void *xj,
                    double *acc, void *stream);
 /* Table value f(s) at n device values s (gravity_kernel_avx2.c:69-107). */
 int pg5t_dev_eval(int n, const float *s, float *out);
+/* As pg5_dev_set_launch (defaults: 64 i per block up to 32768 i and 256
+ * beyond, 16 blocks per CU up to 16384 i and 32 beyond, 256 j per range at
+ * least). */
 int pg5t_dev_set_launch(int bi, int blocks_per_cu, int min_jper);
 void *pg5t_dev_stream(int devid);
 /* Learned cut-off: a 1-16-16-1 tanh FP32 network, trained on the GPU on

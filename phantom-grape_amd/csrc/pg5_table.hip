@@ -100,9 +100,9 @@ int g_batch_bpc[2] = {0, 0};  // persistent blocks per CU per mode (0 = default)
 int g_batch_block = 0;        // table-mode block (= j-tile) size (0 = default 1024)
 Board g_boards[kMaxDev];
 std::mutex g_mutex;  // guards the table and board creation
-int g_force_bi = 0;
-int g_blocks_per_cu = 8;
-int g_min_jper = 128;
+int g_force_bi = 0;       // pg5t_dev_set_launch; 0 = the default of plan()
+int g_blocks_per_cu = 0;
+int g_min_jper = 0;
 
 int npipes() {
   static int v = [] {
@@ -250,13 +250,21 @@ struct Plan {
   int bi, nsplit, jper;
 };
 
+// Default shape, measured on MI355X (clumpy box, all N x N pairs listed, %
+// of the 38-flop FP32 peak against the earlier 256 / 128 / 64 i per block by
+// ni, 8 blocks per CU, 128 j per range): 64 i per block up to 32768 i, 256
+// beyond; 16 blocks per CU up to 16384 i, 32 beyond; at least 256 j per
+// range.  N = 4096: 30 % (19 %); 16384: 75 % (73 %); 32768: 85 % (80 %);
+// 65536: 88 % (82 %).
 Plan plan(int ni, int nj) {
   Plan p;
-  p.bi = g_force_bi > 0 ? g_force_bi : (ni >= 16384 ? 256 : (ni >= 2048 ? 128 : 64));
+  p.bi = g_force_bi > 0 ? g_force_bi : (ni > 32768 ? 256 : 64);
+  const int bpc = g_blocks_per_cu > 0 ? g_blocks_per_cu : (ni > 16384 ? 32 : 16);
+  const int min_jper = g_min_jper > 0 ? g_min_jper : 256;
   const int nbi = ceil_div(ni, p.bi);
-  const int target = device_cu_count() * g_blocks_per_cu;
+  const int target = device_cu_count() * bpc;
   int ns = std::max(1, ceil_div(target, nbi));
-  ns = std::min(ns, std::max(1, nj / std::max(1, g_min_jper)));
+  ns = std::min(ns, std::max(1, nj / min_jper));
   int jper = ceil_div(std::max(nj, 1), ns);
   jper = (jper + 63) / 64 * 64;
   p.nsplit = std::max(1, ceil_div(std::max(nj, 1), jper));
