@@ -288,7 +288,7 @@ int pg6_cluster_autotune(int nitems, const int *items, const int *coff, const do
                          const double *vel, const double *mass, const float *eps2, double *acc,
                          double *jrk, double *pot, int reps, int *best, float *ms);
 /* Launch shape of the cluster batch: tile in {0 = default 256, 128, 256,
- * 512} j-particles per LDS tile, blocks_per_cu (0 = default 4); a negative
+ * 512} j-particles per LDS tile, blocks_per_cu (0 = default 8); a negative
  * value keeps that part.  pg6_cluster_launch_shape: out2 = {tile,
  * blocks_per_cu} in use. */
 int pg6_cluster_set_launch(int tile, int blocks_per_cu);
