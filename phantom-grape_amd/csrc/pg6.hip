@@ -492,6 +492,8 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
                     const Partials& part, const Outputs& out, hipStream_t st, bool pred_in,
                     double ti, const JState& js) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
+  // LDS kernel with one i-block: 8 wavefronts for 64 i, else 4
+  constexpr int kNT1 = BI == 64 ? pg6::kOneBlockThreads : pg6::kThreads;
   g_timer.before(st);
   if (p.variant == 5)  // 64 i per block, jper a multiple of BI sorted positions
     hipLaunchKernelGGL((pg6::pg6_force_ow<NN, BI / 64>), dim3(ceil_div(ni, 64), p.nsplit),
@@ -506,9 +508,12 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
   else if (p.variant == 1)
     hipLaunchKernelGGL((pg6::pg6_force_s<NN>), grid, dim3(BI), 0, st, ni, ipos, ivel, iid, eps2,
                        nj, p.jper, jp, part, out, p.nsplit);
-  else if (pred_in)
-    hipLaunchKernelGGL((pg6::pg6_force<BI, NN, true>), grid, dim3(pg6::kThreads), 0, st, ni,
-                       ipos, ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);
+  else if (grid.x == 1 && pred_in)  // (pred_in implies one i-block)
+    hipLaunchKernelGGL((pg6::pg6_force<BI, NN, true, kNT1>), grid, dim3(kNT1), 0, st, ni, ipos,
+                       ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);
+  else if (grid.x == 1)
+    hipLaunchKernelGGL((pg6::pg6_force<BI, NN, false, kNT1>), grid, dim3(kNT1), 0, st, ni, ipos,
+                       ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);
   else
     hipLaunchKernelGGL((pg6::pg6_force<BI, NN, false>), grid, dim3(pg6::kThreads), 0, st, ni,
                        ipos, ivel, iid, eps2, nj, p.jper, jp, part, out, p.nsplit, ti, js);

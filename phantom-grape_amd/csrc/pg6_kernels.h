@@ -318,9 +318,10 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 
 // Force + jerk + potential (+ nearest neighbour) kernel.
 //
-// Block = 256 threads.  BI i-particles per block (64, 128 or 256); the block's
-// G = 256/BI wave groups split each LDS tile's j-particles between them
-// (group g takes j = g, g+G, ...), and are combined in fixed order at the end.
+// Block = NT threads (256; kOneBlockThreads for a launch with one i-block of
+// 64).  BI i-particles per block (64, 128 or 256); the block's G = NT/BI wave
+// groups split each LDS tile's j-particles between them (group g takes
+// j = g, g+G, ...), and are combined in fixed order at the end.
 // blockIdx.y selects the j-range [y*jper, min(nj,(y+1)*jper)).
 // With nsplit == 1 the kernel writes final outputs, otherwise partials.
 // PRED: the block predicts the j of each tile itself (the arithmetic of
@@ -330,14 +331,20 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 // predictor launch and the predicted set's round trip through memory go
 // away.  The nearest neighbour's id and distance then come from the tile in
 // LDS (Partials::nid / rdist, or the outputs directly).
-static_assert(kThreads == kJTile, "one thread stages one j of a tile");
+// A launch with one i-block of 64 runs blocks of kOneBlockThreads = 8
+// wavefronts (two per SIMD at one block per CU): 64 i against 65536 j in 256
+// j-ranges, 18.9 us per predict + force call against 19.8 with 4 wavefronts
+// (no change at 16384 j); it is used with and without PRED alike, so the two
+// sum in the same order.
+constexpr int kOneBlockThreads = 512;
 
-template <int BI, bool NN, bool PRED>
-__global__ __launch_bounds__(kThreads) void pg6_force(
+template <int BI, bool NN, bool PRED, int NT = kThreads>
+__global__ __launch_bounds__(NT) void pg6_force(
     int ni, const double* __restrict__ ipos, const double* __restrict__ ivel,
     const int* __restrict__ iid, float eps2, int nj, int jper, JPred jp,
     Partials part, Outputs out, int nsplit, double ti, JState js) {
-  constexpr int G = kThreads / BI;
+  static_assert(NT >= kJTile && NT % BI == 0, "one thread stages at most one j of a tile");
+  constexpr int G = NT / BI;
   __shared__ double s_x[kJTile], s_y[kJTile], s_z[kJTile];
   __shared__ float4 s_vm[kJTile];
   __shared__ int s_id[kJTile];
@@ -458,11 +465,11 @@ __global__ __launch_bounds__(kThreads) void pg6_force(
 
   // Combine the G wave groups in fixed order g = 0..G-1.
   if (G > 1) {
-    __shared__ double r_d[7][kThreads];
-    __shared__ float r_f[kThreads];
-    __shared__ int r_n[kThreads];
-    __shared__ int r_id[PRED ? kThreads : 1];
-    __shared__ float r_dist[PRED ? kThreads : 1];
+    __shared__ double r_d[7][NT];
+    __shared__ float r_f[NT];
+    __shared__ int r_n[NT];
+    __shared__ int r_id[PRED ? NT : 1];
+    __shared__ float r_dist[PRED ? NT : 1];
     __syncthreads();
     r_d[0][tid] = ax; r_d[1][tid] = ay; r_d[2][tid] = az; r_d[3][tid] = pot;
     r_d[4][tid] = jx; r_d[5][tid] = jy; r_d[6][tid] = jz;
