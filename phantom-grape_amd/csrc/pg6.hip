@@ -492,8 +492,8 @@ void launch_force_t(const LaunchPlan& p, int ni, const double* ipos, const doubl
                     const Partials& part, const Outputs& out, hipStream_t st, bool pred_in,
                     double ti, const JState& js) {
   dim3 grid(ceil_div(ni, BI), p.nsplit);
-  // LDS kernel with one i-block: 8 wavefronts for 64 i, else 4
-  constexpr int kNT1 = BI == 64 ? pg6::kOneBlockThreads : pg6::kThreads;
+  // LDS kernel with one i-block: 8 wavefronts for up to 128 i, else 4
+  constexpr int kNT1 = BI <= 128 ? pg6::kOneBlockThreads : pg6::kThreads;
   g_timer.before(st);
   if (p.variant == 5)  // 64 i per block, jper a multiple of BI sorted positions
     hipLaunchKernelGGL((pg6::pg6_force_ow<NN, BI / 64>), dim3(ceil_div(ni, 64), p.nsplit),

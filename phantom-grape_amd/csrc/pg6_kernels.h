@@ -319,7 +319,7 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 // Force + jerk + potential (+ nearest neighbour) kernel.
 //
 // Block = NT threads (256; kOneBlockThreads for a launch with one i-block of
-// 64).  BI i-particles per block (64, 128 or 256); the block's G = NT/BI wave
+// 64 or 128).  BI i-particles per block (64, 128 or 256); the block's G = NT/BI wave
 // groups split each LDS tile's j-particles between them (group g takes
 // j = g, g+G, ...), and are combined in fixed order at the end.
 // blockIdx.y selects the j-range [y*jper, min(nj,(y+1)*jper)).
@@ -331,11 +331,11 @@ __device__ inline float nn_distance(const JPred& jp, int j, double xi, double yi
 // predictor launch and the predicted set's round trip through memory go
 // away.  The nearest neighbour's id and distance then come from the tile in
 // LDS (Partials::nid / rdist, or the outputs directly).
-// A launch with one i-block of 64 runs blocks of kOneBlockThreads = 8
-// wavefronts (two per SIMD at one block per CU): 64 i against 65536 j in 256
-// j-ranges, 18.9 us per predict + force call against 19.8 with 4 wavefronts
-// (no change at 16384 j); it is used with and without PRED alike, so the two
-// sum in the same order.
+// A launch with one i-block of 64 or 128 runs blocks of kOneBlockThreads = 8
+// wavefronts (two per SIMD at one block per CU): against 65536 j in 256
+// j-ranges, 64 i take 18.9 us per predict + force call against 19.8 with 4
+// wavefronts, 128 i 24.4 against 27.3 (no change at 16384 j); it is used with
+// and without PRED alike, so the two sum in the same order.
 constexpr int kOneBlockThreads = 512;
 
 template <int BI, bool NN, bool PRED, int NT = kThreads>
